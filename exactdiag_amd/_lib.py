@@ -1,0 +1,85 @@
+"""ctypes binding of ``libexactdiag_hip.so`` (the C ABI in include/exactdiag_hip.h).
+
+The library is the only compute path: importing a function that needs it
+raises when the library has not been built or cannot be loaded, there is no
+CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import pathlib
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libexactdiag_hip.so"
+
+ED_OK, ED_ERR_ARG, ED_ERR_HIP, ED_ERR_STATE, ED_ERR_CAPACITY = 0, -1, -2, -3, -4
+
+(OP_SZQ, OP_NQ, OP_APLUS_DOWN, OP_APLUS_UP, OP_AMINUS_DOWN, OP_AMINUS_UP, OP_CURRENT_RUNG,
+ OP_CURRENT_LEG) = range(8)
+
+EXPORTED = [
+    "ed_last_error", "ed_version", "ed_sector_create", "ed_sector_destroy", "ed_basis_build", "ed_basis_size",
+    "ed_basis_export", "ed_lookup", "ed_lowest", "ed_term_columns", "ed_op_columns", "ed_hv", "ed_op_apply",
+    "ed_lanczos_run", "ed_cf_eval", "ed_vec_dots", "ed_vec_axpys", "ed_vec_combine", "ed_vec_norm2", "ed_vec_scale",
+]
+
+_vp = ct.c_void_p
+_i64 = ct.c_int64
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int)
+
+_SIGNATURES = {
+    "ed_last_error": (ct.c_char_p, []),
+    "ed_version": (ct.c_int, []),
+    "ed_sector_create": (ct.c_int, [ct.c_int] * 5 + [ct.POINTER(_vp)]),
+    "ed_sector_destroy": (ct.c_int, [_vp]),
+    "ed_basis_build": (ct.c_int, [_vp, _vp]),
+    "ed_basis_size": (_i64, [_vp]),
+    "ed_basis_export": (ct.c_int, [_vp, _vp, _vp, _vp]),
+    "ed_lookup": (ct.c_int, [_vp, _vp, _vp, _i64, _vp]),
+    "ed_lowest": (ct.c_int, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "ed_term_columns": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp]),
+    "ed_op_columns": (ct.c_int, [_vp, _vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp]),
+    "ed_hv": (ct.c_int, [_vp, _dp, _vp, _vp, _vp, ct.c_int, _vp]),
+    "ed_op_apply": (ct.c_int, [_vp, _vp, ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _vp]),
+    "ed_lanczos_run": (ct.c_int, [_vp, _dp, _vp, ct.c_int, _dp, _dp, _ip, _vp, ct.c_int, _vp]),
+    "ed_cf_eval": (ct.c_int, [_dp, _dp, ct.c_int, _dp, ct.c_int, ct.c_double, ct.c_double, ct.c_double, _dp]),
+    "ed_vec_dots": (ct.c_int, [_vp, ct.c_int, _i64, _vp, _i64, _dp, _vp]),
+    "ed_vec_axpys": (ct.c_int, [_vp, ct.c_int, _i64, _dp, _vp, _i64, _vp]),
+    "ed_vec_combine": (ct.c_int, [_vp, ct.c_int, _i64, _dp, _vp, _i64, _vp]),
+    "ed_vec_norm2": (ct.c_int, [_vp, _i64, _dp, _vp]),
+    "ed_vec_scale": (ct.c_int, [_vp, _i64, ct.c_double, ct.c_double, _vp]),
+}
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library once (raises LibraryMissing when it is absent)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise LibraryMissing(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        handle = ct.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int) -> None:
+    """Turn a status code into the exception the reference raises for it."""
+    if rc == ED_OK:
+        return
+    msg = lib().ed_last_error().decode(errors="replace")
+    if rc == ED_ERR_ARG:
+        raise ValueError(msg)
+    if rc == ED_ERR_CAPACITY:
+        raise IndexError(msg)
+    raise RuntimeError(f"exactdiag_hip status {rc}: {msg}")

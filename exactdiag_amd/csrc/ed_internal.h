@@ -1,0 +1,120 @@
+// Host-side sector object and error plumbing shared by the library's
+// translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ed_common.h"
+#include "exactdiag_hip.h"
+
+namespace ed {
+
+void set_error(const std::string& msg);
+
+struct Fail {
+    int status;
+    std::string msg;
+};
+
+#define ED_HIP(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::ed::Fail{ED_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};            \
+    } while (0)
+
+#define ED_REQUIRE(cond, status, msg)              \
+    do {                                           \
+        if (!(cond)) throw ::ed::Fail{status, msg}; \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return ED_OK;
+    } catch (const Fail& e) {
+        set_error(e.msg);
+        return e.status;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ED_ERR_ARG;
+    }
+}
+
+template <class T>
+T* dev_alloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    ED_HIP(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <class T>
+T* dev_upload(const std::vector<T>& v, hipStream_t st) {
+    T* p = dev_alloc<T>(v.size());
+    if (!v.empty()) ED_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return p;
+}
+
+inline void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Combination list of one rule: reversed site pairs (or single sites) with
+// (diagonal, off-diagonal) weights, as combinations_from_start_end_points
+// returns them (general/matrix_rules_utils.pyx:15-87).
+struct RuleHost {
+    int arity = 2;
+    std::vector<int8_t> idx;     // n_combo * 2
+    std::vector<double> w;       // n_combo * 4: diag re, im, off re, im
+    int valid = 0, make = 0, spin_proj = 0, density = 0, fermion = 0;
+    int max_per_col = 1;
+    int n_combo() const { return int(idx.size() / 2); }
+};
+
+enum RuleValid { RV_HOP = 0, RV_SWAP = 1, RV_NO_HOLES = 2, RV_ALL_UP = 3, RV_ALL_DOWN = 4, RV_ALL_HOLES = 5 };
+enum RuleMake { RM_SWAP = 0, RM_SAME = 1, RM_ADD_DOWN = 2, RM_ADD_UP = 3, RM_ADD_HOLE = 4 };
+
+RuleHost hamiltonian_rule(int L, int term);
+RuleHost operator_rule(int L, int op, int qx, int qy);
+
+}  // namespace ed
+
+struct ed_sector {
+    int L = 0, N = 0, h = 0, ne = 0, nd = 0, kx = 0, ky = 0;
+    int64_t n_spin = 0, n_hole = 0;
+    std::vector<uint32_t> comb;        // COMB_W * COMB_W
+    std::vector<double> amp;           // see SectorView::amp
+    std::vector<double> sqrt_tab;      // N + 1
+    std::vector<uint32_t> hole_by_rank;
+    std::vector<uint32_t> canon_gmask;
+    std::vector<int32_t> canon_block, block_of_rank;
+    std::vector<int64_t> block_rank, block_base, block_table;
+    std::vector<uint32_t> block_hole;
+    std::vector<uint8_t> block_sym;
+    int64_t D = -1;
+    int64_t table_size = 0;
+
+    uint32_t* d_comb = nullptr;
+    double* d_amp = nullptr;
+    double* d_sqrt = nullptr;
+    uint32_t* d_canon_gmask = nullptr;
+    int32_t* d_canon_block = nullptr;
+    int32_t* d_block_of_rank = nullptr;
+    int64_t* d_block_base = nullptr;
+    int64_t* d_block_table = nullptr;
+    uint32_t* d_block_hole = nullptr;
+    int32_t* d_table = nullptr;
+    uint64_t* d_key = nullptr;
+    uint8_t* d_orbit = nullptr;
+
+    ed::SectorView host_view() const;  // host pointers (tables only)
+    ed::SectorView view() const;       // device pointers
+    void release();
+};

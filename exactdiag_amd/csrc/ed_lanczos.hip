@@ -1,0 +1,392 @@
+// Lanczos recurrence, continued fraction and dense vector helpers for gfx950.
+//
+// The recurrence keeps its coefficients on the device: per step one fused
+// H·v (with the workgroup partials of <u, Hu>), a one-workgroup reduction
+// that writes a_n, one fused update w = s_n Hu - a_n s_n u_n - b_n s_{n-1}
+// u_{n-1} (with partials of ||w||^2) and a reduction that writes b_{n+1} and
+// s_{n+1} = 1 / b_{n+1}.  Vectors are never rescaled in place: u_n is the
+// unnormalised Lanczos vector and s_n its scale, which saves a full pass per
+// step.  No host synchronisation inside the loop except a stop check every
+// 64 steps.  All reductions are fixed-order, so runs are reproducible.
+
+#include <cmath>
+#include <vector>
+
+#include "ed_internal.h"
+#include "ed_matrix.h"
+
+namespace ed {
+namespace {
+
+constexpr int BS = 256;
+
+struct LzDev {
+    double* a;  // [nsteps]
+    double* b;  // [nsteps + 1]
+    double* s;  // [nsteps + 1]
+    int* flag;  // [0] stopped, [1] count
+};
+
+template <class VT>
+__device__ inline void ldc(const VT* p, int64_t i, double& re, double& im) {
+    const VT v = p[i];
+    re = double(v.x);
+    im = double(v.y);
+}
+
+template <class VT>
+__device__ inline void stc(VT* p, int64_t i, double re, double im) {
+    VT o;
+    o.x = decltype(o.x)(re);
+    o.y = decltype(o.y)(im);
+    p[i] = o;
+}
+
+__device__ inline void block_sum2(double& re, double& im) {
+    __shared__ double red[2 * BS];
+    red[2 * threadIdx.x] = re;
+    red[2 * threadIdx.x + 1] = im;
+    __syncthreads();
+    for (int st = BS / 2; st > 0; st >>= 1) {
+        if (int(threadIdx.x) < st) {
+            red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
+            red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
+        }
+        __syncthreads();
+    }
+    re = red[0];
+    im = red[1];
+}
+
+template <class VT>
+__global__ void __launch_bounds__(BS) copy_in_kernel(const double* src, VT* dst, int64_t D, double* partial) {
+    const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
+    double n2 = 0.0;
+    if (i < D) {
+        const double re = src[2 * i], im = src[2 * i + 1];
+        stc(dst, i, re, im);
+        double r2, i2;
+        ldc(dst, i, r2, i2);
+        n2 = r2 * r2 + i2 * i2;
+    }
+    double z = 0.0;
+    block_sum2(n2, z);
+    if (threadIdx.x == 0) partial[2 * blockIdx.x] = n2, partial[2 * blockIdx.x + 1] = 0.0;
+}
+
+__global__ void lz_init_kernel(LzDev lz, const double* partial, int64_t n) {
+    double re = 0.0, im = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += BS) re += partial[2 * k];
+    block_sum2(re, im);
+    if (threadIdx.x == 0) {
+        lz.b[0] = 0.0;
+        lz.s[0] = 1.0 / std::sqrt(re);
+        lz.flag[0] = re > 0.0 ? 0 : 1;
+        lz.flag[1] = 0;
+    }
+}
+
+__global__ void lz_reduce_a_kernel(LzDev lz, const double* partial, int64_t n, int step) {
+    if (lz.flag[0]) return;
+    double re = 0.0, im = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += BS) {
+        re += partial[2 * k];
+        im += partial[2 * k + 1];
+    }
+    block_sum2(re, im);
+    if (threadIdx.x == 0) {
+        const double sn = lz.s[step];
+        lz.a[step] = sn * sn * re;
+    }
+}
+
+template <class VT>
+__global__ void __launch_bounds__(BS) lz_update_kernel(LzDev lz, int step, int64_t D, const VT* __restrict__ y,
+                                                       const VT* __restrict__ u, VT* __restrict__ up,
+                                                       VT* __restrict__ basis, double* partial) {
+    if (lz.flag[0]) return;
+    const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
+    const double sn = lz.s[step], an = lz.a[step], bn = lz.b[step];
+    const double snm = step > 0 ? lz.s[step - 1] : 0.0;
+    double n2 = 0.0;
+    if (i < D) {
+        double yr, yi, ur, ui, pr, pi;
+        ldc(y, i, yr, yi);
+        ldc(u, i, ur, ui);
+        ldc(up, i, pr, pi);
+        const double vr = sn * ur, vi = sn * ui;
+        const double hr = sn * yr, hi = sn * yi;
+        const double qr = snm * pr, qi = snm * pi;
+        const double wr = hr - an * vr - bn * qr;
+        const double wi = hi - an * vi - bn * qi;
+        stc(up, i, wr, wi);
+        if (basis) stc(basis, i, vr, vi);
+        double r2, i2;
+        ldc(up, i, r2, i2);
+        n2 = r2 * r2 + i2 * i2;
+    }
+    double z = 0.0;
+    block_sum2(n2, z);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = n2;
+        partial[2 * blockIdx.x + 1] = 0.0;
+    }
+}
+
+__global__ void lz_reduce_b_kernel(LzDev lz, const double* partial, int64_t n, int step) {
+    if (lz.flag[0]) return;
+    double re = 0.0, im = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += BS) re += partial[2 * k];
+    block_sum2(re, im);
+    if (threadIdx.x == 0) {
+        const double bnp = std::sqrt(re);
+        lz.b[step + 1] = bnp;
+        lz.s[step + 1] = 1.0 / bnp;
+        lz.flag[1] = step + 1;
+        if (bnp < 1e-8) lz.flag[0] = 1;
+    }
+}
+
+// general/cython_lanczos_diagonalization.pyx:42-48 per omega.
+__global__ void cf_kernel(const double* a, const double* b, int n, const double* ws, int nw, double e0, double eta,
+                          double factor, double* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nw) return;
+    const double zr = ws[k] + e0, zi = eta;
+    double dr = zr - a[n - 1], di = zi;
+    for (int i = 1; i < n; ++i) {
+        const double bb = b[n - i] * b[n - i];
+        const double den = dr * dr + di * di;
+        const double qr = bb * dr / den, qi = -bb * di / den;  // bb / det
+        dr = zr - a[n - 1 - i] - qr;
+        di = zi - qi;
+    }
+    out[k] = factor * (-di / (dr * dr + di * di));
+}
+
+__global__ void __launch_bounds__(BS) dots_kernel(const double* V, int64_t ld, const double* w, int64_t D,
+                                                  double* partial, int nwg) {
+    const int k = blockIdx.y;
+    double re = 0.0, im = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x; i < D; i += int64_t(nwg) * BS) {
+        const double vr = V[2 * (k * ld + i)], vi = V[2 * (k * ld + i) + 1];
+        const double xr = w[2 * i], xi = w[2 * i + 1];
+        re += vr * xr + vi * xi;
+        im += vr * xi - vi * xr;
+    }
+    block_sum2(re, im);
+    if (threadIdx.x == 0) {
+        partial[2 * (int64_t(k) * nwg + blockIdx.x)] = re;
+        partial[2 * (int64_t(k) * nwg + blockIdx.x) + 1] = im;
+    }
+}
+
+__global__ void dots_reduce_kernel(const double* partial, int nwg, double* out) {
+    const int k = blockIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int j = threadIdx.x; j < nwg; j += BS) {
+        re += partial[2 * (int64_t(k) * nwg + j)];
+        im += partial[2 * (int64_t(k) * nwg + j) + 1];
+    }
+    block_sum2(re, im);
+    if (threadIdx.x == 0) {
+        out[2 * k] = re;
+        out[2 * k + 1] = im;
+    }
+}
+
+__global__ void combine_kernel(const double* V, int nv, int64_t ld, const double* c, double* out, int64_t D,
+                               int accumulate_negative) {
+    const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
+    if (i >= D) return;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < nv; ++k) {
+        const double vr = V[2 * (k * ld + i)], vi = V[2 * (k * ld + i) + 1];
+        const double cr = c[2 * k], ci = c[2 * k + 1];
+        re += cr * vr - ci * vi;
+        im += cr * vi + ci * vr;
+    }
+    if (accumulate_negative) {
+        out[2 * i] -= re;
+        out[2 * i + 1] -= im;
+    } else {
+        out[2 * i] = re;
+        out[2 * i + 1] = im;
+    }
+}
+
+__global__ void scale_kernel(double* w, int64_t D, double cr, double ci) {
+    const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
+    if (i >= D) return;
+    const double re = w[2 * i], im = w[2 * i + 1];
+    w[2 * i] = cr * re - ci * im;
+    w[2 * i + 1] = cr * im + ci * re;
+}
+
+template <class VT>
+void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
+                   int* n_out, VT* d_basis, int precision, hipStream_t st) {
+    const int64_t D = s->D;
+    const int64_t nwg = (D + BS - 1) / BS;
+    const HvWeights hw = make_weights(w);
+    VT* buf[3] = {dev_alloc<VT>(size_t(D)), dev_alloc<VT>(size_t(D)), dev_alloc<VT>(size_t(D))};
+    double* partial = dev_alloc<double>(size_t(2 * std::max<int64_t>(nwg, hv_partials(s))));
+    LzDev lz{dev_alloc<double>(size_t(nsteps)), dev_alloc<double>(size_t(nsteps + 1)),
+             dev_alloc<double>(size_t(nsteps + 1)), dev_alloc<int>(2)};
+    auto cleanup = [&] {
+        for (auto* p : buf) dev_free(p);
+        dev_free(partial);
+        dev_free(lz.a);
+        dev_free(lz.b);
+        dev_free(lz.s);
+        dev_free(lz.flag);
+    };
+    try {
+        VT* u = buf[0];
+        VT* up = buf[1];
+        VT* y = buf[2];
+        ED_HIP(hipMemsetAsync(up, 0, size_t(D) * sizeof(VT), st));
+        hipLaunchKernelGGL((copy_in_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, d_v0, u, D, partial);
+        hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(BS), 0, st, lz, partial, nwg);
+        ED_HIP(hipGetLastError());
+        int flag[2] = {0, 0};
+        for (int n = 0; n < nsteps; ++n) {
+            launch_hv(s, hw, u, y, partial, precision, 1, st);
+            hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(BS), 0, st, lz, partial, hv_partials(s), n);
+            hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, lz, n, D, y, u, up,
+                               d_basis ? d_basis + int64_t(n) * D : nullptr, partial);
+            hipLaunchKernelGGL(lz_reduce_b_kernel, dim3(1), dim3(BS), 0, st, lz, partial, nwg, n);
+            ED_HIP(hipGetLastError());
+            VT* t = up;  // w (u_{n+1}) now lives in up
+            up = u;
+            u = t;
+            if ((n & 63) == 63) {
+                ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
+                ED_HIP(hipStreamSynchronize(st));
+                if (flag[0]) break;
+            }
+        }
+        ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
+        ED_HIP(hipStreamSynchronize(st));
+        const int cnt = flag[1];
+        if (cnt > 0) {
+            ED_HIP(hipMemcpyAsync(a, lz.a, size_t(cnt) * sizeof(double), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipMemcpyAsync(b, lz.b, size_t(cnt) * sizeof(double), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+        }
+        *n_out = cnt;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+}  // namespace
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" {
+
+int ed_lanczos_run(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
+                   int* n_out, double* d_basis, int precision, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
+        ED_REQUIRE(nsteps >= 1 && a && b && n_out, ED_ERR_ARG, "bad Lanczos arguments");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        *n_out = 0;
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (precision == 0)
+            lanczos_run_t<double2>(s, w, d_v0, nsteps, a, b, n_out, reinterpret_cast<double2*>(d_basis), 0, st);
+        else
+            lanczos_run_t<float2>(s, w, d_v0, nsteps, a, b, n_out, reinterpret_cast<float2*>(d_basis), 1, st);
+    });
+}
+
+int ed_cf_eval(const double* a, const double* b, int n, const double* ws, int n_w, double e0, double eta,
+               double factor, double* out) {
+    return guarded([&] {
+        ED_REQUIRE(n >= 1 && n_w >= 0, ED_ERR_ARG, "bad continued-fraction sizes");
+        if (n_w == 0) return;
+        hipStream_t st = nullptr;
+        std::vector<double> ab(a, a + n), bb(b, b + n), wv(ws, ws + n_w);
+        double* da = dev_upload(ab, st);
+        double* db = dev_upload(bb, st);
+        double* dw = dev_upload(wv, st);
+        double* dout = dev_alloc<double>(size_t(n_w));
+        hipLaunchKernelGGL(cf_kernel, dim3(unsigned((n_w + 127) / 128)), dim3(128), 0, st, da, db, n, dw, n_w, e0, eta,
+                           factor, dout);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(out, dout, size_t(n_w) * sizeof(double), hipMemcpyDeviceToHost);
+        dev_free(da);
+        dev_free(db);
+        dev_free(dw);
+        dev_free(dout);
+        ED_HIP(e);
+    });
+}
+
+int ed_vec_dots(const double* d_V, int nv, int64_t ld, const double* d_w, int64_t D, double* out, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(nv >= 0 && nv <= 65535 && D >= 0, ED_ERR_ARG, "bad sizes");
+        if (nv == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const int nwg = int(std::min<int64_t>(std::max<int64_t>((D + BS - 1) / BS, 1), 1024));
+        double* partial = dev_alloc<double>(size_t(2) * nwg * nv);
+        double* dout = dev_alloc<double>(size_t(2 * nv));
+        hipLaunchKernelGGL(dots_kernel, dim3(unsigned(nwg), unsigned(nv)), dim3(BS), 0, st, d_V, ld, d_w, D, partial,
+                           nwg);
+        hipLaunchKernelGGL(dots_reduce_kernel, dim3(unsigned(nv)), dim3(BS), 0, st, partial, nwg, dout);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, size_t(2 * nv) * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        dev_free(partial);
+        dev_free(dout);
+        ED_HIP(e);
+    });
+}
+
+static int combine_impl(const double* d_V, int nv, int64_t ld, const double* c, double* d_out, int64_t D,
+                        int negative, void* stream) {
+    return guarded([&] {
+        if (D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        std::vector<double> cv(c, c + 2 * nv);
+        if (cv.empty()) cv.assign(2, 0.0);
+        double* dc = dev_upload(cv, st);
+        hipLaunchKernelGGL(combine_kernel, dim3(unsigned((D + BS - 1) / BS)), dim3(BS), 0, st, d_V, nv, ld, dc, d_out,
+                           D, negative);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        dev_free(dc);
+        ED_HIP(e);
+    });
+}
+
+int ed_vec_axpys(const double* d_V, int nv, int64_t ld, const double* c, double* d_w, int64_t D, void* stream) {
+    return combine_impl(d_V, nv, ld, c, d_w, D, 1, stream);
+}
+
+int ed_vec_combine(const double* d_V, int nv, int64_t ld, const double* c, double* d_out, int64_t D, void* stream) {
+    return combine_impl(d_V, nv, ld, c, d_out, D, 0, stream);
+}
+
+int ed_vec_norm2(const double* d_w, int64_t D, double* out, void* stream) {
+    double z[2];
+    int rc = ed_vec_dots(d_w, 1, 0, d_w, D, z, stream);
+    *out = z[0];
+    return rc;
+}
+
+int ed_vec_scale(double* d_w, int64_t D, double cr, double ci, void* stream) {
+    return guarded([&] {
+        if (D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(scale_kernel, dim3(unsigned((D + BS - 1) / BS)), dim3(BS), 0, st, d_w, D, cr, ci);
+        ED_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

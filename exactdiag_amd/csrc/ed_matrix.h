@@ -1,0 +1,19 @@
+// Internal interface between the matrix kernels and the Lanczos driver.
+#pragma once
+
+#include "ed_internal.h"
+
+namespace ed {
+
+struct HvWeights {
+    C2 w[4];     // tl, tr, jl, jr
+    int active;  // bit t set when |w_t| >= 1e-6
+};
+
+HvWeights make_weights(const double* w);
+void launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
+               int remap, hipStream_t st);
+int64_t hv_partials(const ed_sector* s);
+void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st);
+
+}  // namespace ed

@@ -1,0 +1,625 @@
+// Matrix elements, matrix-free H·v and excitation operators for gfx950.
+//
+// Two kernels compute matrix elements:
+//  * column_kernel: one lane per column, the reference's per-column
+//    algorithm step by step (general/column_functions.h:99-207): rules in
+//    combination order, the orbit walk for every final state, duplicate
+//    merging in order, near-zero removal, sort by row.  Bit-exact values;
+//    used for parity, for operator matrices and for small sectors.
+//  * hv_kernel: one lane per row in gather form, y_f = sum_j conj(H_jf) x_j,
+//    which by hermiticity equals row f of the reference's matrix.  Bonds are
+//    found with whole-mask bit operations, representatives through the
+//    hole-pattern table and indices through combinatorial ranking, so no
+//    orbit walk and no search runs in the hot loop.  No atomics; the
+//    per-workgroup partial of <x, y> feeds the Lanczos coefficient a_n.
+
+#include <algorithm>
+#include <vector>
+
+#include "ed_internal.h"
+#include "ed_orbit.h"
+#include "ed_matrix.h"
+
+namespace ed {
+
+namespace {
+
+constexpr int MAXC = 72;  // per-column capacity of the exact kernel
+
+struct RuleView {
+    int n_combo, arity, valid, make, spin_proj, density, fermion, max_per_col;
+    const int8_t* idx;
+    const double* w;
+};
+
+__device__ inline int site_value(uint32_t H, uint32_t Dn, int i) {  // 0 hole, 1 down, 2 up
+    const uint32_t b = 1u << i;
+    return (H & b) ? 0 : ((Dn & b) ? 1 : 2);
+}
+
+__device__ inline bool rule_valid(const RuleView& r, uint32_t H, uint32_t Dn, int a, int b) {
+    switch (r.valid) {
+        case RV_HOP: return site_value(H, Dn, a) == 0 && site_value(H, Dn, b) != 0;
+        case RV_SWAP: {
+            const int va = site_value(H, Dn, a), vb = site_value(H, Dn, b);
+            return va != vb && va != 0 && vb != 0;
+        }
+        case RV_NO_HOLES: return site_value(H, Dn, a) != 0 && (r.arity < 2 || site_value(H, Dn, b) != 0);
+        case RV_ALL_UP: return site_value(H, Dn, a) == 2 && (r.arity < 2 || site_value(H, Dn, b) == 2);
+        case RV_ALL_DOWN: return site_value(H, Dn, a) == 1 && (r.arity < 2 || site_value(H, Dn, b) == 1);
+        case RV_ALL_HOLES: return site_value(H, Dn, a) == 0 && (r.arity < 2 || site_value(H, Dn, b) == 0);
+    }
+    return false;
+}
+
+__device__ inline void set_site(uint32_t& H, uint32_t& Dn, int i, int v) {
+    const uint32_t b = 1u << i;
+    H = v == 0 ? (H | b) : (H & ~b);
+    Dn = v == 1 ? (Dn | b) : (Dn & ~b);
+}
+
+// general/column_functions.h:317-346: operators act from the last index to
+// the first; each counts the electrons above it plus the operators already
+// placed above it.
+__device__ inline int commutation_parity(uint32_t E, int arity, int a, int b) {
+    int count = 0;
+    uint32_t placed = 0;
+    const int idx[2] = {a, b};
+    for (int k = arity - 1; k >= 0; --k) {
+        const int p = idx[k];
+        const uint32_t above = p >= 31 ? 0u : (~0u << (p + 1));
+        count += popc32(E & above) + popc32(placed & above);
+        placed |= 1u << p;
+    }
+    return count & 1;
+}
+
+__device__ inline bool cabs_gt(C2 v, double thr2) { return v.re * v.re + v.im * v.im > thr2; }
+
+__global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView fin, RuleView r, int triangle,
+                                                      int32_t* count, int64_t* rows, double* vals) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ini.D) return;
+    const uint64_t key = ini.key[i];
+    const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
+    const uint32_t E = ~H & ini.full;
+    const int n = ini.orbit[i];
+    const C2 dw0{1.0, 0.0};
+    const C2 ow0{ini.sqrt_tab[n], 0.0};
+    C2 cv[MAXC];
+    int64_t ci[MAXC];
+    int cnt = 0;
+    bool overflow = false;
+    auto add = [&](C2 v, int64_t j) {
+        for (int k = 0; k < cnt; ++k)
+            if (ci[k] == j) {
+                cv[k] = cadd(cv[k], v);
+                return;
+            }
+        if (cnt < MAXC) {
+            cv[cnt] = v;
+            ci[cnt] = j;
+            ++cnt;
+        } else {
+            overflow = true;
+        }
+    };
+    // state-preserving rules reach the same final state from every combination
+    bool same_cached = false;
+    int64_t same_j = -1;
+    C2 same_c{0, 0};
+    for (int c = 0; c < r.n_combo; ++c) {
+        const int a = r.idx[2 * c], b = r.idx[2 * c + 1];
+        C2 dw = cmul(dw0, C2{r.w[4 * c], r.w[4 * c + 1]});
+        C2 ow = cmul(ow0, C2{r.w[4 * c + 2], r.w[4 * c + 3]});
+        if (!rule_valid(r, H, Dn, a, b)) continue;
+        if (r.spin_proj) {  // tJ_spin_half_ladder/matrix_element_functions.pyx:71-82
+            C2 f{1.0, 0.0};
+            if (site_value(H, Dn, a) == 1) f = cmul(f, C2{-1.0, 0.0});
+            if (r.arity > 1 && site_value(H, Dn, b) == 1) f = cmul(f, C2{-1.0, 0.0});
+            dw = cmul(dw, f);
+            ow = cmul(ow, f);
+        }
+        if (cabs_gt(dw, 1e-16)) add(dw, i);
+        if (ow.re * ow.re + ow.im * ow.im < 1e-16) continue;
+        uint32_t H2 = H, D2 = Dn;
+        int64_t j;
+        C2 cc;
+        if (r.make == RM_SAME && same_cached) {
+            j = same_j;
+            cc = same_c;
+        } else {
+            switch (r.make) {
+                case RM_SWAP: {
+                    const int va = site_value(H, Dn, a), vb = site_value(H, Dn, b);
+                    set_site(H2, D2, a, vb);
+                    set_site(H2, D2, b, va);
+                    break;
+                }
+                case RM_SAME: break;
+                case RM_ADD_DOWN:
+                    set_site(H2, D2, a, 1);
+                    if (r.arity > 1) set_site(H2, D2, b, 1);
+                    break;
+                case RM_ADD_UP:
+                    set_site(H2, D2, a, 2);
+                    if (r.arity > 1) set_site(H2, D2, b, 2);
+                    break;
+                case RM_ADD_HOLE:
+                    set_site(H2, D2, a, 0);
+                    if (r.arity > 1) set_site(H2, D2, b, 0);
+                    break;
+            }
+            const OrbitResult o = orbit_walk(fin, H2, D2, false);
+            j = -1;
+            cc = C2{0, 0};
+            if (o.n > 0) {
+                const Found fd = lookup_rep(fin, uint32_t(o.best >> 32), uint32_t(o.best));
+                j = fd.j;
+                cc = lowest_amplitude(fin, H2, o.best_p, o.n);
+            }
+            if (r.make == RM_SAME) {
+                same_cached = true;
+                same_j = j;
+                same_c = cc;
+            }
+        }
+        if (j < 0 || (triangle && j < i)) continue;
+        C2 v = cmul(ow, cc);
+        if (r.fermion && commutation_parity(E, r.arity, a, b)) v = C2{v.re * -1.0, v.im * -1.0};
+        add(v, j);
+    }
+    // drop near-zero entries, sort by row (general/column_functions.h:79-97)
+    int m = 0;
+    for (int k = 0; k < cnt; ++k)
+        if (!(cv[k].re * cv[k].re + cv[k].im * cv[k].im < 1e-16)) {
+            cv[m] = cv[k];
+            ci[m] = ci[k];
+            ++m;
+        }
+    for (int k = 1; k < m; ++k) {
+        const int64_t ki = ci[k];
+        const C2 kv = cv[k];
+        int q = k - 1;
+        while (q >= 0 && ci[q] > ki) {
+            ci[q + 1] = ci[q];
+            cv[q + 1] = cv[q];
+            --q;
+        }
+        ci[q + 1] = ki;
+        cv[q + 1] = kv;
+    }
+    count[i] = overflow ? MAXC + 1 : m;
+    const int keep = m < r.max_per_col ? m : r.max_per_col;
+    for (int k = 0; k < keep; ++k) {
+        rows[i * r.max_per_col + k] = ci[k];
+        vals[2 * (i * r.max_per_col + k)] = cv[k].re;
+        vals[2 * (i * r.max_per_col + k) + 1] = cv[k].im;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free H·v (gather form).
+
+template <class VT>
+__device__ inline C2 load_c(const VT* p, int64_t i) {
+    const VT v = p[i];
+    return {double(v.x), double(v.y)};
+}
+
+template <class VT>
+__device__ inline void store_c(VT* p, int64_t i, C2 v) {
+    VT o;
+    o.x = decltype(o.x)(v.re);
+    o.y = decltype(o.y)(v.im);
+    p[i] = o;
+}
+
+__device__ inline int64_t lookup_in_block(const SectorView& s, int blk, uint32_t Hc, uint32_t Dr, int& n) {
+    const int64_t sr = rank_spins(Hc, Dr, s.comb);
+    const int64_t off = s.block_table[blk];
+    if (off < 0) {
+        n = s.N;
+        return s.block_base[blk] + sr;
+    }
+    const int32_t j = s.table[off + sr];
+    n = j >= 0 ? int(s.orbit[j]) : 0;
+    return j;
+}
+
+// conj(val) * x_j accumulated with weight; val = scale * sign * A[p][n_j].
+template <class VT>
+__device__ inline void accumulate(const SectorView& s, const VT* __restrict__ x, uint32_t Hsrc, int p, int blk,
+                                  uint32_t Hc, uint32_t Dr, double scale, C2 W, C2& acc) {
+    int nj;
+    const int64_t j = lookup_in_block(s, blk, Hc, Dr, nj);
+    if (j < 0) return;
+    const C2 a = amp_at(s, p, nj);
+    const double sg = double(image_sign(Hsrc, p, s.L, s.ne_even, s.full)) * scale;
+    const C2 v{sg * a.re, -(sg * a.im)};  // conj(val)
+    const C2 xj = load_c(x, j);
+    const C2 t = cmul(v, xj);
+    acc = cadd(acc, cmul(W, t));
+}
+
+template <class VT, int BS>
+__global__ void __launch_bounds__(BS) hv_kernel(SectorView s, HvWeights w, const VT* __restrict__ x,
+                                                VT* __restrict__ y, double* __restrict__ partial, int remap) {
+    int bid = blockIdx.x;
+    if (remap) {
+        // contiguous row ranges per XCD: consecutive workgroups go round
+        // robin over the 8 XCDs, so give XCD k the k-th eighth of the rows
+        const int nwg = gridDim.x;
+        const int per = nwg / 8;
+        if (bid < per * 8) bid = (bid & 7) * per + (bid >> 3);
+    }
+    const int64_t f = int64_t(bid) * BS + threadIdx.x;
+    C2 acc{0.0, 0.0}, dot{0.0, 0.0};
+    if (f < s.D) {
+        const uint64_t key = s.key[f];
+        const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
+        const uint32_t full = s.full;
+        const uint32_t E = ~H & full, Up = E & ~Dn;
+        const double sqf = s.sqrt_tab[s.orbit[f]];
+        const C2 xf = load_c(x, f);
+        const int64_t hr = rank_holes(H, s.comb);
+        const uint32_t gm_self = s.canon_gmask[hr];
+        const int blk_self = s.canon_block[hr];
+        // exchange: the hole pattern is unchanged, only its stabiliser can
+        // lower the new down pattern
+        auto exchange = [&](uint32_t D2, C2 W) {
+            uint32_t gm = gm_self;
+            int bp = ctz32(gm);
+            uint32_t best = image_mask(D2, bp, s.N, full);
+            gm &= gm - 1;
+            while (gm) {
+                const int p = ctz32(gm);
+                const uint32_t d = image_mask(D2, p, s.N, full);
+                if (d < best) {
+                    best = d;
+                    bp = p;
+                }
+                gm &= gm - 1;
+            }
+            accumulate(s, x, H, bp, blk_self, H, best, 0.5 * sqf, W, acc);
+        };
+        if (w.active & 4) {  // jl: bonds (i, i + 2), general/matrix_setup.pyx:95-101
+            const uint32_t d2 = rotl_n(Dn, s.N - 2, s.N, full), u2 = rotl_n(Up, s.N - 2, s.N, full);
+            uint32_t anti = ((Dn & u2) | (Up & d2)) & full;
+            if (s.N == 2) anti = 0;
+            const C2 W = w.w[2];
+            const double diag = -0.5 * double(popc32(anti));
+            acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+            while (anti) {
+                const int i = ctz32(anti);
+                anti &= anti - 1;
+                const int k = (i + 2) % s.N;
+                exchange(Dn ^ (1u << i) ^ (1u << k), W);
+            }
+        }
+        if (w.active & 8) {  // jr: bonds (2x, 2x + 1)
+            uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
+            const C2 W = w.w[3];
+            const double diag = -0.5 * double(popc32(anti));
+            acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+            while (anti) {
+                const int i = ctz32(anti);
+                anti &= anti - 1;
+                exchange(Dn ^ (3u << i), W);
+            }
+        }
+        if (w.active & 3) {  // hopping onto each hole, tl from both leg neighbours, tr from the rung partner
+            uint32_t holes = H;
+            while (holes) {
+                const int t = ctz32(holes);
+                holes &= holes - 1;
+                for (int dir = 0; dir < 3; ++dir) {
+                    int src;
+                    C2 W;
+                    if (dir < 2) {
+                        if (!(w.active & 1)) continue;
+                        src = dir == 0 ? (t + 2) % s.N : (t - 2 + s.N) % s.N;
+                        W = w.w[0];
+                    } else {
+                        if (!(w.active & 2)) continue;
+                        src = t ^ 1;
+                        W = w.w[1];
+                    }
+                    if (!(E & (1u << src))) continue;
+                    const uint32_t bt = 1u << t, bs = 1u << src;
+                    const uint32_t H2 = H ^ bt ^ bs;
+                    const uint32_t D2 = (Dn & bs) ? (Dn ^ bs ^ bt) : Dn;
+                    const uint32_t above_s = src >= 31 ? 0u : (~0u << (src + 1));
+                    const uint32_t above_t = t >= 31 ? 0u : (~0u << (t + 1));
+                    const int par = (popc32(E & above_s) + popc32(E & above_t) + (t < src ? 1 : 0)) & 1;
+                    const int64_t hr2 = rank_holes(H2, s.comb);
+                    const int blk = s.canon_block[hr2];
+                    if (blk < 0) continue;
+                    uint32_t gm = s.canon_gmask[hr2];
+                    int bp = ctz32(gm);
+                    const uint32_t Hc = image_mask(H2, bp, s.N, full);
+                    uint32_t best = image_mask(D2, bp, s.N, full);
+                    gm &= gm - 1;
+                    while (gm) {
+                        const int p = ctz32(gm);
+                        const uint32_t d = image_mask(D2, p, s.N, full);
+                        if (d < best) {
+                            best = d;
+                            bp = p;
+                        }
+                        gm &= gm - 1;
+                    }
+                    accumulate(s, x, H2, bp, blk, Hc, best, par ? sqf : -sqf, W, acc);
+                }
+            }
+        }
+        store_c(y, f, acc);
+        dot = cmul(C2{xf.re, -xf.im}, acc);
+    }
+    if (partial) {
+        // deterministic workgroup reduction of <x, y>
+        __shared__ double red[2 * BS];
+        red[2 * threadIdx.x] = dot.re;
+        red[2 * threadIdx.x + 1] = dot.im;
+        __syncthreads();
+        for (int st = BS / 2; st > 0; st >>= 1) {
+            if (int(threadIdx.x) < st) {
+                red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
+                red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            partial[2 * bid] = red[0];
+            partial[2 * bid + 1] = red[1];
+        }
+    }
+}
+
+// Szq and N(q) in gather form: final row j has the same representative as
+// initial column i; the element is the reference's per-site sum for column i
+// (matrix_setup.pyx:235-246), computed with the same operations.
+__global__ void diag_op_gather_kernel(SectorView ini, SectorView fin, RuleView r, const double* __restrict__ x,
+                                      double* __restrict__ y) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= fin.D) return;
+    const uint64_t key = fin.key[j];
+    const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
+    const Found fi = lookup_rep(ini, H, Dn);
+    if (fi.j < 0) {
+        y[2 * j] = 0.0;
+        y[2 * j + 1] = 0.0;
+        return;
+    }
+    const int nf = fin.orbit[j];
+    const C2 cc = amp_at(fin, 0, nf);
+    const C2 ow0{ini.sqrt_tab[fi.n], 0.0};
+    C2 acc{0.0, 0.0};
+    bool first = true;
+    for (int c = 0; c < r.n_combo; ++c) {
+        const int a = r.idx[2 * c];
+        C2 ow = cmul(ow0, C2{r.w[4 * c + 2], r.w[4 * c + 3]});
+        const int va = site_value(H, Dn, a);
+        if (va == 0) continue;
+        if (r.spin_proj) {
+            C2 f{1.0, 0.0};
+            if (va == 1) f = cmul(f, C2{-1.0, 0.0});
+            ow = cmul(ow, f);
+        }
+        if (!(ow.re * ow.re + ow.im * ow.im >= 1e-16)) continue;
+        const C2 v = cmul(ow, cc);
+        acc = first ? v : cadd(acc, v);
+        first = false;
+    }
+    if (first || acc.re * acc.re + acc.im * acc.im < 1e-16) {
+        y[2 * j] = 0.0;
+        y[2 * j + 1] = 0.0;
+        return;
+    }
+    const C2 xi{x[2 * fi.j], x[2 * fi.j + 1]};
+    const C2 o = cmul(xi, acc);
+    y[2 * j] = o.re;
+    y[2 * j + 1] = o.im;
+}
+
+__global__ void scatter_columns_kernel(int64_t D, int max_per_col, const int32_t* count, const int64_t* rows,
+                                       const double* vals, const double* __restrict__ x, double* y) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const C2 xi{x[2 * i], x[2 * i + 1]};
+    const int c = count[i];
+    for (int k = 0; k < c; ++k) {
+        const int64_t e = i * max_per_col + k;
+        const C2 v = cmul(xi, C2{vals[2 * e], vals[2 * e + 1]});
+        atomicAdd(&y[2 * rows[e]], v.re);
+        atomicAdd(&y[2 * rows[e] + 1], v.im);
+    }
+}
+
+__global__ void zero_kernel(double* p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void reduce_partials_kernel(const double* partial, int64_t n, double* out) {
+    __shared__ double red[512];
+    double re = 0.0, im = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += 256) {
+        re += partial[2 * k];
+        im += partial[2 * k + 1];
+    }
+    red[2 * threadIdx.x] = re;
+    red[2 * threadIdx.x + 1] = im;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (int(threadIdx.x) < st) {
+            red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
+            red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0];
+        out[1] = red[1];
+    }
+}
+
+struct RuleDev {
+    RuleView v{};
+    int8_t* d_idx = nullptr;
+    double* d_w = nullptr;
+    RuleDev(const RuleHost& h, hipStream_t st) {
+        d_idx = dev_upload(h.idx, st);
+        d_w = dev_upload(h.w, st);
+        v.n_combo = h.n_combo();
+        v.arity = h.arity;
+        v.valid = h.valid;
+        v.make = h.make;
+        v.spin_proj = h.spin_proj;
+        v.density = h.density;
+        v.fermion = h.fermion;
+        v.max_per_col = h.max_per_col;
+        v.idx = d_idx;
+        v.w = d_w;
+    }
+    ~RuleDev() {
+        dev_free(d_idx);
+        dev_free(d_w);
+    }
+};
+
+void run_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, int triangle, int max_per_col,
+                 int32_t* d_count, int64_t* d_rows, double* d_vals, hipStream_t st) {
+    ED_REQUIRE(ini && ini->D >= 0 && fin && fin->D >= 0, ED_ERR_STATE, "basis not built");
+    ED_REQUIRE(ini->L == fin->L, ED_ERR_ARG, "sectors of different ladders");
+    ED_REQUIRE(max_per_col >= 1 && max_per_col <= MAXC, ED_ERR_ARG, "max_per_col out of range");
+    if (ini->D == 0) return;
+    RuleDev rd(rule, st);
+    rd.v.max_per_col = max_per_col;
+    const int T = 128;
+    hipLaunchKernelGGL(column_kernel, dim3(unsigned((ini->D + T - 1) / T)), dim3(T), 0, st, ini->view(),
+                       fin->view(), rd.v, triangle, d_count, d_rows, d_vals);
+    ED_HIP(hipGetLastError());
+    ED_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+// Launch helper shared with the Lanczos driver.
+void launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
+               int remap, hipStream_t st) {
+    constexpr int BS = 256;
+    const int64_t nwg = (s->D + BS - 1) / BS;
+    if (nwg == 0) return;
+    if (precision == 0)
+        hipLaunchKernelGGL((hv_kernel<double2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(), w,
+                           static_cast<const double2*>(x), static_cast<double2*>(y), partial, remap);
+    else
+        hipLaunchKernelGGL((hv_kernel<float2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(), w,
+                           static_cast<const float2*>(x), static_cast<float2*>(y), partial, remap);
+    ED_HIP(hipGetLastError());
+}
+
+int64_t hv_partials(const ed_sector* s) { return (s->D + 255) / 256; }
+
+void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, partial, n, out);
+    ED_HIP(hipGetLastError());
+}
+
+HvWeights make_weights(const double* w) {
+    HvWeights hw{};
+    for (int t = 0; t < 4; ++t) {
+        hw.w[t] = C2{w[2 * t], w[2 * t + 1]};
+        // general/sparse_matrices.py:89: terms with |weight| < 1e-6 are skipped
+        if (w[2 * t] * w[2 * t] + w[2 * t + 1] * w[2 * t + 1] >= 1e-12) hw.active |= 1 << t;
+    }
+    return hw;
+}
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" {
+
+int ed_term_columns(const ed_sector* s, int term, int triangle, int max_per_col, int32_t* d_count, int64_t* d_rows,
+                    double* d_vals, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s, ED_ERR_ARG, "sector is null");
+        run_columns(s, s, hamiltonian_rule(s->L, term), triangle, max_per_col, d_count, d_rows, d_vals,
+                    static_cast<hipStream_t>(stream));
+    });
+}
+
+int ed_op_columns(const ed_sector* ini, const ed_sector* fin, int op, int qx, int qy, int max_per_col,
+                  int32_t* d_count, int64_t* d_rows, double* d_vals, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(ini && fin, ED_ERR_ARG, "sector is null");
+        run_columns(ini, fin, operator_rule(ini->L, op, qx, qy), 0, max_per_col, d_count, d_rows, d_vals,
+                    static_cast<hipStream_t>(stream));
+    });
+}
+
+int ed_hv(const ed_sector* s, const double* w, const void* d_x, void* d_y, double* d_dot, int precision,
+          void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (s->D == 0) return;
+        const HvWeights hw = make_weights(w);
+        double* partial = nullptr;
+        if (d_dot) partial = dev_alloc<double>(size_t(2 * hv_partials(s)));
+        launch_hv(s, hw, d_x, d_y, partial, precision, 1, st);
+        if (d_dot) {
+            launch_reduce_partials(partial, hv_partials(s), d_dot, st);
+            ED_HIP(hipStreamSynchronize(st));
+            dev_free(partial);
+        }
+    });
+}
+
+int ed_op_apply(const ed_sector* ini, const ed_sector* fin, int op, int qx, int qy, const double* d_x, double* d_y,
+                void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(ini && fin && ini->D >= 0 && fin->D >= 0, ED_ERR_STATE, "basis not built");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const RuleHost rule = operator_rule(ini->L, op, qx, qy);
+        const int T = 256;
+        if (fin->D == 0) return;
+        if (op == ED_OP_SZQ || op == ED_OP_NQ) {
+            ED_REQUIRE(ini->h == fin->h && ini->nd == fin->nd, ED_ERR_ARG, "Szq/N(q) keep holes and spins");
+            RuleDev rd(rule, st);
+            hipLaunchKernelGGL(diag_op_gather_kernel, dim3(unsigned((fin->D + T - 1) / T)), dim3(T), 0, st,
+                               ini->view(), fin->view(), rd.v, d_x, d_y);
+            ED_HIP(hipGetLastError());
+            ED_HIP(hipStreamSynchronize(st));
+            return;
+        }
+        const int mpc = rule.max_per_col;
+        int32_t* cnt = dev_alloc<int32_t>(size_t(std::max<int64_t>(ini->D, 1)));
+        int64_t* rows = dev_alloc<int64_t>(size_t(std::max<int64_t>(ini->D, 1) * mpc));
+        double* vals = dev_alloc<double>(size_t(std::max<int64_t>(ini->D, 1) * mpc * 2));
+        try {
+            run_columns(ini, fin, rule, 0, mpc, cnt, rows, vals, st);
+            hipLaunchKernelGGL(zero_kernel, dim3(unsigned((2 * fin->D + T - 1) / T)), dim3(T), 0, st, d_y,
+                               2 * fin->D);
+            if (ini->D > 0)
+                hipLaunchKernelGGL(scatter_columns_kernel, dim3(unsigned((ini->D + T - 1) / T)), dim3(T), 0, st,
+                                   ini->D, mpc, cnt, rows, vals, d_x, d_y);
+            ED_HIP(hipGetLastError());
+            ED_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            dev_free(cnt);
+            dev_free(rows);
+            dev_free(vals);
+            throw;
+        }
+        dev_free(cnt);
+        dev_free(rows);
+        dev_free(vals);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""GPU symmetry sector: basis, lookup, matrix elements, H·v, operators, Lanczos.
+
+A :class:`Sector` owns one ``ed_sector`` handle of the HIP library: the
+sector (num_rungs, num_holes, num_down_spins, k) of the reference's
+``Hamiltonian_Config`` (tJ_spin_half_ladder/configs.py:40-147) with its basis
+built on the GPU.  Vectors are torch tensors on the current HIP device
+(complex128, or complex64 for the reduced-precision form); torch provides
+device memory and the stream, all arithmetic runs in the library's kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from exactdiag_amd import _lib
+from exactdiag_amd._lib import check, lib
+
+TERMS = ("tl", "tr", "jl", "jr")
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("exactdiag_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream():
+    return ct.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ct.c_void_p(t.data_ptr())
+
+
+def _dbl(arr):
+    arr = np.ascontiguousarray(arr, np.float64)
+    return arr, arr.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def weights_array(weights) -> np.ndarray:
+    """Four complex term weights (tl, tr, jl, jr) as 8 doubles."""
+    if isinstance(weights, dict):
+        weights = [weights.get(t, 0.0) for t in TERMS]
+    w = np.zeros(4, np.complex128)
+    w[: len(weights)] = np.asarray(weights, np.complex128)
+    return w.view(np.float64).copy()
+
+
+class Sector:
+    """Basis of one (L, h, nd, kx, ky) sector, built on the GPU."""
+
+    def __init__(self, num_rungs: int, num_holes: int, num_down_spins: int, kx: int, ky: int):
+        self.L, self.h, self.nd, self.kx, self.ky = num_rungs, num_holes, num_down_spins, int(kx), int(ky)
+        self.N = 2 * num_rungs
+        self._lib = lib()
+        torch = _torch()
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        h = ct.c_void_p()
+        check(self._lib.ed_sector_create(num_rungs, num_holes, num_down_spins, self.kx, self.ky, ct.byref(h)))
+        self._h = h
+        check(self._lib.ed_basis_build(self._h, _stream()))
+        self.dim = int(self._lib.ed_basis_size(self._h))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.ed_sector_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def shape(self):
+        return (self.dim, self.dim)
+
+    # -- basis ---------------------------------------------------------
+    def basis(self):
+        """(dense_to_sparse int64, num_sparse_in_dense int32) as numpy arrays."""
+        torch = _torch()
+        sp = torch.empty(max(self.dim, 1), dtype=torch.int64, device=self.device)
+        orb = torch.empty(max(self.dim, 1), dtype=torch.int32, device=self.device)
+        check(self._lib.ed_basis_export(self._h, _ptr(sp), _ptr(orb), _stream()))
+        torch.cuda.synchronize()
+        return sp[: self.dim].cpu().numpy(), orb[: self.dim].cpu().numpy()
+
+    def lookup(self, sparse):
+        torch = _torch()
+        s = torch.as_tensor(np.ascontiguousarray(sparse, np.int64), device=self.device)
+        out = torch.empty_like(s)
+        check(self._lib.ed_lookup(self._h, _ptr(s), _ptr(out), s.numel(), _stream()))
+        return out.cpu().numpy()
+
+    def lowest(self, sparse):
+        torch = _torch()
+        s = torch.as_tensor(np.ascontiguousarray(sparse, np.int64), device=self.device)
+        rep = torch.empty_like(s)
+        amp = torch.empty(s.numel(), dtype=torch.complex128, device=self.device)
+        check(self._lib.ed_lowest(self._h, _ptr(s), _ptr(rep), _ptr(amp), s.numel(), _stream()))
+        return rep.cpu().numpy(), amp.cpu().numpy()
+
+    # -- matrix elements ---------------------------------------------------
+    def _columns(self, fn, args, max_per_col, dim):
+        torch = _torch()
+        n = max(dim, 1)
+        cnt = torch.zeros(n, dtype=torch.int32, device=self.device)
+        rows = torch.empty(n * max_per_col, dtype=torch.int64, device=self.device)
+        vals = torch.empty(n * max_per_col, dtype=torch.complex128, device=self.device)
+        check(fn(*args, max_per_col, _ptr(cnt), _ptr(rows), _ptr(vals), _stream()))
+        cnt = cnt[:dim].cpu().numpy()
+        if dim and cnt.max(initial=0) > max_per_col:
+            raise IndexError(f"max_values_per_column={max_per_col} set too low, {int(cnt.max())} found")
+        rows = rows.view(-1, max_per_col)[:dim].cpu().numpy()
+        vals = vals.view(-1, max_per_col)[:dim].cpu().numpy()
+        ptr = np.zeros(dim + 1, np.int64)
+        np.cumsum(cnt, out=ptr[1:])
+        mask = np.arange(max_per_col)[None, :] < cnt[:, None]
+        return ptr, rows[mask], vals[mask]
+
+    def term_columns(self, term: int, triangle: bool = True, max_per_col: int | None = None):
+        """Columns of one Hamiltonian term before de-duplication (CSC: ptr, rows, vals)."""
+        mpc = max_per_col or (2 * self.N + 2)
+        return self._columns(self._lib.ed_term_columns, (self._h, int(term), int(triangle)), mpc, self.dim)
+
+    def op_columns(self, final: "Sector", op: int, qx: int, qy: int, max_per_col: int | None = None):
+        mpc = max_per_col or (2 * self.N + 2)
+        return self._columns(self._lib.ed_op_columns, (self._h, final._h, int(op), int(qx), int(qy)), mpc, self.dim)
+
+    # -- products ------------------------------------------------------------
+    def hv(self, weights, x, out=None, with_dot=False):
+        """y = H x (matrix-free).  x: complex128 or complex64 tensor on the device."""
+        torch = _torch()
+        prec = 0 if x.dtype == torch.complex128 else 1
+        if x.dtype not in (torch.complex128, torch.complex64):
+            raise TypeError("x must be complex128 or complex64")
+        x = x.contiguous()
+        y = torch.empty_like(x) if out is None else out
+        w, wp = _dbl(weights_array(weights))
+        dot = torch.zeros(1, dtype=torch.complex128, device=self.device) if with_dot else None
+        check(self._lib.ed_hv(self._h, wp, _ptr(x), _ptr(y), _ptr(dot) if dot is not None else None, prec,
+                              _stream()))
+        if with_dot:
+            return y, complex(dot.cpu().numpy()[0])
+        return y
+
+    def op_apply(self, final: "Sector", op: int, qx: int, qy: int, x):
+        torch = _torch()
+        x = x.to(torch.complex128).contiguous()
+        y = torch.empty(max(final.dim, 1), dtype=torch.complex128, device=self.device)[: final.dim]
+        check(self._lib.ed_op_apply(self._h, final._h, int(op), int(qx), int(qy), _ptr(x), _ptr(y), _stream()))
+        return y
+
+    def lanczos(self, weights, v0, nsteps: int, keep_basis: bool = False, precision: int = 0):
+        """Plain Lanczos coefficients (a, b) from v0; optionally the vectors (nsteps x D)."""
+        torch = _torch()
+        v0 = v0.to(torch.complex128).contiguous()
+        a = np.zeros(nsteps, np.float64)
+        b = np.zeros(nsteps + 1, np.float64)
+        n = ct.c_int(0)
+        w, wp = _dbl(weights_array(weights))
+        basis = None
+        if keep_basis:
+            dt = torch.complex128 if precision == 0 else torch.complex64
+            basis = torch.empty((nsteps, self.dim), dtype=dt, device=self.device)
+        check(self._lib.ed_lanczos_run(self._h, wp, _ptr(v0), int(nsteps), a.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                       b.ctypes.data_as(ct.POINTER(ct.c_double)), ct.byref(n),
+                                       _ptr(basis) if basis is not None else None, int(precision), _stream()))
+        k = n.value
+        if keep_basis:
+            return a[:k].copy(), b[:k].copy(), basis[:k]
+        return a[:k].copy(), b[:k].copy()
+
+
+def cf_eval(a, b, ws, e0, eta, factor):
+    """factor * Im(1/det(z)) for z = ws + e0 + i*eta on the GPU."""
+    a, ap = _dbl(a)
+    b, bp = _dbl(b)
+    ws, wsp = _dbl(ws)
+    out = np.empty(len(ws), np.float64)
+    check(lib().ed_cf_eval(ap, bp, len(a), wsp, len(ws), float(e0), float(eta), float(factor),
+                           out.ctypes.data_as(ct.POINTER(ct.c_double))))
+    return out

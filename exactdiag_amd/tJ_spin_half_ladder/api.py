@@ -1,0 +1,121 @@
+"""Top-level entry points (reference: tJ_spin_half_ladder/api.py).
+
+Same functions and return shapes as the reference.  Plotting
+(api.py:202-209) and position correlations (api.py:104-113) are outside the
+GPU hot path and not provided.
+"""
+
+from __future__ import annotations
+
+import copy
+import dataclasses
+import logging
+import typing
+from itertools import product
+
+import numpy as np
+
+from exactdiag_amd.general.lanczos_diagonalization import get_lowest_eigenpairs, get_spectrum
+from exactdiag_amd.tJ_spin_half_ladder import configs
+
+_logger = logging.getLogger(__name__)
+
+
+class IntSymmetryQs(typing.NamedTuple):
+    leg: int
+    rung: int
+
+
+def get_eigenpairs(config):
+    """eigenvalues and eigenvectors[i, j] of the j-th eigenpair (api.py:33-42)."""
+    if config.hamiltonian.symmetry_qs is None:
+        return get_all_k_eigenpairs(config)
+    return get_lowest_eigenpairs(config)
+
+
+def get_all_k_eigenpairs(config):
+    """{(kx, ky): (eigenvalues, eigenvectors)} over kx <= L/2 (api.py:45-59)."""
+    mut_config = copy.deepcopy(config)
+    mkx = config.hamiltonian.num_rungs // 2 + 1
+    collected = {}
+    for kx in range(mkx):
+        for ky in range(2):
+            mut_config.hamiltonian.symmetry_qs = {"leg": kx, "rung": ky}
+            collected[(kx, ky)] = get_lowest_eigenpairs(mut_config)
+    return collected
+
+
+@dataclasses.dataclass
+class Spectrum:
+    ws: np.ndarray
+    spectrum: np.ndarray
+    config: typing.Any
+    info: dict = dataclasses.field(default_factory=dict)
+
+    def __post_init__(self):
+        self.ws = np.array(self.ws)
+        self.spectrum = np.array(self.spectrum)
+
+
+def get_excitation_spectrum(config, limited_qs: bool = True) -> Spectrum:
+    """The spectra the config names (api.py:83-101)."""
+    base_name = str(config.spectrum.name)
+    info = {}
+    if base_name in {"current_rung", "current_leg"}:
+        ws, spectrum = get_spectrum(config=config)
+    elif base_name in {"Szq", "Nq"}:
+        ws, spectrum, info = get_Szq_spectra(config=config, limited_qs=limited_qs)
+    elif base_name in {"spectral_function_plus", "spectral_function_minus"}:
+        ws, spectrum, info = get_spectral_function_spectra(config=config, limited_qs=limited_qs)
+    else:
+        raise ValueError(f"{base_name} is not supported")
+    return Spectrum(ws, spectrum, config, info)
+
+
+def get_Szq_spectra(config, limited_qs: bool):
+    """All-q S(q, w) (api.py:116-132): with limited_qs, qx > L//2 reuses L - qx."""
+    config = copy.deepcopy(config)
+    mkx, qxs, qys = get_mkx_qxs_qys(config, limited_qs=False)
+    qs_list = product(qxs, qys)
+    spectra = np.empty((len(qxs), len(qys), config.spectrum.omega_steps))
+    ws = np.empty((len(qxs), len(qys), config.spectrum.omega_steps))
+    for i, qx in enumerate(qxs):
+        for j, qy in enumerate(qys):
+            if limited_qs and qx > mkx // 2:
+                qx = mkx - qx
+            config.spectrum.operator_symmetry_qs = {"leg": qx, "rung": qy}
+            ws[i, j, :], spectra[i, j, :] = get_spectrum(config)
+    return ws, spectra, {"qs_list": qs_list}
+
+
+def get_spectral_function_spectra(config, limited_qs: bool):
+    """A(k, w) from the plus and minus parts (api.py:135-161)."""
+    config = copy.deepcopy(config)
+    mkx, qxs, qys = get_mkx_qxs_qys(config=config, limited_qs=limited_qs)
+    plus_minus_conditions = [
+        (configs.Spectrum_Name.SPECTRAL_FUNCTION_PLUS, config.hamiltonian.num_holes > 0),
+        (configs.Spectrum_Name.SPECTRAL_FUNCTION_MINUS, config.hamiltonian.num_holes < config.hamiltonian.num_nodes),
+    ]
+    qs_list = list(product(qxs, qys))
+    shape = [sum(c for _, c in plus_minus_conditions), len(qs_list), config.spectrum.omega_steps]
+    spectra = np.empty(shape, dtype=float)
+    ws = np.empty(shape, dtype=float)
+    for i, (qx, qy) in enumerate(qs_list):
+        if limited_qs and qx > mkx // 2 + 1:
+            continue
+        qs = np.array([qx, qy], dtype=int)
+        for o, (name, condition) in enumerate(plus_minus_conditions):
+            if not condition:
+                continue
+            if name == configs.Spectrum_Name.SPECTRAL_FUNCTION_MINUS:
+                qs = -qs
+            config.spectrum.operator_symmetry_qs = {"leg": int(qs[0]), "rung": int(qs[1] % 2)}
+            config.spectrum.name = name
+            ws[o, i, :], spectra[o, i, :] = get_spectrum(config)
+    return ws, spectra, {"qs_list": qs_list}
+
+
+def get_mkx_qxs_qys(config, limited_qs: bool):
+    mkx = config.hamiltonian.num_rungs
+    qxs = range(mkx // 2 + 1) if limited_qs else range(mkx)
+    return mkx, qxs, range(2)

@@ -1,0 +1,231 @@
+"""Wiring of sectors, Hamiltonians and excitation operators
+(reference: tJ_spin_half_ladder/matrix_setup.pyx).
+
+``setup_hamiltonian`` / ``setup_excitation_operator`` / ``py_get_ladder_translators``
+keep the reference's signatures and return values; the objects behind them
+are GPU sectors and matrix-free operators instead of CSR files.
+"""
+
+from __future__ import annotations
+
+import copy
+import functools
+import logging
+
+import numpy as np
+
+from exactdiag_amd import _lib
+from exactdiag_amd.general.sparse_matrices import Excitation_Operator, Hamiltonian_Operator
+from exactdiag_amd.sector import Sector
+from exactdiag_amd.tJ_spin_half_ladder import configs
+
+_logger = logging.getLogger(__name__)
+
+_SECTORS: "dict[tuple, Sector]" = {}
+
+
+def get_sector(config: "configs.Hamiltonian_Config") -> Sector:
+    """The GPU sector of a Hamiltonian config (built once per process and key)."""
+    key = config.sector_key()
+    s = _SECTORS.get(key)
+    if s is None:
+        s = Sector(*key)
+        _SECTORS[key] = s
+    return s
+
+
+def clear_sector_cache():
+    for s in _SECTORS.values():
+        s.close()
+    _SECTORS.clear()
+
+
+class Symmetries:
+    """Inspection view of the ladder symmetry group (symmetry_ladder.h:55-111)."""
+
+    def __init__(self, num_rungs, symmetry_qs):
+        self.num_rungs = num_rungs
+        self.symmetry_qs = list(symmetry_qs)
+
+    def get_basis_length(self):
+        return 2 * self.num_rungs
+
+    def get_shift_periodicities(self):
+        return [self.num_rungs, 2]
+
+    def get_num_shifts(self):
+        return 2
+
+    def get_index_by_relative_shift(self, initial_index, relative_shift):
+        N = 2 * self.num_rungs
+        i = initial_index % N
+        leg = i // 2 + int(relative_shift[0])
+        s1 = int(relative_shift[1])
+        rung = (i + int(np.fmod(s1, 2))) % 2
+        return (2 * leg + rung) % N
+
+
+class Translator:
+    """Sparse index <-> state and lowest-state lookups (State_Index_Amplitude_Translator)."""
+
+    def __init__(self, sector: Sector, config):
+        self.sector = sector
+        self.num_nodes = sector.N
+        self.num_holes = sector.h
+        self.num_down_spins = sector.nd
+        self.comb = configs.set_combinatorics(sector.N)
+        self.num_spin_states = int(self.comb[sector.nd, sector.N - sector.h])
+
+    def get_num_minor_sparse_states(self):
+        return self.num_spin_states
+
+    def sparse_index_to_state(self, sparse_index):
+        """symmetry_ladder.h:314-344."""
+        N, comb = self.num_nodes, self.comb
+        rh, rs = self.num_holes, self.num_down_spins
+        hi, si = divmod(int(sparse_index), self.num_spin_states)
+        state = np.empty(N, np.int16)
+        for j in range(N - 1, -1, -1):
+            if rh > 0 and hi >= comb[rh, j]:
+                state[j] = 0
+                hi -= comb[rh, j]
+                rh -= 1
+                continue
+            if rs > 0 and si >= comb[rs, j - rh]:
+                state[j] = 1
+                si -= comb[rs, j - rh]
+                rs -= 1
+                continue
+            state[j] = 2
+        return state
+
+    def state_to_sparse_index(self, state):
+        """symmetry_ladder.h:346-361."""
+        hi = si = 0
+        ah, ad = int(state[0] == 0), int(state[0] == 1)
+        for i in range(1, self.num_nodes):
+            if state[i] == 0:
+                ah += 1
+                hi += self.comb[ah, i]
+            elif state[i] == 1:
+                ad += 1
+                si += self.comb[ad, i - ah]
+        return int(si + hi * self.num_spin_states)
+
+    def get_lowest_sparse_amplitude_from_state(self, state):
+        """(sparse index of the lowest image or -1, its amplitude), general/symmetry.h:240-269."""
+        rep, amp = self.sector.lowest([self.state_to_sparse_index(state)])
+        return int(rep[0]), complex(amp[0])
+
+
+class Basis_Map:
+    """Dense <-> sparse index map of a sector (Basis_Index_Map, basis_indexing.h:31-185)."""
+
+    def __init__(self, sector: Sector):
+        self.sector = sector
+        self._host = None
+
+    def _arrays(self):
+        if self._host is None:
+            self._host = self.sector.basis()
+        return self._host
+
+    def get_num_states(self):
+        return self.sector.dim
+
+    def get_sparse(self, dense_index):
+        return int(self._arrays()[0][dense_index])
+
+    def get_dense(self, sparse_index):
+        return int(self.sector.lookup([sparse_index])[0])
+
+    def get_num_sparse_in_dense(self, dense_index):
+        return int(self._arrays()[1][dense_index])
+
+    def get_num_sparse_in_dense_from_sparse(self, sparse_index):
+        d = self.get_dense(sparse_index)
+        return self.get_num_sparse_in_dense(d) if d > -1 else 0
+
+
+def py_get_ladder_translators(config: "configs.Hamiltonian_Config"):
+    """(translator, basis map, symmetries), matrix_setup.pyx:377-406."""
+    s = get_sector(config)
+    return Translator(s, config), Basis_Map(s), Symmetries(config.num_rungs, config.symmetry_qs.to_npint32())
+
+
+def _weights_dict(config):
+    return {n: getattr(config.weights, n, 0) for n in ("tl", "tr", "jl", "jr")}
+
+
+def setup_hamiltonian(config: "configs.Hamiltonian_Config"):
+    """Callable returning the Hamiltonian (matrix_setup.pyx:41-129)."""
+    weights = _weights_dict(config)
+
+    def get():
+        return Hamiltonian_Operator(get_sector(config), weights)
+
+    return get
+
+
+OPERATOR_IDS = {
+    configs.Spectrum_Name.SZQ: _lib.OP_SZQ,
+    configs.Spectrum_Name.DENSITY_Q: _lib.OP_NQ,
+    configs.Spectrum_Name.CURRENT_RUNG: _lib.OP_CURRENT_RUNG,
+    configs.Spectrum_Name.CURRENT_LEG: _lib.OP_CURRENT_LEG,
+}
+
+
+def excitation_setup(initial_config):
+    """Final-sector config, operator id and q of an excitation (matrix_setup.pyx:195-330)."""
+    ham = initial_config.hamiltonian
+    num_rungs = ham.num_rungs
+    kx, ky = (int(v) for v in ham.symmetry_qs.to_npint32())
+    qs = initial_config.spectrum.operator_symmetry_qs
+    qx, qy = (int(v) for v in qs.to_npint32()) if qs is not None else (0, 0)
+    final = copy.deepcopy(initial_config)
+    name = configs.Spectrum_Name(initial_config.spectrum.name)
+    if name in (configs.Spectrum_Name.SZQ, configs.Spectrum_Name.DENSITY_Q, configs.Spectrum_Name.CURRENT_LEG):
+        op = OPERATOR_IDS[name]
+    elif name == configs.Spectrum_Name.CURRENT_RUNG:
+        qy += 1  # matrix_setup.pyx:251
+        op = _lib.OP_CURRENT_RUNG
+    elif name == configs.Spectrum_Name.SPECTRAL_FUNCTION_PLUS:
+        final.hamiltonian.num_holes -= 1
+        if ham.total_spin_projection > 0:
+            final.hamiltonian.total_spin_projection -= 1
+            op = _lib.OP_APLUS_DOWN
+        else:
+            final.hamiltonian.total_spin_projection += 1
+            op = _lib.OP_APLUS_UP
+    elif name == configs.Spectrum_Name.SPECTRAL_FUNCTION_MINUS:
+        final.hamiltonian.num_holes += 1
+        if ham.total_spin_projection <= 0:
+            final.hamiltonian.total_spin_projection += 1
+            op = _lib.OP_AMINUS_DOWN
+        else:
+            final.hamiltonian.total_spin_projection -= 1
+            op = _lib.OP_AMINUS_UP
+    else:
+        raise ValueError(f"{name} not supported")
+    half = (num_rungs - 1) // 2
+    final.hamiltonian.symmetry_qs.leg = (kx + qx + half) % num_rungs - half
+    final.hamiltonian.symmetry_qs.rung = (ky + qy) % 2
+    return final, op, qx, qy
+
+
+def setup_excitation_operator(initial_config):
+    """(get_operator, final Hamiltonian config, get_excited_H), matrix_setup.pyx:195-374."""
+    final, op, qx, qy = excitation_setup(initial_config)
+    get_excited_H = final.hamiltonian.setup()
+    name = str(initial_config.spectrum.name)
+
+    def get_operator():
+        return Excitation_Operator(get_sector(initial_config.hamiltonian), get_sector(final.hamiltonian), op, qx,
+                                   qy, name)
+
+    _logger.debug(f"Finished setting up operator {name} with {initial_config}.")
+    return get_operator, final.hamiltonian, get_excited_H
+
+
+# keep functools imported for callers that build partials like the reference
+partial = functools.partial

@@ -1,0 +1,134 @@
+/* C ABI of the MI355X (gfx950) t-J ladder exact-diagonalisation library
+ * (libexactdiag_hip.so).  Plain pointers and sizes only.  Every function
+ * returns 0 on success and a negative status on failure; ed_last_error()
+ * gives the thread's last message.  Device pointers are HIP device memory
+ * owned by the caller; `stream` is a hipStream_t (0 = null stream).
+ *
+ * Complex vectors are interleaved (re, im) doubles ("c128") or floats
+ * ("c64").  Dense basis indices follow the reference's order (ascending
+ * sparse index).
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference package root src/exactdiag/).
+ */
+#ifndef EXACTDIAG_HIP_H
+#define EXACTDIAG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ed_sector ed_sector;
+
+enum ed_status {
+    ED_OK = 0,
+    ED_ERR_ARG = -1,      /* invalid argument (the reference raises ValueError) */
+    ED_ERR_HIP = -2,      /* a HIP runtime call failed */
+    ED_ERR_STATE = -3,    /* call order violated (e.g. basis not built) */
+    ED_ERR_CAPACITY = -4  /* output capacity too small (the reference raises IndexError) */
+};
+
+/* Operator ids for ed_op_columns / ed_op_apply. */
+enum ed_operator {
+    ED_OP_SZQ = 0,          /* tJ_spin_half_ladder/matrix_setup.pyx:235-246 */
+    ED_OP_NQ = 1,           /* density N(q): no counterpart in the reference */
+    ED_OP_APLUS_DOWN = 2,   /* matrix_setup.pyx:281-300, adds a down spin */
+    ED_OP_APLUS_UP = 3,     /* matrix_setup.pyx:281-300, adds an up spin */
+    ED_OP_AMINUS_DOWN = 4,  /* matrix_setup.pyx:302-325, removes a down spin */
+    ED_OP_AMINUS_UP = 5,    /* matrix_setup.pyx:302-325, removes an up spin */
+    ED_OP_CURRENT_RUNG = 6, /* matrix_setup.pyx:248-263 (caller passes qy already shifted by one) */
+    ED_OP_CURRENT_LEG = 7   /* matrix_setup.pyx:265-279 */
+};
+
+const char* ed_last_error(void);
+int ed_version(void);
+
+/* Sector = (num_rungs L, num_holes h, num_down_spins nd, k = (kx, ky)).
+ * Replaces Symmetries_Single_Spin_Half_2leg_Ladder + State_Index_Amplitude_
+ * Translator (tJ_spin_half_ladder/symmetry_ladder.h:55-406) as built by
+ * py_get_ladder_translators (tJ_spin_half_ladder/matrix_setup.pyx:377-406).
+ * Host tables (binomials, amplitudes, hole patterns) are built here. */
+int ed_sector_create(int L, int h, int nd, int kx, int ky, ed_sector** out);
+int ed_sector_destroy(ed_sector* s);
+
+/* Enumerate the symmetric basis on the GPU.  Replaces calc_basis_map
+ * (general/basis_indexing.h:188-242). */
+int ed_basis_build(ed_sector* s, void* stream);
+int64_t ed_basis_size(const ed_sector* s); /* Basis_Index_Map::get_num_states, basis_indexing.h:182-184 */
+
+/* Basis in the reference's format: sparse index (int64) and orbit size
+ * (int32) per dense index; both device pointers.  Replaces
+ * Basis_Index_Map::get_sparse / get_num_sparse_in_dense (basis_indexing.h:148-150, :170-172). */
+int ed_basis_export(const ed_sector* s, int64_t* d_sparse, int32_t* d_orbit, void* stream);
+
+/* Sparse index -> dense index (-1 when absent) for n device-resident
+ * indices.  Replaces Basis_Index_Map::get_dense (basis_indexing.h:152-168). */
+int ed_lookup(const ed_sector* s, const int64_t* d_sparse, int64_t* d_dense, int64_t n, void* stream);
+
+/* Lowest image of n arbitrary states given by sparse index: out sparse index
+ * of the representative (-1 if incompatible) and its normalised amplitude
+ * (c128).  Replaces get_lowest_sparse_amplitude_from_state (general/symmetry.h:240-269). */
+int ed_lowest(const ed_sector* s, const int64_t* d_sparse, int64_t* d_rep_sparse, double* d_amp, int64_t n,
+              void* stream);
+
+/* Matrix elements of one Hamiltonian term (0 tl, 1 tr, 2 jl, 3 jr) column by
+ * column, before the reference's unique-value table: for column i,
+ * d_count[i] entries at d_rows/d_vals[i * max_per_col ...], sorted by row.
+ * triangle = keep rows >= i only.  Replaces calc_column_matrix_elements with
+ * the rules of setup_hamiltonian (general/column_functions.h:150-207,
+ * tJ_spin_half_ladder/matrix_setup.pyx:80-129). */
+int ed_term_columns(const ed_sector* s, int term, int triangle, int max_per_col, int32_t* d_count, int64_t* d_rows,
+                    double* d_vals, void* stream);
+
+/* Same for an excitation operator from sector `init` to sector `fin`
+ * (setup_excitation_operator, matrix_setup.pyx:195-374). */
+int ed_op_columns(const ed_sector* init, const ed_sector* fin, int op, int qx, int qy, int max_per_col,
+                  int32_t* d_count, int64_t* d_rows, double* d_vals, void* stream);
+
+/* Matrix-free y = sum_T w_T H_T x over the four terms (w: 4 complex
+ * weights as 8 doubles, host memory).  Optional d_dot (>= 1 c128 on device):
+ * receives <x, y>.  precision: 0 = c128 vectors, 1 = c64 vectors (FP64 sums
+ * inside a row, FP32 storage).  Replaces Added_Sparse_Matrices.dot
+ * (general/sparse_matrices.py:233-237) over sparse_matrix_dot
+ * (general/cython_sparse_dot.pyx:40-100). */
+int ed_hv(const ed_sector* s, const double* w, const void* d_x, void* d_y, double* d_dot, int precision,
+          void* stream);
+
+/* y = O x for an excitation operator (x in sector init, y in sector fin).
+ * Replaces excitation_operator.dot in lanczos_dynamical_response
+ * (general/cython_lanczos_diagonalization.pyx:36). */
+int ed_op_apply(const ed_sector* init, const ed_sector* fin, int op, int qx, int qy, const double* d_x, double* d_y,
+                void* stream);
+
+/* Plain Lanczos recurrence from start vector d_v0 (c128, need not be
+ * normalised): a[0..n), b[0..n) with b[0] = 0, stopping when b < 1e-8, as
+ * _lanczos_get_anbn (general/cython_lanczos_diagonalization.pyx:50-77).
+ * Returns the number of coefficients in *n_out.  d_basis (optional,
+ * nsteps * D c128) receives the normalised Lanczos vectors.  precision as in
+ * ed_hv; the coefficient reductions are FP64 in both. */
+int ed_lanczos_run(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
+                   int* n_out, double* d_basis, int precision, void* stream);
+
+/* Continued fraction of general/cython_lanczos_diagonalization.pyx:42-48
+ * evaluated on the GPU: out[k] = factor * Im(1 / det(z_k)) for z_k = ws[k] +
+ * e0 + i*eta.  Host arrays in and out. */
+int ed_cf_eval(const double* a, const double* b, int n, const double* ws, int n_w, double e0, double eta,
+               double factor, double* out);
+
+/* Dense vector helpers for the Krylov eigensolver (all c128, device).
+ * ed_vec_dots: out[k] = <V_k, w> for k < nv (host out, FP64 deterministic).
+ * ed_vec_axpys: w -= sum_k c[k] V_k (c host).  ed_vec_combine: out = sum_k
+ * c[k] V_k (c host).  ed_vec_norm2: host out = ||w||^2.  ed_vec_scale: w *= c. */
+int ed_vec_dots(const double* d_V, int nv, int64_t ld, const double* d_w, int64_t D, double* out, void* stream);
+int ed_vec_axpys(const double* d_V, int nv, int64_t ld, const double* c, double* d_w, int64_t D, void* stream);
+int ed_vec_combine(const double* d_V, int nv, int64_t ld, const double* c, double* d_out, int64_t D, void* stream);
+int ed_vec_norm2(const double* d_w, int64_t D, double* out, void* stream);
+int ed_vec_scale(double* d_w, int64_t D, double cr, double ci, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXACTDIAG_HIP_H */
