@@ -21,6 +21,8 @@ EXPORTED = [
     "ed_last_error", "ed_version", "ed_sector_create", "ed_sector_destroy", "ed_basis_build", "ed_basis_size",
     "ed_basis_export", "ed_lookup", "ed_lowest", "ed_term_columns", "ed_op_columns", "ed_hv", "ed_op_apply",
     "ed_lanczos_run", "ed_cf_eval", "ed_vec_dots", "ed_vec_axpys", "ed_vec_combine", "ed_vec_norm2", "ed_vec_scale",
+    "ed_mlp_create", "ed_mlp_destroy", "ed_mlp_num_params", "ed_mlp_get_params", "ed_mlp_set_params",
+    "ed_mlp_forward", "ed_mlp_gradients", "ed_mlp_train_step", "ed_gemm_bf16",
 ]
 
 _vp = ct.c_void_p
@@ -49,6 +51,15 @@ _SIGNATURES = {
     "ed_vec_combine": (ct.c_int, [_vp, ct.c_int, _i64, _dp, _vp, _i64, _vp]),
     "ed_vec_norm2": (ct.c_int, [_vp, _i64, _dp, _vp]),
     "ed_vec_scale": (ct.c_int, [_vp, _i64, ct.c_double, ct.c_double, _vp]),
+    "ed_mlp_create": (ct.c_int, [ct.POINTER(ct.c_int), ct.c_int, ct.c_uint64, ct.POINTER(_vp)]),
+    "ed_mlp_destroy": (ct.c_int, [_vp]),
+    "ed_mlp_num_params": (_i64, [_vp]),
+    "ed_mlp_get_params": (ct.c_int, [_vp, ct.POINTER(ct.c_float), _i64]),
+    "ed_mlp_set_params": (ct.c_int, [_vp, ct.POINTER(ct.c_float), _i64]),
+    "ed_mlp_forward": (ct.c_int, [_vp, _vp, ct.c_int, _vp, _vp]),
+    "ed_mlp_gradients": (ct.c_int, [_vp, _vp, _vp, ct.c_int, ct.POINTER(ct.c_float), _dp, _vp]),
+    "ed_mlp_train_step": (ct.c_int, [_vp, _vp, _vp, ct.c_int, ct.c_float, _dp, _vp]),
+    "ed_gemm_bf16": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,127 @@
+"""All-q dynamical spin structure factor S(q, w) on one GPU.
+
+The reference computes all-q S(q, w) by looping over q and calling
+``get_spectrum`` for each (tJ_spin_half_ladder/api.py:116-132): one ground
+state, then per q a final-sector basis, the Szq operator, a plain Lanczos
+run and the continued fraction.  This module runs the same sequence with
+every step on the device and without disk caches, and measures it:
+
+* ``mode="plain"``: FP64 vectors, the configured number of Lanczos steps,
+  one q after the other (the plain FP64 GPU Lanczos of the benchmark);
+* ``mode="fast"``: FP32 vectors with FP64 coefficient reductions, fewer
+  Lanczos steps and a continued-fraction tail (``closure``) in place of the
+  missing levels, which is the time-to-accuracy form.
+
+Spectral L1 error = sum_w |S - S_ref| dw / sum_w |S_ref| dw on the same grid.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+from exactdiag_amd import _lib
+from exactdiag_amd.general import krylov
+from exactdiag_amd.sector import Sector, cf_eval
+
+
+def final_momentum(L, kx, ky, qx, qy):
+    """tJ_spin_half_ladder/matrix_setup.pyx:329-330."""
+    half = (L - 1) // 2
+    return (kx + qx + half) % L - half, (ky + qy) % 2
+
+
+def distinct_qs(L, limited=True):
+    """(qx, qy) pairs computed; with ``limited`` qx > L//2 reuses L - qx (api.py:126-131)."""
+    qxs = range(L // 2 + 1) if limited else range(L)
+    return [(qx, qy) for qx in qxs for qy in (0, 1)]
+
+
+def spectral_l1(S, S_ref, ws):
+    dw = np.gradient(np.asarray(ws))
+    return float(np.sum(np.abs(S - S_ref) * dw) / np.sum(np.abs(S_ref) * dw))
+
+
+@dataclasses.dataclass
+class AllQResult:
+    ws: np.ndarray
+    qs: list
+    spectra: dict          # (qx, qy) -> S(w)
+    coefficients: dict     # (qx, qy) -> (a, b, norm2)
+    e0: float
+    timings: dict
+
+
+def ground_state(L, h, nd, kx, ky, weights, tol=1e-12, seed=0):
+    import torch
+
+    s = Sector(L, h, nd, kx, ky)
+    w = np.asarray(weights, np.complex128)
+    from exactdiag_amd.sector import weights_array
+
+    wa = weights_array(w)
+
+    def apply(v):
+        return s.hv(wa, v)
+
+    vals, vecs, _ = krylov.lowest_eigenpairs(apply, s.dim, 1, tol=tol, v0=krylov.start_vector(s.dim, seed))
+    torch.cuda.synchronize()
+    return s, float(vals[0]), vecs[0]
+
+
+def run_allq(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), precision=0, limited=True, closure=None, gs=None,
+             keep_sectors=False):
+    """All-q S(q, w) for the Szq operator.  Returns AllQResult with timings in seconds."""
+    import torch
+
+    timings = {}
+    t0 = time.perf_counter()
+    if gs is None:
+        s0, e0, psi = ground_state(L, h, nd, k0[0], k0[1], weights)
+    else:
+        s0, e0, psi = gs
+    torch.cuda.synchronize()
+    timings["ground_state"] = time.perf_counter() - t0
+    from exactdiag_amd.sector import weights_array
+
+    wa = weights_array(np.asarray(weights, np.complex128))
+    spectra, coeffs = {}, {}
+    t_basis = t_lz = t_cf = t_op = 0.0
+    sectors = {}
+    for qx, qy in distinct_qs(L, limited):
+        t1 = time.perf_counter()
+        kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
+        sec = sectors.get((kx, ky))
+        if sec is None:
+            sec = Sector(L, h, nd, kx, ky)
+            if keep_sectors:
+                sectors[(kx, ky)] = sec
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        phi = s0.op_apply(sec, _lib.OP_SZQ, qx, qy, psi)
+        norm2 = krylov.vec_norm(phi) ** 2 if sec.dim else 0.0
+        t3 = time.perf_counter()
+        if norm2 < 1e-16:
+            spectra[(qx, qy)] = np.zeros(len(ws))
+            coeffs[(qx, qy)] = (np.zeros(0), np.zeros(0), 0.0)
+            t_basis += t2 - t1
+            t_op += t3 - t2
+            continue
+        a, b = sec.lanczos(wa, phi, nsteps, precision=precision)
+        t4 = time.perf_counter()
+        if closure is not None:
+            a, b = closure(a, b)
+        spectra[(qx, qy)] = cf_eval(a, b, ws, e0, eta, -norm2 / np.pi)
+        coeffs[(qx, qy)] = (a, b, norm2)
+        t5 = time.perf_counter()
+        t_basis += t2 - t1
+        t_op += t3 - t2
+        t_lz += t4 - t3
+        t_cf += t5 - t4
+        if not keep_sectors:
+            sec.close()
+    timings.update(final_basis=t_basis, operator=t_op, lanczos=t_lz, continued_fraction=t_cf,
+                   total=time.perf_counter() - t0)
+    return AllQResult(np.asarray(ws), distinct_qs(L, limited), spectra, coeffs, e0, timings)

@@ -113,6 +113,7 @@ struct ed_sector {
     int32_t* d_table = nullptr;
     uint64_t* d_key = nullptr;
     uint8_t* d_orbit = nullptr;
+    double* d_partial = nullptr;  // per-workgroup partials of the fused H·v dot (calls on one sector serialise on it)
 
     ed::SectorView host_view() const;  // host pointers (tables only)
     ed::SectorView view() const;       // device pointers

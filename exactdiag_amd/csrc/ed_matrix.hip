@@ -569,14 +569,8 @@ int ed_hv(const ed_sector* s, const double* w, const void* d_x, void* d_y, doubl
         hipStream_t st = static_cast<hipStream_t>(stream);
         if (s->D == 0) return;
         const HvWeights hw = make_weights(w);
-        double* partial = nullptr;
-        if (d_dot) partial = dev_alloc<double>(size_t(2 * hv_partials(s)));
-        launch_hv(s, hw, d_x, d_y, partial, precision, 1, st);
-        if (d_dot) {
-            launch_reduce_partials(partial, hv_partials(s), d_dot, st);
-            ED_HIP(hipStreamSynchronize(st));
-            dev_free(partial);
-        }
+        launch_hv(s, hw, d_x, d_y, d_dot ? s->d_partial : nullptr, precision, 1, st);
+        if (d_dot) launch_reduce_partials(s->d_partial, hv_partials(s), d_dot, st);
     });
 }
 

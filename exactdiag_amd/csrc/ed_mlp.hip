@@ -1,0 +1,427 @@
+// Learned continued-fraction closure: a small MLP trained and evaluated with
+// hand-written BF16 MFMA kernels on gfx950.  No counterpart in the reference.
+//
+// Network: n_in -> [n_hidden, GELU] x n_layers -> n_out (linear).  FP32
+// master weights and Adam state; every matrix product runs on
+// v_mfma_f32_32x32x16_bf16 with BF16 operands and FP32 accumulation.
+// Row-major layouts: activations [batch][features], weights W[k][n].
+//
+// MFMA 32x32x16 BF16 lane layout (64 lanes): lane l supplies
+// A[m = l % 32][k = 8 (l / 32) + i] and B[k = 8 (l / 32) + i][n = l % 32] for
+// i = 0..7 and receives D[m = 8 (j / 4) + 4 (l / 32) + j % 4][n = l % 32] in
+// accumulator j = 0..15.
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "ed_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace ed {
+namespace {
+
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffff) ? 0x40 : 0));  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return uint16_t(u >> 16);
+}
+
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    return __builtin_bit_cast(float, uint32_t(h) << 16);
+}
+
+// C[M][N] (fp32) = alpha * opA(A)[M][K] . opB(B)[K][N] (+ beta * C), BF16 operands
+// addressed with element strides so transposed operands need no copies.
+// One wavefront per 32x32 tile of C.
+__global__ void __launch_bounds__(64) gemm_bf16_kernel(int M, int N, int K, const uint16_t* __restrict__ A,
+                                                       int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
+                                                       int64_t sbk, int64_t sbn, float* __restrict__ C,
+                                                       int64_t ldc, int accumulate) {
+    const int lane = threadIdx.x;
+    const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+    const int r = lane & 31, half = lane >> 5;
+    f32x16 acc;
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x8 a, b;
+        for (int i = 0; i < 8; ++i) {
+            const int k = k0 + 8 * half + i;
+            const int m = m0 + r, n = n0 + r;
+            const uint16_t av = (m < M && k < K) ? A[int64_t(m) * sam + int64_t(k) * sak] : uint16_t(0);
+            const uint16_t bv = (n < N && k < K) ? B[int64_t(k) * sbk + int64_t(n) * sbn] : uint16_t(0);
+            a[i] = __builtin_bit_cast(__bf16, av);
+            b[i] = __builtin_bit_cast(__bf16, bv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    for (int j = 0; j < 16; ++j) {
+        const int m = m0 + 8 * (j >> 2) + 4 * half + (j & 3);
+        const int n = n0 + r;
+        if (m < M && n < N) {
+            float* c = &C[int64_t(m) * ldc + n];
+            *c = accumulate ? *c + acc[j] : acc[j];
+        }
+    }
+}
+
+__device__ inline float gelu(float x) {
+    const float k = 0.7978845608028654f;  // sqrt(2 / pi)
+    return 0.5f * x * (1.0f + tanhf(k * (x + 0.044715f * x * x * x)));
+}
+
+__device__ inline float gelu_grad(float x) {
+    const float k = 0.7978845608028654f;
+    const float u = k * (x + 0.044715f * x * x * x);
+    const float t = tanhf(u);
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+__global__ void to_bf16_kernel(const float* in, uint16_t* out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f32_to_bf16(in[i]);
+}
+
+// z += bias; hidden layers also write a = gelu(z) in fp32 and bf16.
+__global__ void bias_act_kernel(float* z, const float* bias, int rows, int cols, int act, float* a, uint16_t* a16) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(rows) * cols) return;
+    const float v = z[i] + bias[i % cols];
+    z[i] = v;
+    if (act) {
+        const float g = gelu(v);
+        a[i] = g;
+        a16[i] = f32_to_bf16(g);
+    }
+}
+
+// dY = 2 (Y - T) / count; per-block partial of sum (Y - T)^2.
+__global__ void mse_grad_kernel(const float* y, const float* t, int64_t n, float scale, float* dy, uint16_t* dy16,
+                                double* partial) {
+    __shared__ double red[256];
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        const float d = y[i] - t[i];
+        const float g = 2.0f * d * scale;
+        dy[i] = g;
+        dy16[i] = f32_to_bf16(g);
+        s = double(d) * double(d);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (int(threadIdx.x) < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// dZ = dA * gelu'(Z) in fp32 and bf16.
+__global__ void gelu_back_kernel(const float* da, const float* z, int64_t n, float* dz, uint16_t* dz16) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = da[i] * gelu_grad(z[i]);
+    dz[i] = g;
+    dz16[i] = f32_to_bf16(g);
+}
+
+// db[n] = sum over rows of dZ[row][n] (fixed order).
+__global__ void bias_grad_kernel(const float* dz, int rows, int cols, float* db) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; ++r) s += dz[int64_t(r) * cols + c];
+    db[c] = s;
+}
+
+__global__ void adam_kernel(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2, uint16_t* w16) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    w16[i] = f32_to_bf16(wi);
+}
+
+__global__ void reduce_double_kernel(const double* p, int n, double* out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) s += p[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (int(threadIdx.x) < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+inline unsigned nblk(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
+
+void gemm(hipStream_t st, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+          int64_t sbk, int64_t sbn, float* C, int64_t ldc, int accumulate) {
+    const dim3 grid{unsigned((N + 31) / 32), unsigned((M + 31) / 32), 1u};
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(64), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
+                       accumulate);
+}
+
+}  // namespace
+}  // namespace ed
+
+using namespace ed;
+
+struct ed_mlp {
+    std::vector<int> dims;         // n_in, hidden..., n_out
+    std::vector<int64_t> w_off;    // offset of W_l in the parameter vector
+    std::vector<int64_t> b_off;    // offset of b_l
+    int64_t n_params = 0;
+    float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
+    uint16_t* d_p16 = nullptr;
+    int64_t step = 0;
+    int cap = 0;                   // batch capacity of the activation buffers
+    std::vector<float*> z, a, dz;  // per layer (index l = 1..L)
+    std::vector<uint16_t*> a16, dz16;
+    uint16_t* x16 = nullptr;
+    float* da = nullptr;
+    double* partial = nullptr;
+    double* loss = nullptr;
+
+    int layers() const { return int(dims.size()) - 1; }
+
+    void free_acts() {
+        for (auto* p : z) dev_free(p);
+        for (auto* p : a) dev_free(p);
+        for (auto* p : dz) dev_free(p);
+        for (auto* p : a16) dev_free(p);
+        for (auto* p : dz16) dev_free(p);
+        dev_free(x16);
+        dev_free(da);
+        dev_free(partial);
+        dev_free(loss);
+        z.clear();
+        a.clear();
+        dz.clear();
+        a16.clear();
+        dz16.clear();
+        x16 = nullptr;
+        da = nullptr;
+        partial = nullptr;
+        loss = nullptr;
+        cap = 0;
+    }
+
+    void ensure(int batch) {
+        if (batch <= cap) return;
+        free_acts();
+        const int L = layers();
+        int wmax = 0;
+        for (int d : dims) wmax = std::max(wmax, d);
+        z.assign(L + 1, nullptr);
+        a.assign(L + 1, nullptr);
+        dz.assign(L + 1, nullptr);
+        a16.assign(L + 1, nullptr);
+        dz16.assign(L + 1, nullptr);
+        for (int l = 1; l <= L; ++l) {
+            z[l] = dev_alloc<float>(size_t(batch) * dims[l]);
+            a[l] = dev_alloc<float>(size_t(batch) * dims[l]);
+            dz[l] = dev_alloc<float>(size_t(batch) * dims[l]);
+            a16[l] = dev_alloc<uint16_t>(size_t(batch) * dims[l]);
+            dz16[l] = dev_alloc<uint16_t>(size_t(batch) * dims[l]);
+        }
+        x16 = dev_alloc<uint16_t>(size_t(batch) * dims[0]);
+        da = dev_alloc<float>(size_t(batch) * wmax);
+        partial = dev_alloc<double>(size_t(nblk(int64_t(batch) * dims.back())) + 1);
+        loss = dev_alloc<double>(1);
+        cap = batch;
+    }
+
+    void release() {
+        free_acts();
+        dev_free(d_p);
+        dev_free(d_g);
+        dev_free(d_m);
+        dev_free(d_v);
+        dev_free(d_p16);
+    }
+
+    // forward pass for `batch` rows of x (fp32, device); output in z[L]
+    void forward(const float* x, int batch, hipStream_t st) {
+        ensure(batch);
+        const int L = layers();
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(int64_t(batch) * dims[0])), dim3(256), 0, st, x, x16,
+                           int64_t(batch) * dims[0]);
+        const uint16_t* in16 = x16;
+        for (int l = 1; l <= L; ++l) {
+            const int K = dims[l - 1], N = dims[l];
+            gemm(st, batch, N, K, in16, K, 1, d_p16 + w_off[l - 1], N, 1, z[l], N, 0);
+            hipLaunchKernelGGL(bias_act_kernel, dim3(nblk(int64_t(batch) * N)), dim3(256), 0, st, z[l],
+                               d_p + b_off[l - 1], batch, N, l < L ? 1 : 0, a[l], a16[l]);
+            in16 = a16[l];
+        }
+    }
+
+    // forward + backward of the mean squared error; gradients into d_g
+    void backward(const float* x, const float* t, int batch, hipStream_t st) {
+        forward(x, batch, st);
+        const int L = layers();
+        const int64_t n_out = int64_t(batch) * dims[L];
+        const unsigned nb = nblk(n_out);
+        hipLaunchKernelGGL(mse_grad_kernel, dim3(nb), dim3(256), 0, st, z[L], t, n_out, 1.0f / float(n_out), dz[L],
+                           dz16[L], partial);
+        hipLaunchKernelGGL(reduce_double_kernel, dim3(1), dim3(256), 0, st, partial, int(nb), loss);
+        for (int l = L; l >= 1; --l) {
+            const int K = dims[l - 1], N = dims[l];
+            const uint16_t* in16 = l > 1 ? a16[l - 1] : x16;
+            // dW[k][n] = sum_b in[b][k] dZ[b][n]
+            gemm(st, K, N, batch, in16, 1, K, dz16[l], N, 1, d_g + w_off[l - 1], N, 0);
+            hipLaunchKernelGGL(bias_grad_kernel, dim3(nblk(N)), dim3(256), 0, st, dz[l], batch, N,
+                               d_g + b_off[l - 1]);
+            if (l > 1) {
+                // dA[b][k] = sum_n dZ[b][n] W[k][n]
+                gemm(st, batch, K, N, dz16[l], N, 1, d_p16 + w_off[l - 1], 1, N, da, K, 0);
+                hipLaunchKernelGGL(gelu_back_kernel, dim3(nblk(int64_t(batch) * K)), dim3(256), 0, st, da, z[l - 1],
+                                   int64_t(batch) * K, dz[l - 1], dz16[l - 1]);
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int ed_mlp_create(const int* dims, int n_dims, uint64_t seed, ed_mlp** out) {
+    return guarded([&] {
+        ED_REQUIRE(out && dims && n_dims >= 2, ED_ERR_ARG, "need at least input and output widths");
+        auto* m = new ed_mlp;
+        m->dims.assign(dims, dims + n_dims);
+        for (int d : m->dims) ED_REQUIRE(d >= 1 && d <= 16384, ED_ERR_ARG, "layer width out of range");
+        int64_t off = 0;
+        for (int l = 1; l < n_dims; ++l) {
+            m->w_off.push_back(off);
+            off += int64_t(m->dims[l - 1]) * m->dims[l];
+            m->b_off.push_back(off);
+            off += m->dims[l];
+        }
+        m->n_params = off;
+        // He-style initialisation on the host, seeded
+        std::vector<float> p(size_t(off), 0.0f);
+        std::mt19937_64 rng(seed);
+        for (int l = 1; l < n_dims; ++l) {
+            std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / float(m->dims[l - 1])));
+            for (int64_t i = 0; i < int64_t(m->dims[l - 1]) * m->dims[l]; ++i) p[m->w_off[l - 1] + i] = nd(rng);
+        }
+        std::vector<uint16_t> p16(p.size());
+        for (size_t i = 0; i < p.size(); ++i) p16[i] = f32_to_bf16(p[i]);
+        hipStream_t st = nullptr;
+        m->d_p = dev_upload(p, st);
+        m->d_p16 = dev_upload(p16, st);
+        std::vector<float> zero(p.size(), 0.0f);
+        m->d_g = dev_upload(zero, st);
+        m->d_m = dev_upload(zero, st);
+        m->d_v = dev_upload(zero, st);
+        ED_HIP(hipStreamSynchronize(st));
+        *out = m;
+    });
+}
+
+int ed_mlp_destroy(ed_mlp* m) {
+    return guarded([&] {
+        if (!m) return;
+        m->release();
+        delete m;
+    });
+}
+
+int64_t ed_mlp_num_params(const ed_mlp* m) { return m ? m->n_params : -1; }
+
+int ed_mlp_get_params(const ed_mlp* m, float* host, int64_t n) {
+    return guarded([&] {
+        ED_REQUIRE(m && n == m->n_params, ED_ERR_ARG, "parameter count mismatch");
+        ED_HIP(hipMemcpy(host, m->d_p, size_t(n) * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int ed_mlp_set_params(ed_mlp* m, const float* host, int64_t n) {
+    return guarded([&] {
+        ED_REQUIRE(m && n == m->n_params, ED_ERR_ARG, "parameter count mismatch");
+        std::vector<float> p(host, host + n);
+        std::vector<uint16_t> p16(p.size());
+        for (size_t i = 0; i < p.size(); ++i) p16[i] = f32_to_bf16(p[i]);
+        ED_HIP(hipMemcpy(m->d_p, p.data(), size_t(n) * sizeof(float), hipMemcpyHostToDevice));
+        ED_HIP(hipMemcpy(m->d_p16, p16.data(), size_t(n) * sizeof(uint16_t), hipMemcpyHostToDevice));
+        std::vector<float> zero(p.size(), 0.0f);
+        ED_HIP(hipMemcpy(m->d_m, zero.data(), size_t(n) * sizeof(float), hipMemcpyHostToDevice));
+        ED_HIP(hipMemcpy(m->d_v, zero.data(), size_t(n) * sizeof(float), hipMemcpyHostToDevice));
+        m->step = 0;
+    });
+}
+
+int ed_mlp_forward(ed_mlp* m, const float* d_x, int batch, float* d_y, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && batch >= 1, ED_ERR_ARG, "bad batch");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        m->forward(d_x, batch, st);
+        ED_HIP(hipMemcpyAsync(d_y, m->z[m->layers()], size_t(batch) * m->dims.back() * sizeof(float),
+                              hipMemcpyDeviceToDevice, st));
+        ED_HIP(hipGetLastError());
+    });
+}
+
+int ed_mlp_gradients(ed_mlp* m, const float* d_x, const float* d_t, int batch, float* host_grad, double* loss,
+                     void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && batch >= 1, ED_ERR_ARG, "bad batch");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        m->backward(d_x, d_t, batch, st);
+        ED_HIP(hipGetLastError());
+        if (host_grad)
+            ED_HIP(hipMemcpyAsync(host_grad, m->d_g, size_t(m->n_params) * sizeof(float), hipMemcpyDeviceToHost, st));
+        double l = 0.0;
+        ED_HIP(hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, st));
+        ED_HIP(hipStreamSynchronize(st));
+        if (loss) *loss = l / double(int64_t(batch) * m->dims.back());
+    });
+}
+
+int ed_mlp_train_step(ed_mlp* m, const float* d_x, const float* d_t, int batch, float lr, double* loss,
+                      void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && batch >= 1, ED_ERR_ARG, "bad batch");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        m->backward(d_x, d_t, batch, st);
+        m->step += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - std::pow(b1, float(m->step)), c2 = 1.0f - std::pow(b2, float(m->step));
+        hipLaunchKernelGGL(adam_kernel, dim3(nblk(m->n_params)), dim3(256), 0, st, m->d_p, m->d_g, m->d_m, m->d_v,
+                           m->n_params, lr, b1, b2, 1e-8f, c1, c2, m->d_p16);
+        ED_HIP(hipGetLastError());
+        if (loss) {
+            double l = 0.0;
+            ED_HIP(hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+            *loss = l / double(int64_t(batch) * m->dims.back());
+        }
+    });
+}
+
+// C = A . B with BF16 operands given as raw uint16 and element strides.
+int ed_gemm_bf16(int M, int N, int K, const uint16_t* d_A, int64_t sam, int64_t sak, const uint16_t* d_B, int64_t sbk,
+                 int64_t sbn, float* d_C, int64_t ldc, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(M >= 0 && N >= 0 && K >= 0, ED_ERR_ARG, "negative GEMM size");
+        if (M == 0 || N == 0) return;
+        gemm(static_cast<hipStream_t>(stream), M, N, K, d_A, sam, sak, d_B, sbk, sbn, d_C, ldc, 0);
+        ED_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

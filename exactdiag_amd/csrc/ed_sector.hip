@@ -348,7 +348,7 @@ ed::SectorView ed_sector::view() const {
 void ed_sector::release() {
     for (void* p : {(void*)d_comb, (void*)d_amp, (void*)d_sqrt, (void*)d_canon_gmask, (void*)d_canon_block,
                     (void*)d_block_of_rank, (void*)d_block_base, (void*)d_block_table, (void*)d_block_hole,
-                    (void*)d_table, (void*)d_key, (void*)d_orbit})
+                    (void*)d_table, (void*)d_key, (void*)d_orbit, (void*)d_partial})
         dev_free(p);
     d_comb = nullptr;
     d_amp = d_sqrt = nullptr;
@@ -359,6 +359,7 @@ void ed_sector::release() {
     d_table = nullptr;
     d_key = nullptr;
     d_orbit = nullptr;
+    d_partial = nullptr;
 }
 
 extern "C" {
@@ -536,6 +537,7 @@ int ed_basis_build(ed_sector* s, void* stream) {
         s->d_key = dev_alloc<uint64_t>(size_t(s->D));
         s->d_orbit = dev_alloc<uint8_t>(size_t(s->D));
         s->d_table = dev_alloc<int32_t>(size_t(std::max<int64_t>(s->table_size, 1)));
+        s->d_partial = dev_alloc<double>(size_t(2 * ((s->D + 255) / 256)));
         v = s->view();
         hipLaunchKernelGGL(basis_fill_kernel, grid, dim3(SCAN_THREADS), 0, st, v, flags, d_off, cpb, s->d_key,
                            s->d_orbit, s->d_table);

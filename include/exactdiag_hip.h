@@ -127,6 +127,28 @@ int ed_vec_combine(const double* d_V, int nv, int64_t ld, const double* c, doubl
 int ed_vec_norm2(const double* d_w, int64_t D, double* out, void* stream);
 int ed_vec_scale(double* d_w, int64_t D, double cr, double ci, void* stream);
 
+/* Learned continued-fraction closure (no counterpart in the reference): an
+ * MLP dims[0] -> dims[1] ... -> dims[n-1] with GELU on hidden layers, BF16
+ * weights in MFMA GEMMs with FP32 accumulation, FP32 master weights and
+ * Adam.  Parameters are laid out as W_1 (dims[0] x dims[1], row-major), b_1,
+ * W_2, b_2, ...  Inputs/targets are FP32 row-major [batch][features] on the
+ * device.  The loss is the mean squared error over all outputs. */
+typedef struct ed_mlp ed_mlp;
+int ed_mlp_create(const int* dims, int n_dims, uint64_t seed, ed_mlp** out);
+int ed_mlp_destroy(ed_mlp* m);
+int64_t ed_mlp_num_params(const ed_mlp* m);
+int ed_mlp_get_params(const ed_mlp* m, float* host, int64_t n);
+int ed_mlp_set_params(ed_mlp* m, const float* host, int64_t n);
+int ed_mlp_forward(ed_mlp* m, const float* d_x, int batch, float* d_y, void* stream);
+int ed_mlp_gradients(ed_mlp* m, const float* d_x, const float* d_t, int batch, float* host_grad, double* loss,
+                     void* stream);
+int ed_mlp_train_step(ed_mlp* m, const float* d_x, const float* d_t, int batch, float lr, double* loss,
+                      void* stream);
+/* C[M][N] = A[M][K] . B[K][N] with BF16 operands (raw 16-bit, element
+ * strides) and FP32 output, on v_mfma_f32_32x32x16_bf16. */
+int ed_gemm_bf16(int M, int N, int K, const uint16_t* d_A, int64_t sam, int64_t sak, const uint16_t* d_B, int64_t sbk,
+                 int64_t sbn, float* d_C, int64_t ldc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -793,6 +793,34 @@ int orc_build_term_csr(void* p, int term, int threads) {
     }
 }
 
+// The reference's stored CSR of one term restricted to rows [lo, hi) of the
+// row-major upper triangle (a bounded sample for the CPU baseline).
+int orc_build_term_csr_rows(void* p, int term, int64_t lo, int64_t hi, int threads) {
+    try {
+        auto* hd = static_cast<Handle*>(p);
+        const Sector& s = *hd->sec;
+        Rule r = hamiltonian_rule(s.N, term);
+        const int64_t n = hi - lo;
+        std::vector<std::vector<cplx>> cv(n);
+        std::vector<std::vector<int64_t>> ci(n);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+        for (long long k = 0; k < n; ++k) column(s, s, r, true, lo + k, cv[k], ci[k]);
+        Columns c;
+        c.ptr.assign(n + 1, 0);
+        for (int64_t k = 0; k < n; ++k) c.ptr[k + 1] = c.ptr[k] + int64_t(ci[k].size());
+        for (int64_t k = 0; k < n; ++k) {
+            c.rows.insert(c.rows.end(), ci[k].begin(), ci[k].end());
+            c.vals.insert(c.vals.end(), cv[k].begin(), cv[k].end());
+        }
+        hd->ham.term[term] = to_csr(c, true);
+        hd->ham.built[term] = true;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 int64_t orc_term_csr_sizes(void* p, int term, int64_t* n_unique) {
     auto& m = static_cast<Handle*>(p)->ham.term[term];
     *n_unique = int64_t(m.values.size());

@@ -70,6 +70,7 @@ def lib():
         L.orc_op_columns.argtypes = [ct.c_void_p, ct.c_void_p] + [ct.c_int] * 4 + [_i64p, _i64p, _dp]
         L.orc_op_weights.argtypes = [ct.c_int] * 4 + [_dp]
         L.orc_build_term_csr.argtypes = [ct.c_void_p, ct.c_int, ct.c_int]
+        L.orc_build_term_csr_rows.argtypes = [ct.c_void_p, ct.c_int, ct.c_int64, ct.c_int64, ct.c_int]
         L.orc_term_csr_sizes.restype = ct.c_int64
         L.orc_term_csr_sizes.argtypes = [ct.c_void_p, ct.c_int, _i64p]
         L.orc_term_csr_copy.argtypes = [ct.c_void_p, ct.c_int, _dp, _i16p, _i64p, _i64p]
@@ -103,6 +104,8 @@ def ref():
         R.ref_rank.argtypes = [ct.c_void_p, _i16p]
         R.ref_orbit.argtypes = [ct.c_void_p, _i16p, ct.c_int, _i16p, _dp]
         R.ref_lowest.argtypes = [ct.c_void_p, _i16p, _i64p, _dp]
+        R.ref_csr_dot_rows.argtypes = [ct.c_int64, ct.c_int64, ct.c_int64, _i64p, _i16p, _i64p, _dp, _dp, _dp,
+                                       ct.c_double, ct.c_double, ct.c_int]
         R.ref_csr_dot.argtypes = [ct.c_int64, _i64p, _i16p, _i64p, _dp, _dp, _dp, ct.c_double, ct.c_double,
                                   ct.c_int, ct.c_int]
     return _ref
@@ -193,6 +196,12 @@ class OracleSector:
             if self._lib.orc_build_term_csr(self._h, t, threads or default_threads()) != 0:
                 raise IndexError(self._lib.orc_last_error().decode())
             self._terms_built.add(t)
+
+    def build_csr_rows(self, term, lo, hi, threads=None):
+        """Stored CSR of one term for rows [lo, hi) only (replaces any full build of that term)."""
+        if self._lib.orc_build_term_csr_rows(self._h, term, int(lo), int(hi), threads or default_threads()) != 0:
+            raise IndexError(self._lib.orc_last_error().decode())
+        self._terms_built.add(term)
 
     def term_csr(self, term):
         self.build_csr((term,))
@@ -319,3 +328,11 @@ def ref_csr_dot(values, vinds, minor, major, x, y, weight, first, threads=None):
     R.ref_csr_dot(D, _ptr(major, _i64p), _ptr(vinds, _i16p), _ptr(minor, _i64p),
                   _ptr(values.view(np.float64), _dp), _ptr(x.view(np.float64), _dp), _ptr(y.view(np.float64), _dp),
                   float(np.real(weight)), float(np.imag(weight)), int(first), threads or default_threads())
+
+
+def ref_csr_dot_rows(D, row_lo, values, vinds, minor, major, x, y, weight, threads=None):
+    R = ref()
+    R.ref_csr_dot_rows(D, row_lo, len(major) - 1, _ptr(major, _i64p), _ptr(vinds, _i16p), _ptr(minor, _i64p),
+                       _ptr(values.view(np.float64), _dp), _ptr(x.view(np.float64), _dp),
+                       _ptr(y.view(np.float64), _dp), float(np.real(weight)), float(np.imag(weight)),
+                       threads or default_threads())

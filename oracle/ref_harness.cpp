@@ -139,4 +139,37 @@ void ref_csr_dot(int64_t D, const int64_t* major, const short* value_inds, const
     }
 }
 
+// Rows [row_lo, row_lo + n_rows) of a row-major Hermitian-triangle CSR
+// (major has n_rows + 1 entries): the explicit pass for those rows and their
+// Hermitian-conjugate scatter, each thread owning a range of output
+// indices as in general/cython_sparse_dot.pyx:58-79.
+void ref_csr_dot_rows(int64_t D, int64_t row_lo, int64_t n_rows, const int64_t* major, const short* value_inds,
+                      const int64_t* minor, const double* values, const double* x, double* y, double wr, double wi,
+                      int threads) {
+    using c = std::complex<double>;
+    const c* vals = reinterpret_cast<const c*>(values);
+    const c* vin = reinterpret_cast<const c*>(x);
+    c* vout = reinterpret_cast<c*>(y);
+    const c w(wr, wi);
+#pragma omp parallel num_threads(threads)
+    {
+#pragma omp for schedule(dynamic, 10000)
+        for (long long k = 0; k < n_rows; ++k)
+            vout[row_lo + k] += w * matvec_explicit_elements_pass_row_major(k, (const long long*)major, vals,
+                                                                            value_inds, (const long long*)minor, vin);
+        const long long nt = omp_get_num_threads(), t = omp_get_thread_num();
+        const long long lo = std::min<long long>(D, (D / nt + 1) * t);
+        const long long hi = t == nt - 1 ? D : std::min<long long>(D, (D / nt + 1) * (t + 1));
+        for (long long k = 0; k < n_rows; ++k) {
+            const long long row = row_lo + k;
+            for (long long j = major[k]; j < major[k + 1]; ++j) {
+                const long long col = minor[j];
+                if (col >= hi) break;
+                if (col < lo || col == row) continue;
+                vout[col] += vin[row] * std::conj(vals[value_inds[j]]) * w;
+            }
+        }
+    }
+}
+
 }  // extern "C"
