@@ -14,6 +14,7 @@
 //    per-workgroup partial of <x, y> feeds the Lanczos coefficient a_n.
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "ed_internal.h"
@@ -376,6 +377,180 @@ __global__ void __launch_bounds__(BS) hv_kernel(SectorView s, HvWeights w, const
     }
 }
 
+// Change of the spin rank when the spin pattern Dn becomes D2 by a swap
+// inside the site window [lo, hi] (no other site changes): only downs in the
+// window change their colex term, downs above keep their index.
+__device__ inline int64_t window_rank_delta(uint32_t H, uint32_t Dn, uint32_t D2, int lo, int hi,
+                                            const uint32_t* comb) {
+    const uint32_t below = (1u << lo) - 1u;
+    int kb = popc32(Dn & below), ka = kb, hb = popc32(H & below);
+    int64_t before = 0, after = 0;
+    for (int q = lo; q <= hi; ++q) {
+        const uint32_t bit = 1u << q;
+        if (H & bit) {
+            ++hb;
+            continue;
+        }
+        const int c = q - hb;
+        if (Dn & bit) before += comb[(++kb) * COMB_W + c];
+        if (D2 & bit) after += comb[(++ka) * COMB_W + c];
+    }
+    return after - before;
+}
+
+template <class VT>
+__device__ inline void accumulate_at(const SectorView& s, const VT* __restrict__ x, int64_t j, int nj, uint32_t Hsrc,
+                                     int p, double scale, C2 W, C2& acc) {
+    const C2 a = amp_at(s, p, nj);
+    const double sg = double(image_sign(Hsrc, p, s.L, s.ne_even, s.full)) * scale;
+    const C2 v{sg * a.re, -(sg * a.im)};
+    const C2 t = cmul(v, load_c(x, j));
+    acc = cadd(acc, cmul(W, t));
+}
+
+// H·v with every lane of a wavefront walking the bonds in the same fixed
+// order.  Consecutive rows of a hole-pattern block share the colex high part
+// of their spin pattern, so an exchange across two high sites sends all lanes
+// to one contiguous range of x (coalesced), and exchanges on low sites stay
+// inside the same small slice.  On blocks whose hole pattern has no
+// stabiliser, the exchange target is row + (window rank change): no search,
+// no full rank.
+template <class VT, int BS>
+__global__ void __launch_bounds__(BS) hv_lockstep_kernel(SectorView s, HvWeights w, const VT* __restrict__ x,
+                                                         VT* __restrict__ y, double* __restrict__ partial,
+                                                         int remap) {
+    int bid = blockIdx.x;
+    if (remap) {
+        const int nwg = gridDim.x;
+        const int per = nwg / 8;
+        if (bid < per * 8) bid = (bid & 7) * per + (bid >> 3);
+    }
+    const int64_t f = int64_t(bid) * BS + threadIdx.x;
+    C2 acc{0.0, 0.0}, dot{0.0, 0.0};
+    if (f < s.D) {
+        const uint64_t key = s.key[f];
+        const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
+        const uint32_t full = s.full;
+        const uint32_t E = ~H & full, Up = E & ~Dn;
+        const double sqf = s.sqrt_tab[s.orbit[f]];
+        const C2 xf = load_c(x, f);
+        const int64_t hr = rank_holes(H, s.comb);
+        const uint32_t gm_self = s.canon_gmask[hr];
+        const int blk_self = s.canon_block[hr];
+        const bool generic = (gm_self & (gm_self - 1u)) == 0u;
+        auto exchange = [&](uint32_t D2, int lo, int hi, bool wrap, C2 W) {
+            if (generic) {
+                const int64_t j = wrap ? s.block_base[blk_self] + rank_spins(H, D2, s.comb)
+                                       : f + window_rank_delta(H, Dn, D2, lo, hi, s.comb);
+                accumulate_at(s, x, j, s.N, H, 0, 0.5 * sqf, W, acc);
+                return;
+            }
+            uint32_t gm = gm_self;
+            int bp = ctz32(gm);
+            uint32_t best = image_mask(D2, bp, s.N, full);
+            gm &= gm - 1;
+            while (gm) {
+                const int p = ctz32(gm);
+                const uint32_t d = image_mask(D2, p, s.N, full);
+                if (d < best) {
+                    best = d;
+                    bp = p;
+                }
+                gm &= gm - 1;
+            }
+            accumulate(s, x, H, bp, blk_self, H, best, 0.5 * sqf, W, acc);
+        };
+        if (w.active & 4) {  // jl: bonds (i, i + 2)
+            const uint32_t d2 = rotl_n(Dn, s.N - 2, s.N, full), u2 = rotl_n(Up, s.N - 2, s.N, full);
+            uint32_t anti = ((Dn & u2) | (Up & d2)) & full;
+            if (s.N == 2) anti = 0;
+            const C2 W = w.w[2];
+            const double diag = -0.5 * double(popc32(anti));
+            acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+            for (int i = 0; i < s.N; ++i) {
+                if (!((anti >> i) & 1u)) continue;
+                const int k = (i + 2) % s.N;
+                const bool wrap = k < i;
+                exchange(Dn ^ (1u << i) ^ (1u << k), wrap ? k : i, wrap ? i : k, wrap || s.N == 4, W);
+            }
+        }
+        if (w.active & 8) {  // jr: bonds (2x, 2x + 1)
+            const uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
+            const C2 W = w.w[3];
+            const double diag = -0.5 * double(popc32(anti));
+            acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+            for (int i = 0; i < s.N; i += 2) {
+                if (!((anti >> i) & 1u)) continue;
+                exchange(Dn ^ (3u << i), i, i + 1, false, W);
+            }
+        }
+        if (w.active & 3) {
+            uint32_t holes = H;
+            while (holes) {
+                const int t = ctz32(holes);
+                holes &= holes - 1;
+                for (int dir = 0; dir < 3; ++dir) {
+                    int src;
+                    C2 W;
+                    if (dir < 2) {
+                        if (!(w.active & 1)) continue;
+                        src = dir == 0 ? (t + 2) % s.N : (t - 2 + s.N) % s.N;
+                        W = w.w[0];
+                    } else {
+                        if (!(w.active & 2)) continue;
+                        src = t ^ 1;
+                        W = w.w[1];
+                    }
+                    if (!(E & (1u << src))) continue;
+                    const uint32_t bt = 1u << t, bs = 1u << src;
+                    const uint32_t H2 = H ^ bt ^ bs;
+                    const uint32_t D2 = (Dn & bs) ? (Dn ^ bs ^ bt) : Dn;
+                    const uint32_t above_s = src >= 31 ? 0u : (~0u << (src + 1));
+                    const uint32_t above_t = t >= 31 ? 0u : (~0u << (t + 1));
+                    const int par = (popc32(E & above_s) + popc32(E & above_t) + (t < src ? 1 : 0)) & 1;
+                    const int64_t hr2 = rank_holes(H2, s.comb);
+                    const int blk = s.canon_block[hr2];
+                    if (blk < 0) continue;
+                    uint32_t gm = s.canon_gmask[hr2];
+                    int bp = ctz32(gm);
+                    const uint32_t Hc = image_mask(H2, bp, s.N, full);
+                    uint32_t best = image_mask(D2, bp, s.N, full);
+                    gm &= gm - 1;
+                    while (gm) {
+                        const int p = ctz32(gm);
+                        const uint32_t d = image_mask(D2, p, s.N, full);
+                        if (d < best) {
+                            best = d;
+                            bp = p;
+                        }
+                        gm &= gm - 1;
+                    }
+                    accumulate(s, x, H2, bp, blk, Hc, best, par ? sqf : -sqf, W, acc);
+                }
+            }
+        }
+        store_c(y, f, acc);
+        dot = cmul(C2{xf.re, -xf.im}, acc);
+    }
+    if (partial) {
+        __shared__ double red[2 * BS];
+        red[2 * threadIdx.x] = dot.re;
+        red[2 * threadIdx.x + 1] = dot.im;
+        __syncthreads();
+        for (int st = BS / 2; st > 0; st >>= 1) {
+            if (int(threadIdx.x) < st) {
+                red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
+                red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            partial[2 * bid] = red[0];
+            partial[2 * bid + 1] = red[1];
+        }
+    }
+}
+
 // Szq and N(q) in gather form: final row j has the same representative as
 // initial column i; the element is the reference's per-site sum for column i
 // (matrix_setup.pyx:235-246), computed with the same operations.
@@ -511,12 +686,25 @@ void launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, d
     constexpr int BS = 256;
     const int64_t nwg = (s->D + BS - 1) / BS;
     if (nwg == 0) return;
-    if (precision == 0)
-        hipLaunchKernelGGL((hv_kernel<double2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(), w,
-                           static_cast<const double2*>(x), static_cast<double2*>(y), partial, remap);
-    else
-        hipLaunchKernelGGL((hv_kernel<float2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(), w,
-                           static_cast<const float2*>(x), static_cast<float2*>(y), partial, remap);
+    static const int variant = [] {
+        const char* e = std::getenv("ED_HV_KERNEL");
+        return e ? std::atoi(e) : 1;
+    }();
+    if (variant == 0) {
+        if (precision == 0)
+            hipLaunchKernelGGL((hv_kernel<double2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(), w,
+                               static_cast<const double2*>(x), static_cast<double2*>(y), partial, remap);
+        else
+            hipLaunchKernelGGL((hv_kernel<float2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(), w,
+                               static_cast<const float2*>(x), static_cast<float2*>(y), partial, remap);
+    } else {
+        if (precision == 0)
+            hipLaunchKernelGGL((hv_lockstep_kernel<double2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(),
+                               w, static_cast<const double2*>(x), static_cast<double2*>(y), partial, remap);
+        else
+            hipLaunchKernelGGL((hv_lockstep_kernel<float2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(),
+                               w, static_cast<const float2*>(x), static_cast<float2*>(y), partial, remap);
+    }
     ED_HIP(hipGetLastError());
 }
 

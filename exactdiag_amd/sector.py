@@ -45,6 +45,8 @@ def weights_array(weights) -> np.ndarray:
     """Four complex term weights (tl, tr, jl, jr) as 8 doubles."""
     if isinstance(weights, dict):
         weights = [weights.get(t, 0.0) for t in TERMS]
+    if isinstance(weights, np.ndarray) and weights.dtype == np.float64 and weights.shape == (8,):
+        return weights.copy()  # already interleaved (re, im) pairs
     w = np.zeros(4, np.complex128)
     w[: len(weights)] = np.asarray(weights, np.complex128)
     return w.view(np.float64).copy()

@@ -75,11 +75,18 @@ def test_lanczos_keeps_basis_and_stops_on_invariant_subspace(oracle_mod, torch):
     p = (3, 1, 2, 0, 0)  # D = 10
     g = Sector(*p)
     w = [1, 1, 0.4, 0.4]
-    v0 = torch.ones(g.dim, dtype=torch.complex128, device="cuda")
+    rng = np.random.default_rng(4)
+    v0 = torch.as_tensor(rng.normal(size=g.dim) + 1j * rng.normal(size=g.dim), device="cuda")
     a, b, V = g.lanczos(w, v0, 30, keep_basis=True)
-    assert len(a) <= g.dim  # stopped when b < 1e-8
-    G = (V.conj() @ V.T).cpu().numpy()
-    assert np.max(np.abs(G[:4, :4] - np.eye(4))) < 1e-12
+    assert 4 <= len(a) <= g.dim  # the Krylov space is exhausted: stopped when b < 1e-8
+    G = (V[:4].conj() @ V[:4].T).cpu().numpy()
+    assert np.max(np.abs(G - np.eye(4))) < 1e-12
+    # an eigenvector as start vector stops after one step, as the reference's loop does
+    H = np.stack([g.hv(w, torch.eye(g.dim, dtype=torch.complex128, device="cuda")[:, k].contiguous()).cpu().numpy()
+                  for k in range(g.dim)], axis=1)
+    vals, vecs = np.linalg.eigh(H)
+    a1, b1 = g.lanczos(w, torch.as_tensor(vecs[:, 0], device="cuda"), 30)
+    assert len(a1) == 1 and abs(a1[0] - vals[0]) < 1e-12
 
 
 def _lehmann(H, gs, O, e0, ws, eta):
