@@ -147,13 +147,31 @@ __global__ void lz_reduce_b_kernel(LzDev lz, const double* partial, int64_t n, i
     }
 }
 
-// general/cython_lanczos_diagonalization.pyx:42-48 per omega.
+// general/cython_lanczos_diagonalization.pyx:42-48 per omega.  With b_inf > 0
+// the last level is closed by the tail of a constant continuation (a_inf,
+// b_inf): t(z) = (z - a_inf - sqrt((z - a_inf)^2 - 4 b_inf^2)) / (2 b_inf^2),
+// the branch with Im t <= 0, so det_{n-1} = z - a_{n-1} - b_inf^2 t(z).
 __global__ void cf_kernel(const double* a, const double* b, int n, const double* ws, int nw, double e0, double eta,
-                          double factor, double* out) {
+                          double factor, double a_inf, double b_inf, double* out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nw) return;
     const double zr = ws[k] + e0, zi = eta;
     double dr = zr - a[n - 1], di = zi;
+    if (b_inf > 0.0) {
+        const double ur = zr - a_inf, ui = zi;
+        // w = u^2 - 4 b^2, s = sqrt(w) (principal), t = (u - s) / (2 b^2); pick the sign with Im t <= 0
+        const double wr = ur * ur - ui * ui - 4.0 * b_inf * b_inf, wi = 2.0 * ur * ui;
+        const double mod = std::sqrt(wr * wr + wi * wi);
+        double sr = std::sqrt(0.5 * (mod + wr));
+        double si = std::copysign(std::sqrt(0.5 * (mod - wr)), wi);
+        double tr = (ur - sr) / (2.0 * b_inf * b_inf), ti = (ui - si) / (2.0 * b_inf * b_inf);
+        if (ti > 0.0) {
+            tr = (ur + sr) / (2.0 * b_inf * b_inf);
+            ti = (ui + si) / (2.0 * b_inf * b_inf);
+        }
+        dr -= b_inf * b_inf * tr;
+        di -= b_inf * b_inf * ti;
+    }
     for (int i = 1; i < n; ++i) {
         const double bb = b[n - i] * b[n - i];
         const double den = dr * dr + di * di;
@@ -251,8 +269,8 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         ED_HIP(hipGetLastError());
         int flag[2] = {0, 0};
         for (int n = 0; n < nsteps; ++n) {
-            launch_hv(s, hw, u, y, partial, precision, 1, st);
-            hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(BS), 0, st, lz, partial, hv_partials(s), n);
+            const int64_t np = launch_hv(s, hw, u, y, partial, precision, 1, st);
+            hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(BS), 0, st, lz, partial, np, n);
             hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, lz, n, D, y, u, up,
                                d_basis ? d_basis + int64_t(n) * D : nullptr, partial);
             hipLaunchKernelGGL(lz_reduce_b_kernel, dim3(1), dim3(BS), 0, st, lz, partial, nwg, n);
@@ -305,8 +323,16 @@ int ed_lanczos_run(const ed_sector* s, const double* w, const double* d_v0, int 
     });
 }
 
+int ed_cf_eval_tail(const double* a, const double* b, int n, const double* ws, int n_w, double e0, double eta,
+                    double factor, double a_inf, double b_inf, double* out);
+
 int ed_cf_eval(const double* a, const double* b, int n, const double* ws, int n_w, double e0, double eta,
                double factor, double* out) {
+    return ed_cf_eval_tail(a, b, n, ws, n_w, e0, eta, factor, 0.0, 0.0, out);
+}
+
+int ed_cf_eval_tail(const double* a, const double* b, int n, const double* ws, int n_w, double e0, double eta,
+                    double factor, double a_inf, double b_inf, double* out) {
     return guarded([&] {
         ED_REQUIRE(n >= 1 && n_w >= 0, ED_ERR_ARG, "bad continued-fraction sizes");
         if (n_w == 0) return;
@@ -317,7 +343,7 @@ int ed_cf_eval(const double* a, const double* b, int n, const double* ws, int n_
         double* dw = dev_upload(wv, st);
         double* dout = dev_alloc<double>(size_t(n_w));
         hipLaunchKernelGGL(cf_kernel, dim3(unsigned((n_w + 127) / 128)), dim3(128), 0, st, da, db, n, dw, n_w, e0, eta,
-                           factor, dout);
+                           factor, a_inf, b_inf, dout);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpy(out, dout, size_t(n_w) * sizeof(double), hipMemcpyDeviceToHost);
         dev_free(da);

@@ -11,8 +11,10 @@ struct HvWeights {
 };
 
 HvWeights make_weights(const double* w);
-void launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
-               int remap, hipStream_t st);
+int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
+                  int remap, hipStream_t st);
+int64_t launch_hv_lds(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
+                      hipStream_t st);
 int64_t hv_partials(const ed_sector* s);
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st);
 

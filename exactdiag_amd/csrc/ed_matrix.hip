@@ -680,16 +680,23 @@ void run_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rul
 
 }  // namespace
 
-// Launch helper shared with the Lanczos driver.
-void launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
-               int remap, hipStream_t st) {
+// Launch helper shared with the Lanczos driver; returns the number of
+// workgroup partials written.  ED_HV_KERNEL selects a kernel for
+// measurements: 2 (default) LDS tables + persistent workgroups, 1 lock-step
+// bonds, 0 per-lane bond loop; 2 falls back to 1 when the tables exceed LDS.
+int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
+                  int remap, hipStream_t st) {
     constexpr int BS = 256;
     const int64_t nwg = (s->D + BS - 1) / BS;
-    if (nwg == 0) return;
+    if (nwg == 0) return 0;
     static const int variant = [] {
         const char* e = std::getenv("ED_HV_KERNEL");
-        return e ? std::atoi(e) : 1;
+        return e ? std::atoi(e) : 2;
     }();
+    if (variant == 2) {
+        const int64_t n = launch_hv_lds(s, w, x, y, partial, precision, st);
+        if (n >= 0) return n;
+    }
     if (variant == 0) {
         if (precision == 0)
             hipLaunchKernelGGL((hv_kernel<double2, BS>), dim3(unsigned(nwg)), dim3(BS), 0, st, s->view(), w,
@@ -706,6 +713,7 @@ void launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y, d
                                w, static_cast<const float2*>(x), static_cast<float2*>(y), partial, remap);
     }
     ED_HIP(hipGetLastError());
+    return nwg;
 }
 
 int64_t hv_partials(const ed_sector* s) { return (s->D + 255) / 256; }
@@ -757,8 +765,8 @@ int ed_hv(const ed_sector* s, const double* w, const void* d_x, void* d_y, doubl
         hipStream_t st = static_cast<hipStream_t>(stream);
         if (s->D == 0) return;
         const HvWeights hw = make_weights(w);
-        launch_hv(s, hw, d_x, d_y, d_dot ? s->d_partial : nullptr, precision, 1, st);
-        if (d_dot) launch_reduce_partials(s->d_partial, hv_partials(s), d_dot, st);
+        const int64_t np = launch_hv(s, hw, d_x, d_y, d_dot ? s->d_partial : nullptr, precision, 1, st);
+        if (d_dot) launch_reduce_partials(s->d_partial, np, d_dot, st);
     });
 }
 

@@ -184,12 +184,16 @@ class Sector:
         return a[:k].copy(), b[:k].copy()
 
 
-def cf_eval(a, b, ws, e0, eta, factor):
-    """factor * Im(1/det(z)) for z = ws + e0 + i*eta on the GPU."""
+def cf_eval(a, b, ws, e0, eta, factor, tail=None):
+    """factor * Im(1/det(z)) for z = ws + e0 + i*eta on the GPU.
+
+    ``tail=(a_inf, b_inf)`` closes the fraction with the square-root terminator.
+    """
     a, ap = _dbl(a)
     b, bp = _dbl(b)
     ws, wsp = _dbl(ws)
     out = np.empty(len(ws), np.float64)
-    check(lib().ed_cf_eval(ap, bp, len(a), wsp, len(ws), float(e0), float(eta), float(factor),
-                           out.ctypes.data_as(ct.POINTER(ct.c_double))))
+    a_inf, b_inf = (0.0, 0.0) if tail is None else (float(tail[0]), float(tail[1]))
+    check(lib().ed_cf_eval_tail(ap, bp, len(a), wsp, len(ws), float(e0), float(eta), float(factor), a_inf, b_inf,
+                                out.ctypes.data_as(ct.POINTER(ct.c_double))))
     return out

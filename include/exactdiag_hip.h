@@ -116,6 +116,12 @@ int ed_lanczos_run(const ed_sector* s, const double* w, const double* d_v0, int 
  * e0 + i*eta.  Host arrays in and out. */
 int ed_cf_eval(const double* a, const double* b, int n, const double* ws, int n_w, double e0, double eta,
                double factor, double* out);
+/* As ed_cf_eval, with the fraction closed after level n-1 by the tail of a
+ * constant continuation (a_inf, b_inf) (square-root terminator; b_inf = 0
+ * gives ed_cf_eval).  No counterpart in the reference: used by the
+ * learned-closure form. */
+int ed_cf_eval_tail(const double* a, const double* b, int n, const double* ws, int n_w, double e0, double eta,
+                    double factor, double a_inf, double b_inf, double* out);
 
 /* Dense vector helpers for the Krylov eigensolver (all c128, device).
  * ed_vec_dots: out[k] = <V_k, w> for k < nv (host out, FP64 deterministic).

@@ -1,0 +1,51 @@
+"""The learned closure against the method without it (held-out systems).
+
+The trained weights (exactdiag_amd/data/closure_mlp.npz) come from
+``python -m exactdiag_amd.train_closure``; the test regenerates a few
+1000-step FP64 sequences of systems outside the training set and checks
+that closing the 64-step continued fraction with the network is more
+accurate, against the 1000-step spectrum, than closing it with the
+square-root terminator (the method without the network).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def model():
+    from exactdiag_amd import closure
+
+    if not closure.MODEL_PATH.exists():
+        pytest.skip("closure model not trained yet")
+    return closure.Closure.load()
+
+
+def test_learned_closure_beats_terminator(model):
+    from exactdiag_amd import closure
+    from exactdiag_amd.allq import spectral_l1
+    from exactdiag_amd.sector import cf_eval
+
+    ws = np.linspace(0, 4, 2001)
+    eta = 0.05
+    gains = []
+    # held out: J values and fillings not drawn for training
+    for L, h, J in ((8, 2, 0.33), (10, 2, 0.71)):
+        seqs = closure.lanczos_sequences(L, h, (2 * L - h) // 2, [1.0, 1.0, J, J], nsteps=1000, ops=("Szq",))
+        for s in seqs:
+            if not closure.usable(s["a"], s["b"]):
+                continue
+            f = -s["norm2"] / np.pi
+            ref = cf_eval(s["a"], s["b"], ws, s["e0"], eta, f)
+            at, bt, tail = closure.sqrt_terminator(s["a"], s["b"])
+            ae, be, tail2 = model.extend(s["a"], s["b"])
+            e_term = spectral_l1(cf_eval(at, bt, ws, s["e0"], eta, f, tail=tail), ref, ws)
+            e_nn = spectral_l1(cf_eval(ae, be, ws, s["e0"], eta, f, tail=tail2), ref, ws)
+            gains.append(e_term / e_nn)
+            print(f"L={L} h={h} J={J} q=({s['qx']},{s['qy']}): terminator {e_term:.3e}, learned {e_nn:.3e}",
+                  flush=True)
+    gains = np.array(gains)
+    assert len(gains) >= 8
+    assert np.exp(np.mean(np.log(gains))) > 1.0  # geometric-mean error ratio in favour of the network
