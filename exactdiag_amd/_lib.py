@@ -23,6 +23,8 @@ EXPORTED = [
     "ed_lanczos_run", "ed_cf_eval", "ed_cf_eval_tail", "ed_vec_dots", "ed_vec_axpys", "ed_vec_combine", "ed_vec_norm2", "ed_vec_scale",
     "ed_mlp_create", "ed_mlp_destroy", "ed_mlp_num_params", "ed_mlp_get_params", "ed_mlp_set_params",
     "ed_mlp_forward", "ed_mlp_gradients", "ed_mlp_train_step", "ed_gemm_bf16",
+    "ed_batch_create", "ed_batch_destroy", "ed_batch_offsets", "ed_batch_hv", "ed_batch_lanczos", "ed_batch_tune",
+    "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune",
 ]
 
 _vp = ct.c_void_p
@@ -61,6 +63,15 @@ _SIGNATURES = {
     "ed_mlp_gradients": (ct.c_int, [_vp, _vp, _vp, ct.c_int, ct.POINTER(ct.c_float), _dp, _vp]),
     "ed_mlp_train_step": (ct.c_int, [_vp, _vp, _vp, ct.c_int, ct.c_float, _dp, _vp]),
     "ed_gemm_bf16": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp]),
+    "ed_batch_create": (ct.c_int, [ct.POINTER(_vp), _dp, ct.c_int, ct.POINTER(_vp)]),
+    "ed_batch_destroy": (ct.c_int, [_vp]),
+    "ed_batch_offsets": (ct.c_int, [_vp, ct.POINTER(_i64)]),
+    "ed_batch_hv": (ct.c_int, [_vp, _vp, _vp, _dp, _vp]),
+    "ed_batch_lanczos": (ct.c_int, [_vp, _vp, _ip, ct.c_int, _dp, _vp, _dp, _dp, _ip, ct.c_int, _vp]),
+    "ed_batch_tune": (ct.c_int, [_vp, _vp, _vp, _ip, _ip, _dp, _vp]),
+    "ed_batch_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
+    "ed_batch_get_config": (ct.c_int, [_vp, _ip, _ip, _ip]),
+    "ed_hv_tune": (ct.c_int, [_vp, _dp, ct.c_int, _vp, _vp, _ip, _ip, _dp, _vp]),
 }
 
 _lib = None

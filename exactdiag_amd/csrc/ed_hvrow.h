@@ -1,0 +1,300 @@
+// One row of the gather-form H·v, shared by the single-sector kernel and the
+// batched kernels (gfx950).
+//
+// Row f of y = H x in the gather form y_f = sum_j conj(H_jf) x_j, H_jf the
+// reference's matrix element <j~|H|f~> (general/column_functions.h:99-148 for
+// one element, tJ_spin_half_ladder/matrix_setup.pyx:80-118 for the four
+// terms).  The element computation reads only small per-sector tables
+// (binomials, amplitude table, sqrt(n), per-hole-pattern group elements and
+// blocks, block offsets); the kernels stage them in LDS, or point at their
+// copies in global memory when they do not fit.  Ranks are 32-bit.  An
+// exchange on a hole pattern without stabiliser gets its target as row +
+// window rank change, with the same coefficient for every bond of a term.
+// Gathers of x_j are issued G at a time before any is consumed.
+#pragma once
+
+#include "ed_internal.h"
+#include "ed_matrix.h"
+#include "ed_orbit.h"
+
+namespace ed {
+
+struct LdsLayout {
+    int comb, amp, sq, gm, cb, bb, bt, bytes;
+};
+
+inline LdsLayout lds_layout(const ed_sector* s) {
+    LdsLayout l{};
+    int off = 0;
+    auto take = [&](int bytes) {
+        const int at = off;
+        off += (bytes + 15) & ~15;
+        return at;
+    };
+    l.comb = take(COMB_W * COMB_W * 4);
+    l.amp = take(s->L * 2 * (s->N + 1) * 16);
+    l.sq = take((s->N + 1) * 8);
+    l.gm = take(int(s->n_hole) * 4);
+    l.cb = take(int(s->n_hole) * 4);
+    l.bb = take(int(s->block_rank.size()) * 8);
+    l.bt = take(int(s->block_rank.size()) * 8);
+    l.bytes = off;
+    return l;
+}
+
+constexpr int LDS_CAP = 60 * 1024;
+
+struct Tabs {
+    const uint32_t* comb;
+    const double* amp;
+    const double* sq;
+    const uint32_t* gm;
+    const int32_t* cb;
+    const int64_t* bb;
+    const int64_t* bt;
+};
+
+__device__ inline Tabs global_tabs(const SectorView& s) {
+    return Tabs{s.comb, s.amp, s.sqrt_tab, s.canon_gmask, s.canon_block, s.block_base, s.block_table};
+}
+
+// Copy the tables into LDS (all threads of the workgroup take part; the
+// caller synchronises before use).
+template <int BS>
+__device__ inline Tabs stage_tabs(const SectorView& s, const LdsLayout& lay, unsigned char* lds) {
+    uint32_t* s_comb = reinterpret_cast<uint32_t*>(lds + lay.comb);
+    double* s_amp = reinterpret_cast<double*>(lds + lay.amp);
+    double* s_sq = reinterpret_cast<double*>(lds + lay.sq);
+    uint32_t* s_gm = reinterpret_cast<uint32_t*>(lds + lay.gm);
+    int32_t* s_cb = reinterpret_cast<int32_t*>(lds + lay.cb);
+    int64_t* s_bb = reinterpret_cast<int64_t*>(lds + lay.bb);
+    int64_t* s_bt = reinterpret_cast<int64_t*>(lds + lay.bt);
+    const int t = threadIdx.x;
+    for (int i = t; i < COMB_W * COMB_W; i += BS) s_comb[i] = s.comb[i];
+    const int namp = s.L * 2 * (s.N + 1) * 2;
+    for (int i = t; i < namp; i += BS) s_amp[i] = s.amp[i];
+    for (int i = t; i <= s.N; i += BS) s_sq[i] = s.sqrt_tab[i];
+    for (int i = t; i < s.n_hole_ranks; i += BS) {
+        s_gm[i] = s.canon_gmask[i];
+        s_cb[i] = s.canon_block[i];
+    }
+    for (int i = t; i < s.n_blocks; i += BS) {
+        s_bb[i] = s.block_base[i];
+        s_bt[i] = s.block_table[i];
+    }
+    return Tabs{s_comb, s_amp, s_sq, s_gm, s_cb, s_bb, s_bt};
+}
+
+__device__ inline int32_t rank_holes32(uint32_t H, const uint32_t* comb) {
+    int32_t r = 0;
+    int k = 1;
+    while (H) {
+        r += int32_t(comb[k * COMB_W + ctz32(H)]);
+        ++k;
+        H &= H - 1;
+    }
+    return r;
+}
+
+__device__ inline int32_t rank_spins32(uint32_t H, uint32_t Dn, const uint32_t* comb) {
+    int32_t r = 0;
+    int k = 1;
+    while (Dn) {
+        const int p = ctz32(Dn);
+        r += int32_t(comb[k * COMB_W + p - popc32(H & ((1u << p) - 1u))]);
+        ++k;
+        Dn &= Dn - 1;
+    }
+    return r;
+}
+
+// Spin-rank change when the down pattern Dn becomes D2 and the two differ
+// only inside sites [lo, hi].
+__device__ inline int32_t window_delta32(uint32_t H, uint32_t Dn, uint32_t D2, int lo, int hi, const uint32_t* comb) {
+    const uint32_t below = (1u << lo) - 1u;
+    int kb = popc32(Dn & below), ka = kb, hb = popc32(H & below);
+    int32_t d = 0;
+    for (int q = lo; q <= hi; ++q) {
+        const uint32_t bit = 1u << q;
+        if (H & bit) {
+            ++hb;
+            continue;
+        }
+        const int c = q - hb;
+        if (Dn & bit) d -= int32_t(comb[(++kb) * COMB_W + c]);
+        if (D2 & bit) d += int32_t(comb[(++ka) * COMB_W + c]);
+    }
+    return d;
+}
+
+// One matrix element prepared for the gather: target row and the weighted
+// conjugate amplitude c, so that acc += c * x[j].
+struct Elem {
+    int64_t j;
+    C2 c;
+};
+
+__device__ inline C2 weighted_conj(const Tabs& T, int N, int L, int ne_even, uint32_t full, uint32_t Hsrc, int p,
+                                   int nj, double scale, C2 W) {
+    const int idx = ((p >> 1) * 2 + (p & 1)) * (N + 1) + nj;
+    const double ar = T.amp[2 * idx], ai = T.amp[2 * idx + 1];
+    const double sg = double(image_sign(Hsrc, p, L, ne_even, full)) * scale;
+    return cmul(W, C2{sg * ar, -(sg * ai)});
+}
+
+// Lowest image among the group elements gm (hole part already minimal),
+// then its row: generic blocks by rank, symmetric blocks by table.
+__device__ inline bool locate(const SectorView& s, const Tabs& T, uint32_t Hsrc, uint32_t D2, uint32_t gm, int blk,
+                              int& p_out, int64_t& j_out, int& nj_out) {
+    int bp = ctz32(gm);
+    const uint32_t Hc = image_mask(Hsrc, bp, s.N, s.full);
+    uint32_t best = image_mask(D2, bp, s.N, s.full);
+    gm &= gm - 1;
+    while (gm) {
+        const int p = ctz32(gm);
+        const uint32_t d = image_mask(D2, p, s.N, s.full);
+        if (d < best) {
+            best = d;
+            bp = p;
+        }
+        gm &= gm - 1;
+    }
+    const int32_t sr = rank_spins32(Hc, best, T.comb);
+    const int64_t off = T.bt[blk];
+    p_out = bp;
+    if (off < 0) {
+        j_out = T.bb[blk] + sr;
+        nj_out = s.N;
+        return true;
+    }
+    const int32_t j = s.table[off + sr];
+    if (j < 0) return false;
+    j_out = j;
+    nj_out = int(s.orbit[j]);
+    return true;
+}
+
+template <class VT>
+__device__ inline C2 ldx(const VT* __restrict__ x, int64_t j) {
+    const VT v = x[j];
+    return {double(v.x), double(v.y)};
+}
+
+template <class VT, int G>
+__device__ inline void flush(const VT* __restrict__ x, const Elem (&e)[G], C2& acc) {
+    C2 v[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) v[k] = ldx(x, e[k].j);
+#pragma unroll
+    for (int k = 0; k < G; ++k) acc = cadd(acc, cmul(e[k].c, v[k]));
+}
+
+// (H x)_f; also returns x_f for the fused <x, Hx>.  Exchange elements are
+// gathered G at a time, hops three at a time (the three bonds onto a hole).
+template <class VT, int G>
+__device__ inline C2 hv_row(const SectorView& s, const Tabs& T, const HvWeights& w, const VT* __restrict__ x,
+                            int64_t f, C2& xf) {
+    const int N = s.N, L = s.L, ne_even = s.ne_even;
+    const uint32_t full = s.full;
+    const uint64_t key = s.key[f];
+    const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
+    const uint32_t E = ~H & full, Up = E & ~Dn;
+    const double sqf = T.sq[s.orbit[f]];
+    xf = ldx(x, f);
+    const int32_t hr = rank_holes32(H, T.comb);
+    const uint32_t gm_self = T.gm[hr];
+    const int blk_self = T.cb[hr];
+    const bool generic = (gm_self & (gm_self - 1u)) == 0u;
+    C2 acc{0.0, 0.0};
+    // an exchange element: generic blocks by window rank change (coefficient cg, the same for every bond of the
+    // term), others by the stabiliser search
+    auto exchange = [&](uint32_t D2, int lo, int hi, bool wrap, C2 W, C2 cg, Elem& e) {
+        if (generic) {
+            e.j = wrap ? T.bb[blk_self] + rank_spins32(H, D2, T.comb) : f + window_delta32(H, Dn, D2, lo, hi, T.comb);
+            e.c = cg;
+            return;
+        }
+        int p, nj;
+        int64_t j;
+        if (locate(s, T, H, D2, gm_self, blk_self, p, j, nj)) {
+            e.j = j;
+            e.c = weighted_conj(T, N, L, ne_even, full, H, p, nj, 0.5 * sqf, W);
+        }
+    };
+    if (w.active & 4) {  // jl: bonds (i, i + 2)
+        const uint32_t d2 = rotl_n(Dn, N - 2, N, full), u2 = rotl_n(Up, N - 2, N, full);
+        const uint32_t anti = N == 2 ? 0u : (((Dn & u2) | (Up & d2)) & full);
+        const C2 W = w.w[2];
+        const C2 cg = weighted_conj(T, N, L, ne_even, full, H, 0, N, 0.5 * sqf, W);
+        const double diag = -0.5 * double(popc32(anti));
+        acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+        for (int g = 0; g < N; g += G) {
+            Elem e[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                e[k] = Elem{f, C2{0.0, 0.0}};
+                const int i = g + k;
+                if (i < N && ((anti >> i) & 1u)) {
+                    const int kk = i + 2 < N ? i + 2 : i + 2 - N;
+                    const bool wrap = kk < i;
+                    exchange(Dn ^ (1u << i) ^ (1u << kk), wrap ? kk : i, wrap ? i : kk, wrap, W, cg, e[k]);
+                }
+            }
+            flush<VT, G>(x, e, acc);
+        }
+    }
+    if (w.active & 8) {  // jr: bonds (2x, 2x + 1)
+        const uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
+        const C2 W = w.w[3];
+        const C2 cg = weighted_conj(T, N, L, ne_even, full, H, 0, N, 0.5 * sqf, W);
+        const double diag = -0.5 * double(popc32(anti));
+        acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+        for (int g = 0; g < L; g += G) {
+            Elem e[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                e[k] = Elem{f, C2{0.0, 0.0}};
+                const int i = 2 * (g + k);
+                if (g + k < L && ((anti >> i) & 1u)) exchange(Dn ^ (3u << i), i, i + 1, false, W, cg, e[k]);
+            }
+            flush<VT, G>(x, e, acc);
+        }
+    }
+    if (w.active & 3) {  // hops onto each hole: leg neighbours (tl) and rung partner (tr)
+        uint32_t holes = H;
+        while (holes) {
+            const int t = ctz32(holes);
+            holes &= holes - 1;
+            Elem e[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                e[k] = Elem{f, C2{0.0, 0.0}};
+                if (k < 2 && !(w.active & 1)) continue;
+                if (k == 2 && !(w.active & 2)) continue;
+                const int src = k == 0 ? (t + 2 < N ? t + 2 : t + 2 - N) : k == 1 ? (t >= 2 ? t - 2 : t - 2 + N)
+                                                                                   : (t ^ 1);
+                if (!(E & (1u << src))) continue;
+                const C2 W = k < 2 ? w.w[0] : w.w[1];
+                const uint32_t bt = 1u << t, bs = 1u << src;
+                const uint32_t H2 = H ^ bt ^ bs;
+                const uint32_t D2 = (Dn & bs) ? (Dn ^ bs ^ bt) : Dn;
+                const uint32_t above_s = src >= 31 ? 0u : (~0u << (src + 1));
+                const uint32_t above_t = t >= 31 ? 0u : (~0u << (t + 1));
+                const int par = (popc32(E & above_s) + popc32(E & above_t) + (t < src ? 1 : 0)) & 1;
+                const int32_t hr2 = rank_holes32(H2, T.comb);
+                const int blk = T.cb[hr2];
+                if (blk < 0) continue;
+                int p, nj;
+                int64_t j;
+                if (locate(s, T, H2, D2, T.gm[hr2], blk, p, j, nj)) {
+                    e[k].j = j;
+                    e[k].c = weighted_conj(T, N, L, ne_even, full, H2, p, nj, par ? sqf : -sqf, W);
+                }
+            }
+            flush<VT, 3>(x, e, acc);
+        }
+    }
+    return acc;
+}
+
+}  // namespace ed

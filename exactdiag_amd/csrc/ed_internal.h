@@ -100,6 +100,8 @@ struct ed_sector {
     std::vector<uint8_t> block_sym;
     int64_t D = -1;
     int64_t table_size = 0;
+    int tune_tiles = 0;   // H·v row tiles per workgroup (0 = default heuristic)
+    int tune_gather = 0;  // H·v gathers in flight per lane (0 = 4)
 
     uint32_t* d_comb = nullptr;
     double* d_amp = nullptr;

@@ -103,12 +103,20 @@ __global__ void lz_reduce_a_kernel(LzDev lz, const double* partial, int64_t n, i
 template <class VT>
 __global__ void __launch_bounds__(BS) lz_update_kernel(LzDev lz, int step, int64_t D, const VT* __restrict__ y,
                                                        const VT* __restrict__ u, VT* __restrict__ up,
-                                                       VT* __restrict__ basis, double* partial) {
+                                                       VT* __restrict__ basis, double2* __restrict__ acc,
+                                                       const double* __restrict__ ritz, double* partial) {
     if (lz.flag[0]) return;
     const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
     const double sn = lz.s[step], an = lz.a[step], bn = lz.b[step];
     const double snm = step > 0 ? lz.s[step - 1] : 0.0;
     double n2 = 0.0;
+    if (acc && i < D) {  // second pass: x += y_n v_n in FP64
+        double ur, ui;
+        ldc(u, i, ur, ui);
+        const double c = ritz[step] * sn;
+        const double2 av = acc[i];
+        acc[i] = double2{av.x + c * ur, av.y + c * ui};
+    }
     if (i < D) {
         double yr, yi, ur, ui, pr, pi;
         ldc(y, i, yr, yi);
@@ -241,9 +249,18 @@ __global__ void scale_kernel(double* w, int64_t D, double cr, double ci) {
     w[2 * i + 1] = cr * im + ci * re;
 }
 
+__global__ void zero2_kernel(double2* p, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) p[i] = double2{0.0, 0.0};
+}
+
+// acc (optional, c128) receives sum_n ritz[n] v_n: the second pass of
+// two-pass Lanczos (the recurrence is deterministic, so it sees the vectors
+// of the first pass).
 template <class VT>
 void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
-                   int* n_out, VT* d_basis, int precision, hipStream_t st) {
+                   int* n_out, VT* d_basis, int precision, hipStream_t st, double2* acc = nullptr,
+                   const double* ritz = nullptr) {
     const int64_t D = s->D;
     const int64_t nwg = (D + BS - 1) / BS;
     const HvWeights hw = make_weights(w);
@@ -251,6 +268,7 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
     double* partial = dev_alloc<double>(size_t(2 * std::max<int64_t>(nwg, hv_partials(s))));
     LzDev lz{dev_alloc<double>(size_t(nsteps)), dev_alloc<double>(size_t(nsteps + 1)),
              dev_alloc<double>(size_t(nsteps + 1)), dev_alloc<int>(2)};
+    double* d_ritz = nullptr;
     auto cleanup = [&] {
         for (auto* p : buf) dev_free(p);
         dev_free(partial);
@@ -258,11 +276,17 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         dev_free(lz.b);
         dev_free(lz.s);
         dev_free(lz.flag);
+        dev_free(d_ritz);
     };
     try {
         VT* u = buf[0];
         VT* up = buf[1];
         VT* y = buf[2];
+        if (acc) {
+            d_ritz = dev_alloc<double>(size_t(nsteps));
+            ED_HIP(hipMemcpyAsync(d_ritz, ritz, size_t(nsteps) * sizeof(double), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(zero2_kernel, dim3(unsigned(nwg)), dim3(BS), 0, st, acc, D);
+        }
         ED_HIP(hipMemsetAsync(up, 0, size_t(D) * sizeof(VT), st));
         hipLaunchKernelGGL((copy_in_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, d_v0, u, D, partial);
         hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(BS), 0, st, lz, partial, nwg);
@@ -272,7 +296,7 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
             const int64_t np = launch_hv(s, hw, u, y, partial, precision, 1, st);
             hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(BS), 0, st, lz, partial, np, n);
             hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, lz, n, D, y, u, up,
-                               d_basis ? d_basis + int64_t(n) * D : nullptr, partial);
+                               d_basis ? d_basis + int64_t(n) * D : nullptr, acc, d_ritz, partial);
             hipLaunchKernelGGL(lz_reduce_b_kernel, dim3(1), dim3(BS), 0, st, lz, partial, nwg, n);
             ED_HIP(hipGetLastError());
             VT* t = up;  // w (u_{n+1}) now lives in up
@@ -287,7 +311,7 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
         ED_HIP(hipStreamSynchronize(st));
         const int cnt = flag[1];
-        if (cnt > 0) {
+        if (cnt > 0 && a && b) {
             ED_HIP(hipMemcpyAsync(a, lz.a, size_t(cnt) * sizeof(double), hipMemcpyDeviceToHost, st));
             ED_HIP(hipMemcpyAsync(b, lz.b, size_t(cnt) * sizeof(double), hipMemcpyDeviceToHost, st));
             ED_HIP(hipStreamSynchronize(st));
@@ -320,6 +344,23 @@ int ed_lanczos_run(const ed_sector* s, const double* w, const double* d_v0, int 
             lanczos_run_t<double2>(s, w, d_v0, nsteps, a, b, n_out, reinterpret_cast<double2*>(d_basis), 0, st);
         else
             lanczos_run_t<float2>(s, w, d_v0, nsteps, a, b, n_out, reinterpret_cast<float2*>(d_basis), 1, st);
+    });
+}
+
+int ed_lanczos_ritz(const ed_sector* s, const double* w, const double* d_v0, int nsteps, const double* ritz,
+                    double* d_x, int precision, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
+        ED_REQUIRE(nsteps >= 1 && ritz && d_x, ED_ERR_ARG, "bad Ritz-vector arguments");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        int n = 0;
+        auto* acc = reinterpret_cast<double2*>(d_x);
+        if (precision == 0)
+            lanczos_run_t<double2>(s, w, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 0, st, acc, ritz);
+        else
+            lanczos_run_t<float2>(s, w, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 1, st, acc, ritz);
     });
 }
 

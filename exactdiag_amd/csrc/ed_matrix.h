@@ -16,6 +16,7 @@ int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y
 int64_t launch_hv_lds(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
                       hipStream_t st);
 int64_t hv_partials(const ed_sector* s);
+int hv_default_tiles(const ed_sector* s);
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st);
 
 }  // namespace ed

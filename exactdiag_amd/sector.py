@@ -156,6 +156,18 @@ class Sector:
             return y, complex(dot.cpu().numpy()[0])
         return y
 
+    def tune(self, weights, precision: int = 0):
+        """Autotune this sector's H·v (tiles per workgroup, gathers in flight); returns (tiles, gather, ms)."""
+        torch = _torch()
+        dt = torch.complex128 if precision == 0 else torch.complex64
+        x = torch.ones(max(self.dim, 1), dtype=dt, device=self.device)
+        y = torch.empty_like(x)
+        w, wp = _dbl(weights_array(weights))
+        t, g, ms = ct.c_int(0), ct.c_int(0), ct.c_double(0.0)
+        check(self._lib.ed_hv_tune(self._h, wp, int(precision), _ptr(x), _ptr(y), ct.byref(t), ct.byref(g),
+                                   ct.byref(ms), _stream()))
+        return t.value, g.value, ms.value
+
     def op_apply(self, final: "Sector", op: int, qx: int, qy: int, x):
         torch = _torch()
         x = x.to(torch.complex128).contiguous()

@@ -39,15 +39,30 @@ def systems(seed=0, n_j=3):
 
 
 def generate(path, nsteps=1000, seed=0):
+    """Every system of ``systems``; the file is rewritten after each system, a failing system is reported and
+    skipped."""
     seqs = []
     t0 = time.perf_counter()
     for L, h, nd, J, ops in systems(seed):
         w = [1.0, 1.0, J, J]
-        s = closure.lanczos_sequences(L, h, nd, w, nsteps=nsteps, ops=ops)
+        try:
+            s = closure.lanczos_sequences(L, h, nd, w, nsteps=nsteps, ops=ops)
+        except (RuntimeError, ValueError) as exc:
+            print(f"L={L} h={h} J={J:.3f}: skipped ({exc})", flush=True)
+            continue
         seqs.extend(s)
+        save_sequences(path, seqs)
         print(f"L={L} h={h} J={J:.3f}: {len(s)} sequences, total {len(seqs)}, {time.perf_counter() - t0:.1f}s",
               flush=True)
+    return seqs
+
+
+def generate_eval(path, nsteps=1000):
+    """The held-out evaluation set: 2x12 ladder, 2 holes, S^z = 0, t = 1, J = 0.4, S^z(q) for every distinct q."""
+    t0 = time.perf_counter()
+    seqs = closure.lanczos_sequences(12, 2, 11, [1.0, 1.0, 0.4, 0.4], nsteps=nsteps, ops=("Szq",))
     save_sequences(path, seqs)
+    print(f"eval set: {len(seqs)} sequences in {time.perf_counter() - t0:.1f}s", flush=True)
     return seqs
 
 
@@ -102,8 +117,11 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--eval", type=pathlib.Path, help="npz of held-out sequences (all-q run) to evaluate on")
     ap.add_argument("--model", type=pathlib.Path, default=closure.MODEL_PATH)
+    ap.add_argument("--gen-eval", type=pathlib.Path, help="write the 2x12, 2-hole evaluation sequences here")
     args = ap.parse_args(argv)
     ws = np.linspace(0, 4, 2001)
+    if args.gen_eval:
+        generate_eval(args.gen_eval)
     if args.data and not args.train:
         generate(args.data)
     if args.train:

@@ -111,6 +111,44 @@ int ed_op_apply(const ed_sector* init, const ed_sector* fin, int op, int qx, int
 int ed_lanczos_run(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
                    int* n_out, double* d_basis, int precision, void* stream);
 
+/* Batched Lanczos over many sectors of uneven size (instances), each with
+ * its own weights (8 doubles per instance as in ed_hv).  Replaces the
+ * one-sector-at-a-time loops of get_all_k_eigenpairs
+ * (tJ_spin_half_ladder/api.py:45-59) and of the spectra drivers
+ * (api.py:116-161), each of which runs calculate_eigenpairs or
+ * lanczos_dynamical_response (general/lanczos_diagonalization.py:53-121,
+ * general/cython_lanczos_diagonalization.pyx:18-77) per sector.
+ * Vectors of all instances form one c128 pool; instance i owns rows
+ * [offsets[i], offsets[i + 1]) (ed_batch_offsets writes n + 1 values).
+ * The sectors must outlive the batch. */
+typedef struct ed_batch ed_batch;
+int ed_batch_create(ed_sector* const* sectors, const double* weights, int n, ed_batch** out);
+int ed_batch_destroy(ed_batch* b);
+int ed_batch_offsets(const ed_batch* b, int64_t* offsets);
+/* d_y = H d_x for every instance; dots (host, 2n doubles, optional) gets
+ * <x_i, H x_i> per instance. */
+int ed_batch_hv(ed_batch* b, const double* d_x, double* d_y, double* dots, void* stream);
+/* Lanczos on every instance, nsteps[i] <= ld steps from the pool vector d_v0
+ * (any norm), stopping an instance when b < 1e-8, all steps of all
+ * instances in one HIP graph launch when use_graph.  a[i*ld + n] and
+ * b[i*(ld+1) + n] (b_0 = 0; b at n_out[i] is the norm of the last
+ * residual); n_out[i] = steps done.  With ritz (host, n*ld) and d_acc
+ * (pool), d_acc = sum_n ritz[i*ld + n] v_n (second pass of two-pass
+ * Lanczos; the recurrence is deterministic). */
+int ed_batch_lanczos(ed_batch* b, const double* d_v0, const int* nsteps, int ld, const double* ritz, double* d_acc,
+                     double* a, double* bcoef, int* n_out, int use_graph, void* stream);
+/* Autotuner (no counterpart in the reference): time the batched H·v over
+ * the tiles per workgroup (1..32) and gathers in flight per lane (2, 4, 8),
+ * keep the fastest.  ed_batch_set_config / get_config set and read it. */
+int ed_batch_tune(ed_batch* b, const double* d_x, double* d_y, int* tiles, int* gather, double* best_ms,
+                  void* stream);
+int ed_batch_set_config(ed_batch* b, int tiles, int gather);
+int ed_batch_get_config(const ed_batch* b, int* tiles, int* gather, int* n_jobs);
+/* The same autotuner for the single-sector H·v of ed_hv / ed_lanczos_run;
+ * the choice is stored in the sector. */
+int ed_hv_tune(ed_sector* s, const double* w, int precision, const double* d_x, double* d_y, int* tiles, int* gather,
+               double* best_ms, void* stream);
+
 /* Continued fraction of general/cython_lanczos_diagonalization.pyx:42-48
  * evaluated on the GPU: out[k] = factor * Im(1 / det(z_k)) for z_k = ws[k] +
  * e0 + i*eta.  Host arrays in and out. */
