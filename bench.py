@@ -1,4 +1,4 @@
-"""Benchmark: matrix-free t-J H·v throughput on one MI355X.
+"""Benchmark: t-J H·v throughput on one MI355X.
 
 Workload: 2x12 ladder (24 sites, periodic along the legs), 2 holes, S^z = 0
 (11 down spins), k = (0, 0) even-parity sector, t_leg = t_rung = 1,
@@ -7,11 +7,14 @@ one H·v of the whole sector with the fused <x, Hx> reduction the Lanczos
 recurrence uses (the hot path of every Lanczos step).  Inputs are seeded
 synthetic vectors already resident in HBM.
 
-Prints one JSON line with the driver's fields plus ``roofline`` (the H·v
-kernel against the 8 TB/s HBM peak, timed with HIP events on the launch
-stream) and ``cpu_baseline`` (the reference's own sparse-dot kernels, from
-oracle/_ref, timed on the host's cores over a bounded row sample of the same
-sector's stored matrix).
+The timed H·v is the stored form (sliced-ELL H built once on the GPU before
+the timed region, as the reference builds its CSR matrices once before its
+Lanczos loop); the matrix-free kernel is timed too and reported under
+``matrix_free``.  Prints one JSON line with the driver's fields plus
+``roofline`` (the H·v kernel against the 8 TB/s HBM peak, timed with HIP
+events on the launch stream) and ``cpu_baseline`` (the reference's own
+sparse-dot kernels, from oracle/_ref, timed on the host's cores over a
+bounded row sample of the same sector's stored matrix).
 """
 
 from __future__ import annotations
@@ -40,12 +43,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per H·v launch from the committed rocprofv3 PMC summary, if any."""
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC summary, if any."""
     p = ROOT / "profiles" / "hv_2x12_h2_traffic.md"
     if not p.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    m = re.search(r"traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]:\s*([0-9.eE+]+)", p.read_text())
     return float(m.group(1)) if m else None
 
 
@@ -67,7 +70,8 @@ def cpu_baseline(threads):
     for t in range(4):
         o.build_csr_rows(t, 0, rows, threads=threads)
         csr.append(o.term_csr(t))
-    log(f"cpu baseline: built sample CSR in {time.perf_counter() - t0:.1f}s")
+    t_build = time.perf_counter() - t0
+    log(f"cpu baseline: built sample CSR in {t_build:.1f}s")
     rng = np.random.default_rng(0)
     x = rng.normal(size=D) + 1j * rng.normal(size=D)
     y = np.zeros(D, np.complex128)
@@ -93,10 +97,87 @@ def cpu_baseline(threads):
         "unit": "Gstates/s",
         "cores": threads,
         "kind": kind,
+        "csr_build_s_per_row": t_build / rows,
         "sample": (f"reference sparse-dot kernels (general/cpp_sparse_dot.h) over rows [0, {rows}) of the "
                    f"2x12 h2 k=(0,0) sector's stored 4-term upper-triangle CSR (explicit + Hermitian passes), "
                    f"{reps} repetitions, {t_total:.1f}s"),
     }
+
+
+ALLQ_WS = (0.0, 4.0, 2001)
+ALLQ_ETA = 0.05
+ALLQ_STEPS_REF = 1000
+ALLQ_STEPS_PLAIN = 256
+ALLQ_STEPS_FAST = 150
+ALLQ_TARGET = 1e-3
+
+
+def allq_section():
+    """All-q S(q, w) of the same ladder (14 distinct q) to <= 1e-3 spectral L1 error: plain FP64 matrix-free
+    Lanczos with truncated fraction vs the fast form (stored H, complex64 vectors, square-root terminator);
+    both from a fresh start (bases, ground state at the reference's tolerance, every q), against the
+    1000-step FP64 spectra."""
+    import numpy as np
+
+    from exactdiag_amd import allq
+
+    ws = np.linspace(*ALLQ_WS)
+    t0 = time.perf_counter()
+    e0, ref = allq.allq_reference(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_REF)
+    log(f"all-q reference ({ALLQ_STEPS_REF} steps) in {time.perf_counter() - t0:.1f}s, E0 = {e0:.12f}")
+    t_plain, s_plain = allq.allq_plain(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_PLAIN)
+    log(f"all-q plain in {t_plain:.2f}s")
+    t_fast, s_fast = allq.allq_fast(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
+    log(f"all-q fast in {t_fast:.2f}s")
+    err_plain = {f"{q[0]},{q[1]}": allq.spectral_l1(s_plain[q], ref[q], ws) for q in ref}
+    err_fast = {f"{q[0]},{q[1]}": allq.spectral_l1(s_fast[q], ref[q], ws) for q in ref}
+    return {
+        "workload": "S(q,w) for the 14 distinct q of the 2x12 ladder, 2 holes, Sz=0, k=(0,0), t=1, J=0.4; "
+                    "eta=0.05, w in [0,4] on 2001 points; L1 = sum|S-S_ref|dw / sum|S_ref|dw vs 1000-step FP64",
+        "target_l1": ALLQ_TARGET,
+        "plain": {"seconds": t_plain, "steps": ALLQ_STEPS_PLAIN, "max_l1": max(err_plain.values()),
+                  "method": "complex128, matrix-free H·v, ground state by thick-restart Lanczos at tol 2e-7, "
+                            "truncated continued fraction, q one after the other"},
+        "fast": {"seconds": t_fast, "steps": ALLQ_STEPS_FAST, "max_l1": max(err_fast.values()),
+                 "method": "stored sliced-ELL H per sector, complex64 vectors with FP64 coefficient reductions, "
+                           "square-root terminator"},
+        "speedup_vs_plain": t_plain / t_fast,
+        "l1_plain": err_plain,
+        "l1_fast": err_fast,
+    }
+
+
+def cpu_allq_estimate(cpu, section, D):
+    """Reference CPU path for the same all-q run, from the measured sparse-dot rate: 15 sector CSR builds
+    (timed on the sample rows, scaled to D rows) + the plain method's products (ground state + 14 x steps)."""
+    if not cpu.get("value"):
+        return None
+    n_hv = 14 * section["plain"]["steps"] + 200  # ARPACK ground state: ~200 products, measured ~150-250 here
+    dot_s = n_hv * D / (cpu["value"] * 1e9)
+    build_s = 15 * cpu.get("csr_build_s_per_row", 0.0) * D
+    total = dot_s + build_s
+    return {"seconds": total, "dot_seconds": dot_s, "csr_build_seconds": build_s, "cores": cpu["cores"],
+            "method": "extrapolated: products at the measured reference sparse-dot rate, CSR builds at the "
+                      "measured build rate of the sample rows (oracle restatement of the reference's column "
+                      "functions), disk I/O and Python overhead of the reference not counted",
+            "speedup_fast_vs_cpu": total / section["fast"]["seconds"]}
+
+
+def time_steps(step, steps, warmup, stream):
+    """W untimed steps, then exactly K steps between synchronisations; (wall s, HIP-event ms) of the K."""
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t1 = time.perf_counter()
+    start.record(stream)
+    for _ in range(steps):
+        step()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t1, start.elapsed_time(end)
 
 
 def main():
@@ -106,6 +187,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
+    ap.add_argument("--no-allq", action="store_true", help="skip the all-q time-to-accuracy section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy one vector (known bytes) for PMC calibration")
     args = ap.parse_args()
@@ -114,10 +196,13 @@ def main():
               f"this build", file=sys.stderr)
         sys.exit(2)
 
+    import ctypes as ct
+
     import numpy as np
     import torch
 
-    from exactdiag_amd.sector import Sector
+    from exactdiag_amd._lib import check, lib
+    from exactdiag_amd.sector import Sector, weights_array
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -127,44 +212,47 @@ def main():
     t_basis = time.perf_counter() - t0
     D = sec.dim
     log(f"basis D={D} built in {t_basis:.3f}s")
+    t0 = time.perf_counter()
+    hm = sec.store(WEIGHTS)
+    torch.cuda.synchronize()
+    t_store = time.perf_counter() - t0
+    log(f"stored H: {hm.nnz} elements ({hm.nnz / D:.2f} per row), {hm.slots} slots, {hm.n_values} unique values, "
+        f"{hm.bytes / 1e9:.3f} GB, built in {t_store:.3f}s")
     rng = np.random.default_rng(0)
     dt = torch.complex128 if args.precision == "c128" else torch.complex64
     x = torch.as_tensor(rng.normal(size=D) + 1j * rng.normal(size=D), device="cuda").to(dt)
     y = torch.empty_like(x)
     dot = torch.zeros(1, dtype=torch.complex128, device="cuda")
-    import ctypes as ct
-
-    from exactdiag_amd._lib import check, lib
-    from exactdiag_amd.sector import weights_array
-
     w = weights_array(WEIGHTS)
     wp = w.ctypes.data_as(ct.POINTER(ct.c_double))
     prec = 0 if dt == torch.complex128 else 1
     stream = torch.cuda.current_stream()
     sp = ct.c_void_p(stream.cuda_stream)
+    xp, yp, dp = ct.c_void_p(x.data_ptr()), ct.c_void_p(y.data_ptr()), ct.c_void_p(dot.data_ptr())
 
-    def step():
-        check(lib().ed_hv(sec.handle, wp, ct.c_void_p(x.data_ptr()), ct.c_void_p(y.data_ptr()),
-                          ct.c_void_p(dot.data_ptr()), prec, sp))
+    def step_stored():
+        check(lib().ed_hmat_hv(hm._h, xp, yp, dp, prec, sp))
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t1 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        step()
-    end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
-    ms_event = start.elapsed_time(end)
-    ms_per_step = ms_event / args.steps
-    value = D * args.steps / (wall) / 1e9
+    def step_matrix_free():
+        check(lib().ed_hv(sec.handle, wp, xp, yp, dp, prec, sp))
+
+    if not args.no_cpu_baseline:
+        try:
+            threads = min(16, len(os.sched_getaffinity(0)))
+            cpu = cpu_baseline(threads)
+        except Exception as e:  # the GPU figure stands on its own
+            cpu = {"value": None, "error": str(e)}
+    wall_mf, ev_mf = time_steps(step_matrix_free, args.steps, args.warmup, stream)
+    y_mf = y.clone()
+    wall, ev = time_steps(step_stored, args.steps, args.warmup, stream)
+    agree = float((y - y_mf).abs().max() / y_mf.abs().max())
+    ms_per_step = ev / args.steps
+    value = D * args.steps / wall / 1e9
     s_bytes = 16 if prec == 0 else 8
     alg = (2 * s_bytes + 12) * D
     achieved = alg / (ms_per_step * 1e-3) / 1e9
-    traffic = traffic_from_profiles() if prec == 0 else None
+    # compulsory bytes of the stored form: x, y, 6 bytes per stored element, count + diagonal index + orbit size
+    alg_stored = (2 * s_bytes + 4) * D + 6 * hm.nnz
     line = {
         "metric": "t-J H·v Gstates/s (% HBM peak); time to all-q S(q,ω) at 1e-3 vs plain",
         "value": value,
@@ -178,24 +266,36 @@ def main():
         "vs_baseline": None,
         "dtype": "complex128" if prec == 0 else "complex64",
         "data": "synthetic (seeded complex normal vectors, model built from its definition)",
-        "config": {"workload": "t-J ladder 2x12, 2 holes, Sz=0, k=(0,0), t=1, J=0.4, one H·v per step",
-                   "dim": D, "basis_build_s": round(t_basis, 4)},
+        "config": {"workload": "t-J ladder 2x12, 2 holes, Sz=0, k=(0,0), t=1, J=0.4, one H·v per step "
+                               "(stored sliced-ELL H, built once before the timed region)",
+                   "dim": D, "basis_build_s": round(t_basis, 4), "stored_build_s": round(t_store, 4),
+                   "stored_bytes": hm.bytes, "stored_elements_per_row": round(hm.nnz / D, 3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "hv_kernel + reduce_partials_kernel, per launch (HIP events)",
-                     "alg_bytes_per_state": 2 * s_bytes + 12, "event_ms_per_launch": ms_per_step},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("hm_spmv") if prec == 0 else None,
+                     "kernel": "hm_spmv_kernel (+ reduce_partials_kernel), per launch (HIP events)",
+                     "alg_bytes_per_state": 2 * s_bytes + 12, "event_ms_per_launch": ms_per_step,
+                     "achieved_incl_matrix": alg_stored / (ms_per_step * 1e-3) / 1e9,
+                     "frac_incl_matrix": alg_stored / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "matrix_free": {"value": D * args.steps / wall_mf / 1e9, "unit": "Gstates/s",
+                        "event_ms_per_launch": ev_mf / args.steps,
+                        "frac": alg / (ev_mf / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": traffic_from_profiles("hv_lds") if prec == 0 else None,
+                        "kernel": "hv_lds_kernel (+ reduce_partials_kernel)",
+                        "max_rel_diff_vs_stored": agree},
     }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu
     if args.copy_calibration:
         for _ in range(3):
             y.copy_(x)  # reads and writes D * itemsize bytes
         torch.cuda.synchronize()
         line["copy_calibration_bytes"] = D * x.element_size()
-    if not args.no_cpu_baseline:
-        try:
-            threads = min(16, len(os.sched_getaffinity(0)))
-            line["cpu_baseline"] = cpu_baseline(threads)
-        except Exception as e:  # the GPU figure stands on its own
-            line["cpu_baseline"] = {"value": None, "error": str(e)}
+    if not args.no_allq and prec == 0:
+        del hm, x, y, y_mf
+        torch.cuda.empty_cache()
+        line["allq"] = allq_section()
+    if "cpu_baseline" in line and line.get("allq"):
+        line["allq"]["cpu_estimate"] = cpu_allq_estimate(line["cpu_baseline"], line["allq"], D)
     print(json.dumps(line), flush=True)
 
 

@@ -24,7 +24,8 @@ EXPORTED = [
     "ed_mlp_create", "ed_mlp_destroy", "ed_mlp_num_params", "ed_mlp_get_params", "ed_mlp_set_params",
     "ed_mlp_forward", "ed_mlp_gradients", "ed_mlp_train_step", "ed_gemm_bf16",
     "ed_batch_create", "ed_batch_destroy", "ed_batch_offsets", "ed_batch_hv", "ed_batch_lanczos", "ed_batch_tune",
-    "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune",
+    "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune", "ed_lanczos_ritz",
+    "ed_hmat_create", "ed_hmat_destroy", "ed_hmat_info", "ed_hmat_hv", "ed_hmat_lanczos_run", "ed_hmat_lanczos_ritz",
 ]
 
 _vp = ct.c_void_p
@@ -72,6 +73,13 @@ _SIGNATURES = {
     "ed_batch_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
     "ed_batch_get_config": (ct.c_int, [_vp, _ip, _ip, _ip]),
     "ed_hv_tune": (ct.c_int, [_vp, _dp, ct.c_int, _vp, _vp, _ip, _ip, _dp, _vp]),
+    "ed_lanczos_ritz": (ct.c_int, [_vp, _dp, _vp, ct.c_int, _dp, _vp, ct.c_int, _vp]),
+    "ed_hmat_create": (ct.c_int, [_vp, _dp, ct.POINTER(_vp), _vp]),
+    "ed_hmat_destroy": (ct.c_int, [_vp]),
+    "ed_hmat_info": (ct.c_int, [_vp, ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64), _ip]),
+    "ed_hmat_hv": (ct.c_int, [_vp, _vp, _vp, _vp, ct.c_int, _vp]),
+    "ed_hmat_lanczos_run": (ct.c_int, [_vp, _vp, ct.c_int, _dp, _dp, _ip, ct.c_int, _vp]),
+    "ed_hmat_lanczos_ritz": (ct.c_int, [_vp, _vp, ct.c_int, _dp, _vp, ct.c_int, _vp]),
 }
 
 _lib = None

@@ -40,8 +40,13 @@ def distinct_qs(L, limited=True):
 
 
 def spectral_l1(S, S_ref, ws):
+    """sum |S - S_ref| dw / sum |S_ref| dw (0 when both vanish, inf when only S_ref does)."""
     dw = np.gradient(np.asarray(ws))
-    return float(np.sum(np.abs(S - S_ref) * dw) / np.sum(np.abs(S_ref) * dw))
+    num = float(np.sum(np.abs(S - S_ref) * dw))
+    den = float(np.sum(np.abs(S_ref) * dw))
+    if den == 0.0:
+        return 0.0 if num == 0.0 else float("inf")
+    return num / den
 
 
 @dataclasses.dataclass
@@ -125,3 +130,106 @@ def run_allq(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), precision=0, limited
     timings.update(final_basis=t_basis, operator=t_op, lanczos=t_lz, continued_fraction=t_cf,
                    total=time.perf_counter() - t0)
     return AllQResult(np.asarray(ws), distinct_qs(L, limited), spectra, coeffs, e0, timings)
+
+
+# -- time to all-q S(q, w) at a given accuracy ----------------------------------
+
+REFERENCE_EIG_TOL = 2e-7  # the reference's ARPACK tolerance (general/lanczos_diagonalization.py:89)
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+def _gs(apply, dim, tol, seed=0):
+    vals, vecs, _ = krylov.lowest_eigenpairs(apply, dim, 1, tol=tol, v0=krylov.start_vector(dim, seed))
+    return float(vals[0]), vecs[0]
+
+
+def allq_reference(L, h, nd, weights, ws, eta, nsteps=1000, k0=(0, 0), limited=True):
+    """S_ref(q, w) for every distinct q: ground state to 1e-12, then nsteps FP64 Lanczos steps (stored H)."""
+    s0 = Sector(L, h, nd, k0[0], k0[1])
+    hm0 = s0.store(weights)
+    e0, psi = _gs(hm0.hv, s0.dim, 1e-12)
+    hm0.close()
+    out = {}
+    for qx, qy in distinct_qs(L, limited):
+        kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
+        sec = Sector(L, h, nd, kx, ky)
+        phi = s0.op_apply(sec, _lib.OP_SZQ, qx, qy, psi)
+        n2 = krylov.vec_norm(phi) ** 2 if sec.dim else 0.0
+        if n2 < 1e-16:
+            out[(qx, qy)] = np.zeros(len(ws))
+        else:
+            hm = sec.store(weights)
+            a, b = hm.lanczos(phi, nsteps)
+            out[(qx, qy)] = cf_eval(a, b, ws, e0, eta, -n2 / np.pi)
+            hm.close()
+        sec.close()
+    s0.close()
+    _sync()
+    return e0, out
+
+
+def allq_plain(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL):
+    """The plain method on the GPU: FP64 vectors, matrix-free H·v, truncated continued fraction, one q
+    after the other (tJ_spin_half_ladder/api.py:116-132 with get_spectrum per q).  Returns (seconds, spectra)."""
+    from exactdiag_amd.sector import weights_array
+
+    wa = weights_array(np.asarray(weights, np.complex128))
+    _sync()
+    t0 = time.perf_counter()
+    s0 = Sector(L, h, nd, k0[0], k0[1])
+    e0, psi = _gs(lambda v: s0.hv(wa, v), s0.dim, tol)
+    out = {}
+    for qx, qy in distinct_qs(L, limited):
+        kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
+        sec = Sector(L, h, nd, kx, ky)
+        phi = s0.op_apply(sec, _lib.OP_SZQ, qx, qy, psi)
+        n2 = krylov.vec_norm(phi) ** 2 if sec.dim else 0.0
+        if n2 < 1e-16:
+            out[(qx, qy)] = np.zeros(len(ws))
+        else:
+            a, b = sec.lanczos(wa, phi, nsteps)
+            out[(qx, qy)] = cf_eval(a, b, ws, e0, eta, -n2 / np.pi)
+        sec.close()
+    s0.close()
+    _sync()
+    return time.perf_counter() - t0, out
+
+
+def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL, precision=1):
+    """The fast form: stored H (built once per sector on the GPU), complex64 Lanczos vectors with FP64
+    coefficient reductions, fewer steps closed by the square-root terminator.  Returns (seconds, spectra)."""
+    from exactdiag_amd.closure import sqrt_terminator
+
+    _sync()
+    t0 = time.perf_counter()
+    s0 = Sector(L, h, nd, k0[0], k0[1])
+    hm0 = s0.store(weights)
+    e0, psi = _gs(hm0.hv, s0.dim, tol)
+    hm0.close()
+    out = {}
+    for qx, qy in distinct_qs(L, limited):
+        kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
+        sec = Sector(L, h, nd, kx, ky)
+        phi = s0.op_apply(sec, _lib.OP_SZQ, qx, qy, psi)
+        n2 = krylov.vec_norm(phi) ** 2 if sec.dim else 0.0
+        if n2 < 1e-16:
+            out[(qx, qy)] = np.zeros(len(ws))
+        else:
+            hm = sec.store(weights)
+            a, b = hm.lanczos(phi, nsteps, precision=precision)
+            hm.close()
+            at, bt, tail = sqrt_terminator(a, b, len(a)) if len(a) == nsteps else (a, b, None)
+            out[(qx, qy)] = cf_eval(at, bt, ws, e0, eta, -n2 / np.pi, tail=tail)
+        sec.close()
+    s0.close()
+    _sync()
+    return time.perf_counter() - t0, out
+
+
+def max_l1(spectra, ref, ws):
+    return max(spectral_l1(spectra[q], ref[q], ws) for q in ref)

@@ -116,9 +116,19 @@ def features(a, b):
     return x.astype(np.float32), c, w
 
 
-def targets(a, b, c, w):
-    t = np.concatenate([(np.asarray(a[N_IN:N_IN + N_OUT]) - c) / w, np.asarray(b[N_IN + 1:N_IN + N_OUT + 1]) / w])
-    return t.astype(np.float32)
+def _block_mean(v, k):
+    """Mean over consecutive blocks of k levels, repeated back to the original length."""
+    if k <= 1:
+        return v
+    return np.asarray(v).reshape(-1, k).mean(1).repeat(k)
+
+
+def targets(a, b, c, w, smooth=1):
+    """The tail a'_64..a'_255, b'_65..b'_256 in bandwidth units; with smooth > 1, block means over smooth levels
+    (the level-by-level fluctuations carry the discrete spectrum and cannot be inferred from 64 levels)."""
+    ta = _block_mean((np.asarray(a[N_IN:N_IN + N_OUT]) - c) / w, smooth)
+    tb = _block_mean(np.asarray(b[N_IN + 1:N_IN + N_OUT + 1]) / w, smooth)
+    return np.concatenate([ta, tb]).astype(np.float32)
 
 
 def usable(a, b):
@@ -159,8 +169,8 @@ class Closure:
 def sqrt_terminator(a, b, n=N_IN):
     """The method without the network: n measured pairs closed by the square-root terminator
     at the bandwidth of the n-step tridiagonal (a_inf = c, b_inf = w / 2)."""
-    if len(a) < n + 1:
-        return np.asarray(a), np.asarray(b), None
+    if len(a) < n:
+        return np.asarray(a), np.asarray(b), None  # Krylov space exhausted: the fraction is exact
     c, w = bandwidth(a, b, n)
     return np.asarray(a[:n]), np.asarray(b[:n]), (c, 0.5 * w)
 
@@ -218,14 +228,14 @@ def lanczos_sequences(L, h, nd, weights, nsteps=1000, ops=("Szq", "Nq", "A+", "A
     return out
 
 
-def training_arrays(seqs):
+def training_arrays(seqs, smooth=1):
     X, T = [], []
     for s in seqs:
         if not usable(s["a"], s["b"]):
             continue
         x, c, w = features(s["a"], s["b"])
         X.append(x)
-        T.append(targets(s["a"], s["b"], c, w))
+        T.append(targets(s["a"], s["b"], c, w, smooth))
     return np.stack(X), np.stack(T)
 
 

@@ -1,0 +1,380 @@
+// Stored Hamiltonian for repeated H·v (gfx950).
+//
+// The reference builds H once per sector as four upper-triangle CSR matrices
+// with int16 indices into a table of unique values (dedup at 1e-6,
+// general/cython_sparse_matrices.pyx:58-241) and applies them with
+// sparse_matrix_dot (general/cpp_sparse_dot.h:59-138) at every Lanczos step.
+// Here the matrix is also built once, on the GPU, for one set of term
+// weights, and laid out for 64-wide wavefronts:
+//  * gather form (row f holds conj(H_jf) for every j, all four terms and the
+//    diagonal), so the product is written once per row, no atomics;
+//  * sliced ELL with slices of 64 consecutive rows: element k of the rows of
+//    slice s sits at sptr[s] + 64 k + lane, so the 64 lanes of a wavefront
+//    read consecutive words (coalesced) at every k;
+//  * per element an int32 column and a uint16 code = index into a table of
+//    exact unique values (bit 15 = negate), 6 bytes per element; per row the
+//    element count (uint8) and the diagonal's value index (uint16);
+//  * the common factor sqrt(n_f) of row f's off-diagonal elements is applied
+//    once per row (from the orbit table), so the value table holds
+//    W_t s_t conj(A[p][n_j]) per (term, group element, target orbit size),
+//    s_t = 1 for hops and 1/2 for exchange.
+// The elements come from the matrix-free walker (ed_hvrow.h), in the same
+// order, in two passes (count, fill).
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "ed_hvrow.h"
+
+namespace ed {
+namespace {
+
+constexpr int BS = 256;
+
+struct CountSink {
+    int n;
+    int djl, djr;
+
+    __device__ inline void diag(int term, double d) {
+        const int c = int(-2.0 * d + 0.5);
+        if (term == 2)
+            djl = c;
+        else
+            djr = c;
+    }
+
+    template <int G>
+    __device__ inline void group(const Elem (&e)[G]) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) n += e[k].code != NO_ELEM;
+    }
+};
+
+struct FillSink {
+    int32_t* col;
+    uint16_t* code;
+    const uint16_t* map;  // combo -> value index
+    int64_t base;         // sptr[slice] + lane
+    int k;
+
+    __device__ inline void diag(int, double) {}
+
+    template <int G>
+    __device__ inline void group(const Elem (&e)[G]) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            if (e[q].code == NO_ELEM) continue;
+            const int64_t at = base + int64_t(k) * 64;
+            col[at] = int32_t(e[q].j);
+            code[at] = uint16_t(map[e[q].code & 0x7fffu] | (e[q].code & 0x8000u));
+            ++k;
+        }
+    }
+};
+
+__global__ void __launch_bounds__(BS) hm_count_kernel(SectorView s, HvWeights w, LdsLayout lay, int lds,
+                                                      const uint16_t* __restrict__ diag_map, uint8_t* cnt,
+                                                      uint16_t* diag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Tabs T = lds ? stage_tabs<BS>(s, lay, smem) : global_tabs(s);
+    __syncthreads();
+    const int64_t f = int64_t(blockIdx.x) * BS + threadIdx.x;
+    if (f >= s.D) return;
+    CountSink c{0, 0, 0};
+    walk_row<4>(s, T, w, f, c);
+    cnt[f] = uint8_t(c.n);
+    diag[f] = diag_map[c.djl * (s.L + 1) + c.djr];
+}
+
+// width[slice] = largest element count of its 64 rows
+__global__ void __launch_bounds__(BS) hm_width_kernel(const uint8_t* cnt, int64_t D, int32_t* width, int64_t nslices) {
+    const int64_t sl = int64_t(blockIdx.x) * (BS / 64) + threadIdx.x / 64;
+    const int64_t f = sl * 64 + (threadIdx.x & 63);
+    int c = f < D ? int(cnt[f]) : 0;
+    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off, 64));
+    if ((threadIdx.x & 63) == 0 && sl < nslices) width[sl] = c;
+}
+
+__global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, LdsLayout lay, int lds,
+                                                     const uint16_t* __restrict__ map,
+                                                     const int64_t* __restrict__ sptr, int32_t* col, uint16_t* code) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Tabs T = lds ? stage_tabs<BS>(s, lay, smem) : global_tabs(s);
+    __syncthreads();
+    const int64_t f = int64_t(blockIdx.x) * BS + threadIdx.x;
+    if (f >= s.D) return;
+    FillSink sink{col, code, map, sptr[f >> 6] + (f & 63), 0};
+    walk_row<4>(s, T, w, f, sink);
+}
+
+// y_f = sqrt(n_f) sum_k (+-V[code]) x_col + V[diag_f] x_f; partial per workgroup = sum conj(x_f) y_f.
+template <class VT>
+__global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* __restrict__ sptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const uint16_t* __restrict__ code,
+                                                     const uint8_t* __restrict__ cnt,
+                                                     const uint16_t* __restrict__ diag,
+                                                     const uint8_t* __restrict__ orbit,
+                                                     const double* __restrict__ sq, const double* __restrict__ vals,
+                                                     int n_vals, const VT* __restrict__ x, VT* __restrict__ y,
+                                                     double* __restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* V = reinterpret_cast<double*>(smem);
+    for (int i = threadIdx.x; i < 2 * n_vals; i += BS) V[i] = vals[i];
+    __syncthreads();
+    const int64_t f = int64_t(blockIdx.x) * BS + threadIdx.x;
+    C2 dot{0.0, 0.0};
+    if (f < D) {
+        const int64_t sl = f >> 6;
+        const int64_t base = sptr[sl] + (f & 63);
+        const int n = cnt[f];
+        C2 acc{0.0, 0.0};
+        int k = 0;
+        for (; k + 4 <= n; k += 4) {
+            int32_t j[4];
+            uint16_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                j[u] = col[base + int64_t(k + u) * 64];
+                c[u] = code[base + int64_t(k + u) * 64];
+            }
+            C2 xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = ldx(x, j[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int vi = c[u] & 0x7fff;
+                const double sg = (c[u] & 0x8000) ? -1.0 : 1.0;
+                const C2 v{sg * V[2 * vi], sg * V[2 * vi + 1]};
+                acc = cadd(acc, cmul(v, xv[u]));
+            }
+        }
+        for (; k < n; ++k) {
+            const int32_t j = col[base + int64_t(k) * 64];
+            const uint16_t c = code[base + int64_t(k) * 64];
+            const C2 xv = ldx(x, j);
+            const int vi = c & 0x7fff;
+            const double sg = (c & 0x8000) ? -1.0 : 1.0;
+            acc = cadd(acc, cmul(C2{sg * V[2 * vi], sg * V[2 * vi + 1]}, xv));
+        }
+        const double sqf = sq[orbit[f]];
+        const C2 xf = ldx(x, f);
+        const int dv = diag[f];
+        const C2 yf = cadd(C2{sqf * acc.re, sqf * acc.im}, cmul(C2{V[2 * dv], V[2 * dv + 1]}, xf));
+        VT o;
+        o.x = decltype(o.x)(yf.re);
+        o.y = decltype(o.y)(yf.im);
+        y[f] = o;
+        dot = cmul(C2{xf.re, -xf.im}, yf);
+    }
+    if (partial) {
+        __shared__ double red[2 * BS];
+        red[2 * threadIdx.x] = dot.re;
+        red[2 * threadIdx.x + 1] = dot.im;
+        __syncthreads();
+        for (int st = BS / 2; st > 0; st >>= 1) {
+            if (int(threadIdx.x) < st) {
+                red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
+                red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            partial[2 * blockIdx.x] = red[0];
+            partial[2 * blockIdx.x + 1] = red[1];
+        }
+    }
+}
+
+uint64_t bits(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    return u;
+}
+
+}  // namespace
+}  // namespace ed
+
+using namespace ed;
+
+struct ed_hmat {
+    const ed_sector* s = nullptr;
+    HvWeights w{};
+    int64_t D = 0, nslices = 0, slots = 0, nnz = 0;
+    int n_vals = 0;
+    int64_t* d_sptr = nullptr;
+    int32_t* d_col = nullptr;
+    uint16_t* d_code = nullptr;
+    uint8_t* d_cnt = nullptr;
+    uint16_t* d_diag = nullptr;
+    double* d_vals = nullptr;
+    double* d_partial = nullptr;
+
+    int64_t bytes() const { return slots * 6 + D * 3 + (nslices + 1) * 8 + int64_t(n_vals) * 16; }
+
+    ~ed_hmat() {
+        for (void* p : {(void*)d_sptr, (void*)d_col, (void*)d_code, (void*)d_cnt, (void*)d_diag, (void*)d_vals,
+                        (void*)d_partial})
+            dev_free(p);
+    }
+};
+
+namespace ed {
+
+const ed_sector* hmat_sector(const ed_hmat* m) { return m->s; }
+
+int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st) {
+    const int64_t nwg = (m->D + BS - 1) / BS;
+    if (nwg == 0) return 0;
+    const size_t lds = size_t(m->n_vals) * 16;
+    const ed_sector* s = m->s;
+    if (precision == 0)
+        hipLaunchKernelGGL(hm_spmv_kernel<double2>, dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col,
+                           m->d_code, m->d_cnt, m->d_diag, s->d_orbit, s->d_sqrt, m->d_vals, m->n_vals,
+                           static_cast<const double2*>(x), static_cast<double2*>(y), partial);
+    else
+        hipLaunchKernelGGL(hm_spmv_kernel<float2>, dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col,
+                           m->d_code, m->d_cnt, m->d_diag, s->d_orbit, s->d_sqrt, m->d_vals, m->n_vals,
+                           static_cast<const float2*>(x), static_cast<float2*>(y), partial);
+    ED_HIP(hipGetLastError());
+    return nwg;
+}
+
+}  // namespace ed
+
+extern "C" {
+
+int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && s->D >= 0 && w && out, ED_ERR_ARG, "bad stored-matrix arguments");
+        *out = nullptr;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        auto* m = new ed_hmat();
+        std::vector<void*> tmp;
+        auto cleanup_tmp = [&] {
+            for (void* p : tmp) dev_free(p);
+            tmp.clear();
+        };
+        try {
+            m->s = s;
+            m->w = make_weights(w);
+            m->D = s->D;
+            const int N = s->N, L = s->L;
+            // exact unique values: off-diagonal combos, then diagonal (jl count, jr count) pairs
+            std::vector<double> vals;
+            std::map<std::pair<uint64_t, uint64_t>, int> uid;
+            auto intern = [&](C2 v) {
+                const auto key = std::make_pair(bits(v.re), bits(v.im));
+                auto it = uid.find(key);
+                if (it != uid.end()) return it->second;
+                const int id = int(vals.size() / 2);
+                vals.push_back(v.re);
+                vals.push_back(v.im);
+                uid.emplace(key, id);
+                return id;
+            };
+            const int ncombo = 4 * N * (N + 1);
+            std::vector<uint16_t> map(size_t(ncombo), 0);
+            for (int t = 0; t < 4; ++t) {
+                if (!(m->w.active & (1 << t))) continue;
+                const double st_ = t < 2 ? 1.0 : 0.5;
+                for (int p = 0; p < N; ++p)
+                    for (int nj = 0; nj <= N; ++nj) {
+                        const int idx = p * (N + 1) + nj;
+                        const double ar = s->amp[2 * idx], ai = s->amp[2 * idx + 1];
+                        map[size_t((t * N + p) * (N + 1) + nj)] =
+                            uint16_t(intern(cmul(m->w.w[t], C2{st_ * ar, -(st_ * ai)})));
+                    }
+            }
+            std::vector<uint16_t> dmap(size_t((N + 1) * (L + 1)), 0);
+            for (int a = 0; a <= N; ++a)
+                for (int b = 0; b <= L; ++b) {
+                    C2 d{0.0, 0.0};
+                    if (m->w.active & 4) d = cadd(d, cmul(m->w.w[2], C2{-0.5 * a, 0.0}));
+                    if (m->w.active & 8) d = cadd(d, cmul(m->w.w[3], C2{-0.5 * b, 0.0}));
+                    dmap[size_t(a * (L + 1) + b)] = uint16_t(intern(d));
+                }
+            m->n_vals = int(vals.size() / 2);
+            ED_REQUIRE(m->n_vals < 0x8000, ED_ERR_CAPACITY, "more than 32767 unique matrix values");
+            m->d_vals = dev_upload(vals, st);
+            const int64_t D = s->D;
+            m->nslices = (D + 63) / 64;
+            m->d_cnt = dev_alloc<uint8_t>(size_t(std::max<int64_t>(D, 1)));
+            m->d_diag = dev_alloc<uint16_t>(size_t(std::max<int64_t>(D, 1)));
+            m->d_partial = dev_alloc<double>(size_t(2 * std::max<int64_t>((D + BS - 1) / BS, 1)));
+            std::vector<int64_t> sptr(static_cast<size_t>(m->nslices + 1), 0);
+            if (D > 0) {
+                uint16_t* d_map = dev_upload(map, st);
+                tmp.push_back(d_map);
+                uint16_t* d_dmap = dev_upload(dmap, st);
+                tmp.push_back(d_dmap);
+                int32_t* d_width = dev_alloc<int32_t>(size_t(m->nslices));
+                tmp.push_back(d_width);
+                const LdsLayout lay = lds_layout(s);
+                const int lds = lay.bytes <= LDS_CAP;
+                const unsigned nwg = unsigned((D + BS - 1) / BS);
+                const SectorView v = s->view();
+                hipLaunchKernelGGL(hm_count_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
+                                   d_dmap, m->d_cnt, m->d_diag);
+                hipLaunchKernelGGL(hm_width_kernel, dim3(unsigned((m->nslices + 3) / 4)), dim3(BS), 0, st, m->d_cnt,
+                                   D, d_width, m->nslices);
+                ED_HIP(hipGetLastError());
+                std::vector<int32_t> width(static_cast<size_t>(m->nslices));
+                ED_HIP(hipMemcpyAsync(width.data(), d_width, width.size() * 4, hipMemcpyDeviceToHost, st));
+                std::vector<uint8_t> cnt(static_cast<size_t>(D));
+                ED_HIP(hipMemcpyAsync(cnt.data(), m->d_cnt, size_t(D), hipMemcpyDeviceToHost, st));
+                ED_HIP(hipStreamSynchronize(st));
+                for (int64_t i = 0; i < m->nslices; ++i) sptr[size_t(i + 1)] = sptr[size_t(i)] + int64_t(width[i]) * 64;
+                for (uint8_t c : cnt) m->nnz += c;
+                m->slots = sptr.back();
+                m->d_sptr = dev_upload(sptr, st);
+                m->d_col = dev_alloc<int32_t>(size_t(std::max<int64_t>(m->slots, 1)));
+                m->d_code = dev_alloc<uint16_t>(size_t(std::max<int64_t>(m->slots, 1)));
+                hipLaunchKernelGGL(hm_fill_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
+                                   d_map, m->d_sptr, m->d_col, m->d_code);
+                ED_HIP(hipGetLastError());
+                ED_HIP(hipStreamSynchronize(st));
+            } else {
+                m->d_sptr = dev_upload(sptr, st);
+                ED_HIP(hipStreamSynchronize(st));
+            }
+            cleanup_tmp();
+        } catch (...) {
+            cleanup_tmp();
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int ed_hmat_destroy(ed_hmat* m) {
+    delete m;
+    return ED_OK;
+}
+
+int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots, int* n_values) {
+    return guarded([&] {
+        ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
+        if (bytes) *bytes = m->bytes();
+        if (nnz) *nnz = m->nnz;
+        if (slots) *slots = m->slots;
+        if (n_values) *n_values = m->n_vals;
+    });
+}
+
+int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && d_x && d_y, ED_ERR_ARG, "null argument");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        if (m->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const int64_t np = launch_hmat(m, d_x, d_y, d_dot ? m->d_partial : nullptr, precision, st);
+        if (d_dot) launch_reduce_partials(m->d_partial, np, d_dot, st);
+    });
+}
+
+}  // extern "C"

@@ -132,6 +132,7 @@ __device__ inline int32_t window_delta32(uint32_t H, uint32_t Dn, uint32_t D2, i
 struct Elem {
     int64_t j;
     C2 c;
+    uint32_t code;  // stored-form element code (elem_code), NO_ELEM for an empty slot
 };
 
 __device__ inline C2 weighted_conj(const Tabs& T, int N, int L, int ne_even, uint32_t full, uint32_t Hsrc, int p,
@@ -180,38 +181,39 @@ __device__ inline C2 ldx(const VT* __restrict__ x, int64_t j) {
     return {double(v.x), double(v.y)};
 }
 
-template <class VT, int G>
-__device__ inline void flush(const VT* __restrict__ x, const Elem (&e)[G], C2& acc) {
-    C2 v[G];
-#pragma unroll
-    for (int k = 0; k < G; ++k) v[k] = ldx(x, e[k].j);
-#pragma unroll
-    for (int k = 0; k < G; ++k) acc = cadd(acc, cmul(e[k].c, v[k]));
+// Element code for the stored form: combo = (term * N + p) * (N + 1) + n_j
+// (term 0..3 = tl, tr, jl, jr; p the group element; n_j the target's orbit
+// size) in the low 15 bits, bit 15 set when the element's sign is negative.
+// NO_ELEM marks an empty slot of a group.
+constexpr uint32_t NO_ELEM = 0xffffffffu;
+
+__device__ inline uint32_t elem_code(int term, int N, int p, int nj, bool neg) {
+    return uint32_t((term * N + p) * (N + 1) + nj) | (neg ? 0x8000u : 0u);
 }
 
-// (H x)_f; also returns x_f for the fused <x, Hx>.  Exchange elements are
-// gathered G at a time, hops three at a time (the three bonds onto a hole).
-template <class VT, int G>
-__device__ inline C2 hv_row(const SectorView& s, const Tabs& T, const HvWeights& w, const VT* __restrict__ x,
-                            int64_t f, C2& xf) {
+// Walk the elements of row f in a fixed order (jl bonds, jr bonds, hops onto
+// each hole), G exchange elements or three hops per group, and hand each
+// group to the sink: sink.diag(term, -count/2) for the exchange diagonal of
+// term 2 (jl) and 3 (jr), sink.group(e) for the elements.
+template <int G, class Sink>
+__device__ inline void walk_row(const SectorView& s, const Tabs& T, const HvWeights& w, int64_t f, Sink& sink) {
     const int N = s.N, L = s.L, ne_even = s.ne_even;
     const uint32_t full = s.full;
     const uint64_t key = s.key[f];
     const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
     const uint32_t E = ~H & full, Up = E & ~Dn;
     const double sqf = T.sq[s.orbit[f]];
-    xf = ldx(x, f);
     const int32_t hr = rank_holes32(H, T.comb);
     const uint32_t gm_self = T.gm[hr];
     const int blk_self = T.cb[hr];
     const bool generic = (gm_self & (gm_self - 1u)) == 0u;
-    C2 acc{0.0, 0.0};
     // an exchange element: generic blocks by window rank change (coefficient cg, the same for every bond of the
     // term), others by the stabiliser search
-    auto exchange = [&](uint32_t D2, int lo, int hi, bool wrap, C2 W, C2 cg, Elem& e) {
+    auto exchange = [&](int term, uint32_t D2, int lo, int hi, bool wrap, C2 W, C2 cg, Elem& e) {
         if (generic) {
             e.j = wrap ? T.bb[blk_self] + rank_spins32(H, D2, T.comb) : f + window_delta32(H, Dn, D2, lo, hi, T.comb);
             e.c = cg;
+            e.code = elem_code(term, N, 0, N, false);
             return;
         }
         int p, nj;
@@ -219,6 +221,7 @@ __device__ inline C2 hv_row(const SectorView& s, const Tabs& T, const HvWeights&
         if (locate(s, T, H, D2, gm_self, blk_self, p, j, nj)) {
             e.j = j;
             e.c = weighted_conj(T, N, L, ne_even, full, H, p, nj, 0.5 * sqf, W);
+            e.code = elem_code(term, N, p, nj, image_sign(H, p, L, ne_even, full) < 0);
         }
     };
     if (w.active & 4) {  // jl: bonds (i, i + 2)
@@ -226,38 +229,36 @@ __device__ inline C2 hv_row(const SectorView& s, const Tabs& T, const HvWeights&
         const uint32_t anti = N == 2 ? 0u : (((Dn & u2) | (Up & d2)) & full);
         const C2 W = w.w[2];
         const C2 cg = weighted_conj(T, N, L, ne_even, full, H, 0, N, 0.5 * sqf, W);
-        const double diag = -0.5 * double(popc32(anti));
-        acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+        sink.diag(2, -0.5 * double(popc32(anti)));
         for (int g = 0; g < N; g += G) {
             Elem e[G];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
-                e[k] = Elem{f, C2{0.0, 0.0}};
+                e[k] = Elem{f, C2{0.0, 0.0}, NO_ELEM};
                 const int i = g + k;
                 if (i < N && ((anti >> i) & 1u)) {
                     const int kk = i + 2 < N ? i + 2 : i + 2 - N;
                     const bool wrap = kk < i;
-                    exchange(Dn ^ (1u << i) ^ (1u << kk), wrap ? kk : i, wrap ? i : kk, wrap, W, cg, e[k]);
+                    exchange(2, Dn ^ (1u << i) ^ (1u << kk), wrap ? kk : i, wrap ? i : kk, wrap, W, cg, e[k]);
                 }
             }
-            flush<VT, G>(x, e, acc);
+            sink.group(e);
         }
     }
     if (w.active & 8) {  // jr: bonds (2x, 2x + 1)
         const uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
         const C2 W = w.w[3];
         const C2 cg = weighted_conj(T, N, L, ne_even, full, H, 0, N, 0.5 * sqf, W);
-        const double diag = -0.5 * double(popc32(anti));
-        acc = cadd(acc, cmul(W, C2{diag * xf.re, diag * xf.im}));
+        sink.diag(3, -0.5 * double(popc32(anti)));
         for (int g = 0; g < L; g += G) {
             Elem e[G];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
-                e[k] = Elem{f, C2{0.0, 0.0}};
+                e[k] = Elem{f, C2{0.0, 0.0}, NO_ELEM};
                 const int i = 2 * (g + k);
-                if (g + k < L && ((anti >> i) & 1u)) exchange(Dn ^ (3u << i), i, i + 1, false, W, cg, e[k]);
+                if (g + k < L && ((anti >> i) & 1u)) exchange(3, Dn ^ (3u << i), i, i + 1, false, W, cg, e[k]);
             }
-            flush<VT, G>(x, e, acc);
+            sink.group(e);
         }
     }
     if (w.active & 3) {  // hops onto each hole: leg neighbours (tl) and rung partner (tr)
@@ -268,7 +269,7 @@ __device__ inline C2 hv_row(const SectorView& s, const Tabs& T, const HvWeights&
             Elem e[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                e[k] = Elem{f, C2{0.0, 0.0}};
+                e[k] = Elem{f, C2{0.0, 0.0}, NO_ELEM};
                 if (k < 2 && !(w.active & 1)) continue;
                 if (k == 2 && !(w.active & 2)) continue;
                 const int src = k == 0 ? (t + 2 < N ? t + 2 : t + 2 - N) : k == 1 ? (t >= 2 ? t - 2 : t - 2 + N)
@@ -289,12 +290,45 @@ __device__ inline C2 hv_row(const SectorView& s, const Tabs& T, const HvWeights&
                 if (locate(s, T, H2, D2, T.gm[hr2], blk, p, j, nj)) {
                     e[k].j = j;
                     e[k].c = weighted_conj(T, N, L, ne_even, full, H2, p, nj, par ? sqf : -sqf, W);
+                    e[k].code = elem_code(k < 2 ? 0 : 1, N, p, nj, (image_sign(H2, p, L, ne_even, full) < 0) == bool(par));
                 }
             }
-            flush<VT, 3>(x, e, acc);
+            sink.group(e);
         }
     }
-    return acc;
+}
+
+// Matrix-free sink: gathers of one group are issued together, then
+// accumulated in element order.
+template <class VT>
+struct GatherSink {
+    const VT* __restrict__ x;
+    C2 xf;
+    HvWeights w;
+    C2 acc;
+
+    __device__ inline void diag(int term, double d) {
+        acc = cadd(acc, cmul(w.w[term], C2{d * xf.re, d * xf.im}));
+    }
+
+    template <int G>
+    __device__ inline void group(const Elem (&e)[G]) {
+        C2 v[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) v[k] = ldx(x, e[k].j);
+#pragma unroll
+        for (int k = 0; k < G; ++k) acc = cadd(acc, cmul(e[k].c, v[k]));
+    }
+};
+
+// (H x)_f; also returns x_f for the fused <x, Hx>.
+template <class VT, int G>
+__device__ inline C2 hv_row(const SectorView& s, const Tabs& T, const HvWeights& w, const VT* __restrict__ x,
+                            int64_t f, C2& xf) {
+    GatherSink<VT> sink{x, ldx(x, f), w, C2{0.0, 0.0}};
+    walk_row<G>(s, T, w, f, sink);
+    xf = sink.xf;
+    return sink.acc;
 }
 
 }  // namespace ed

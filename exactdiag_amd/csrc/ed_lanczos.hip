@@ -260,7 +260,7 @@ __global__ void zero2_kernel(double2* p, int64_t n) {
 template <class VT>
 void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
                    int* n_out, VT* d_basis, int precision, hipStream_t st, double2* acc = nullptr,
-                   const double* ritz = nullptr) {
+                   const double* ritz = nullptr, const ed_hmat* m = nullptr) {
     const int64_t D = s->D;
     const int64_t nwg = (D + BS - 1) / BS;
     const HvWeights hw = make_weights(w);
@@ -293,7 +293,8 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         ED_HIP(hipGetLastError());
         int flag[2] = {0, 0};
         for (int n = 0; n < nsteps; ++n) {
-            const int64_t np = launch_hv(s, hw, u, y, partial, precision, 1, st);
+            const int64_t np =
+                m ? launch_hmat(m, u, y, partial, precision, st) : launch_hv(s, hw, u, y, partial, precision, 1, st);
             hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(BS), 0, st, lz, partial, np, n);
             hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, lz, n, D, y, u, up,
                                d_basis ? d_basis + int64_t(n) * D : nullptr, acc, d_ritz, partial);
@@ -361,6 +362,43 @@ int ed_lanczos_ritz(const ed_sector* s, const double* w, const double* d_v0, int
             lanczos_run_t<double2>(s, w, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 0, st, acc, ritz);
         else
             lanczos_run_t<float2>(s, w, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 1, st, acc, ritz);
+    });
+}
+
+int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
+                        int precision, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
+        const ed_sector* s = hmat_sector(m);
+        ED_REQUIRE(nsteps >= 1 && a && b && n_out, ED_ERR_ARG, "bad Lanczos arguments");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        *n_out = 0;
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const double w0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (precision == 0)
+            lanczos_run_t<double2>(s, w0, d_v0, nsteps, a, b, n_out, nullptr, 0, st, nullptr, nullptr, m);
+        else
+            lanczos_run_t<float2>(s, w0, d_v0, nsteps, a, b, n_out, nullptr, 1, st, nullptr, nullptr, m);
+    });
+}
+
+int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const double* ritz, double* d_x,
+                         int precision, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
+        const ed_sector* s = hmat_sector(m);
+        ED_REQUIRE(nsteps >= 1 && ritz && d_x, ED_ERR_ARG, "bad Ritz-vector arguments");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const double w0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int n = 0;
+        auto* acc = reinterpret_cast<double2*>(d_x);
+        if (precision == 0)
+            lanczos_run_t<double2>(s, w0, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 0, st, acc, ritz, m);
+        else
+            lanczos_run_t<float2>(s, w0, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 1, st, acc, ritz, m);
     });
 }
 

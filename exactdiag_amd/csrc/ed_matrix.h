@@ -3,6 +3,8 @@
 
 #include "ed_internal.h"
 
+struct ed_hmat;
+
 namespace ed {
 
 struct HvWeights {
@@ -17,6 +19,8 @@ int64_t launch_hv_lds(const ed_sector* s, const HvWeights& w, const void* x, voi
                       hipStream_t st);
 int64_t hv_partials(const ed_sector* s);
 int hv_default_tiles(const ed_sector* s);
+int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st);
+const ed_sector* hmat_sector(const ed_hmat* m);
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st);
 
 }  // namespace ed

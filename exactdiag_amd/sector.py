@@ -156,6 +156,20 @@ class Sector:
             return y, complex(dot.cpu().numpy()[0])
         return y
 
+    def lanczos_ritz(self, weights, v0, ritz, precision: int = 0):
+        """Second pass of two-pass Lanczos: sum_n ritz[n] v_n (complex128) from the same start vector."""
+        torch = _torch()
+        v0 = v0.to(torch.complex128).contiguous()
+        r, rp = _dbl(ritz)
+        x = torch.empty(max(self.dim, 1), dtype=torch.complex128, device=self.device)[: self.dim]
+        w, wp = _dbl(weights_array(weights))
+        check(self._lib.ed_lanczos_ritz(self._h, wp, _ptr(v0), len(r), rp, _ptr(x), int(precision), _stream()))
+        return x
+
+    def store(self, weights):
+        """The stored Hamiltonian of this sector for ``weights`` (built once on the GPU)."""
+        return StoredH(self, weights)
+
     def tune(self, weights, precision: int = 0):
         """Autotune this sector's H·v (tiles per workgroup, gathers in flight); returns (tiles, gather, ms)."""
         torch = _torch()
@@ -194,6 +208,72 @@ class Sector:
         if keep_basis:
             return a[:k].copy(), b[:k].copy(), basis[:k]
         return a[:k].copy(), b[:k].copy()
+
+
+class StoredH:
+    """H of one sector and one set of weights stored on the device (sliced ELL, 6 bytes per element).
+
+    The same products as :meth:`Sector.hv` and :meth:`Sector.lanczos`, from
+    the stored elements instead of recomputing them; the sector must stay
+    open while the matrix is used.
+    """
+
+    def __init__(self, sector: Sector, weights):
+        self.sector = sector
+        self.dim = sector.dim
+        self.device = sector.device
+        w, wp = _dbl(weights_array(weights))
+        h = ct.c_void_p()
+        check(lib().ed_hmat_create(sector.handle, wp, ct.byref(h), _stream()))
+        self._h = h
+        b, nnz, slots, nv = ct.c_int64(0), ct.c_int64(0), ct.c_int64(0), ct.c_int(0)
+        check(lib().ed_hmat_info(self._h, ct.byref(b), ct.byref(nnz), ct.byref(slots), ct.byref(nv)))
+        self.bytes, self.nnz, self.slots, self.n_values = b.value, nnz.value, slots.value, nv.value
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().ed_hmat_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def hv(self, x, out=None, with_dot=False):
+        torch = _torch()
+        if x.dtype not in (torch.complex128, torch.complex64):
+            raise TypeError("x must be complex128 or complex64")
+        prec = 0 if x.dtype == torch.complex128 else 1
+        x = x.contiguous()
+        y = torch.empty_like(x) if out is None else out
+        dot = torch.zeros(1, dtype=torch.complex128, device=self.device) if with_dot else None
+        check(lib().ed_hmat_hv(self._h, _ptr(x), _ptr(y), _ptr(dot) if dot is not None else None, prec, _stream()))
+        if with_dot:
+            return y, complex(dot.cpu().numpy()[0])
+        return y
+
+    def lanczos(self, v0, nsteps: int, precision: int = 0):
+        torch = _torch()
+        v0 = v0.to(torch.complex128).contiguous()
+        a = np.zeros(nsteps, np.float64)
+        b = np.zeros(nsteps + 1, np.float64)
+        n = ct.c_int(0)
+        check(lib().ed_hmat_lanczos_run(self._h, _ptr(v0), int(nsteps), a.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                        b.ctypes.data_as(ct.POINTER(ct.c_double)), ct.byref(n), int(precision),
+                                        _stream()))
+        k = n.value
+        return a[:k].copy(), b[:k].copy()
+
+    def lanczos_ritz(self, v0, ritz, precision: int = 0):
+        torch = _torch()
+        v0 = v0.to(torch.complex128).contiguous()
+        r, rp = _dbl(ritz)
+        x = torch.empty(max(self.dim, 1), dtype=torch.complex128, device=self.device)[: self.dim]
+        check(lib().ed_hmat_lanczos_ritz(self._h, _ptr(v0), len(r), rp, _ptr(x), int(precision), _stream()))
+        return x
 
 
 def cf_eval(a, b, ws, e0, eta, factor, tail=None):

@@ -115,6 +115,8 @@ def main(argv=None):
     ap.add_argument("--data", type=pathlib.Path)
     ap.add_argument("--train", action="store_true")
     ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--smooth", type=int, default=1, help="train on block means over this many levels")
+    ap.add_argument("--sweep", action="store_true", help="train over smoothing x steps, keep the best on validation")
     ap.add_argument("--eval", type=pathlib.Path, help="npz of held-out sequences (all-q run) to evaluate on")
     ap.add_argument("--model", type=pathlib.Path, default=closure.MODEL_PATH)
     ap.add_argument("--gen-eval", type=pathlib.Path, help="write the 2x12, 2-hole evaluation sequences here")
@@ -126,22 +128,30 @@ def main(argv=None):
         generate(args.data)
     if args.train:
         seqs = load_sequences(args.data)
-        X, T = closure.training_arrays(seqs)
-        rng = np.random.default_rng(1)
-        perm = rng.permutation(len(X))
-        n_val = max(1, len(X) // 10)
-        val, tr = perm[:n_val], perm[n_val:]
-        t0 = time.perf_counter()
-        model, hist = closure.train(X[tr], T[tr], steps=args.steps)
-        print(f"trained on {len(tr)} sequences in {time.perf_counter() - t0:.1f}s: {hist}", flush=True)
-        yv = model.net.forward(X[val])
-        print(f"validation MSE {float(np.mean((yv - T[val]) ** 2)):.4e} "
-              f"(zero-tail baseline {float(np.mean((T[val] - T[tr].mean(0)) ** 2)):.4e})", flush=True)
-        model.save(args.model)
         usable = [s for s in seqs if closure.usable(s["a"], s["b"])]
-        errs = spectra_errors([usable[i] for i in val], model, ws, 0.05)
-        print("held-out L1 (mean): truncated, sqrt terminator, learned, 256 steps:",
-              json.dumps(errs.mean(0).tolist()), flush=True)
+        rng = np.random.default_rng(1)
+        perm = rng.permutation(len(usable))
+        n_val = max(1, len(usable) // 10)
+        val, tr = perm[:n_val], perm[n_val:]
+        configs = [(args.smooth, args.steps)] if not args.sweep else [
+            (sm, st) for sm in (1, 4, 8) for st in (500, 1500, 4000)]
+        best = None
+        for smooth, steps in configs:
+            X, T = closure.training_arrays(usable, smooth)
+            t0 = time.perf_counter()
+            model, hist = closure.train(X[tr], T[tr], steps=steps)
+            yv = model.net.forward(X[val])
+            errs = spectra_errors([usable[i] for i in val], model, ws, 0.05)
+            gm = float(np.exp(np.mean(np.log(errs[:, 1] / errs[:, 2]))))
+            print(f"smooth={smooth} steps={steps}: {time.perf_counter() - t0:.1f}s, final loss {hist[-1][1]:.3e}, "
+                  f"validation MSE {float(np.mean((yv - T[val]) ** 2)):.3e} (mean-tail baseline "
+                  f"{float(np.mean((T[val] - T[tr].mean(0)) ** 2)):.3e}); validation L1 mean truncated, terminator, "
+                  f"learned, 256 steps: {json.dumps(errs.mean(0).tolist())}; terminator/learned geo-mean {gm:.3f}",
+                  flush=True)
+            if best is None or gm > best[0]:
+                best = (gm, smooth, steps, model)
+        print(f"selected smooth={best[1]} steps={best[2]} (validation gain {best[0]:.3f})", flush=True)
+        best[3].save(args.model)
     if args.eval:
         model = closure.Closure.load(args.model)
         seqs = load_sequences(args.eval)
@@ -150,6 +160,8 @@ def main(argv=None):
         for s, e in zip(seqs, errs):
             print(s["qx"], s["qy"], " ".join(f"{v:.3e}" for v in e))
         print("eval mean", json.dumps(errs.mean(0).tolist()), "max", json.dumps(errs.max(0).tolist()), flush=True)
+        print(f"eval terminator/learned geo-mean {float(np.exp(np.mean(np.log(errs[:, 1] / errs[:, 2])))):.3f}, "
+              f"truncated/learned {float(np.exp(np.mean(np.log(errs[:, 0] / errs[:, 2])))):.3f}", flush=True)
 
 
 if __name__ == "__main__":

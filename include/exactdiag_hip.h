@@ -111,6 +111,36 @@ int ed_op_apply(const ed_sector* init, const ed_sector* fin, int op, int qx, int
 int ed_lanczos_run(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
                    int* n_out, double* d_basis, int precision, void* stream);
 
+/* Second pass of two-pass Lanczos: rerun the recurrence of ed_lanczos_run
+ * from the same d_v0 and accumulate d_x = sum_n ritz[n] v_n (c128, FP64
+ * accumulation also when precision = 1), e.g. the Ritz vector of a
+ * tridiagonal eigenvector.  No counterpart in the reference (which keeps no
+ * Lanczos vectors; its eigenvectors come from ARPACK,
+ * general/lanczos_diagonalization.py:81-121). */
+int ed_lanczos_ritz(const ed_sector* s, const double* w, const double* d_v0, int nsteps, const double* ritz,
+                    double* d_x, int precision, void* stream);
+
+/* Stored Hamiltonian of one sector and one set of weights, built once on the
+ * GPU for repeated products.  Replaces calc_sparse_matrix
+ * (general/cython_sparse_matrices.pyx:58-241: four upper-triangle CSR
+ * matrices with int16 indices into unique values) and Sparse_Matrix.dot /
+ * Added_Sparse_Matrices.dot (general/sparse_matrices.py:83-105, 233-237;
+ * general/cpp_sparse_dot.h:59-138).  Layout: gather form, sliced ELL of
+ * 64-row slices, int32 column + uint16 code into a table of exact unique
+ * values per element.  The sector must outlive the matrix.  ed_hmat_info:
+ * device bytes, stored elements, slots (with padding), unique values.
+ * ed_hmat_hv as ed_hv; ed_hmat_lanczos_run / _ritz as ed_lanczos_run /
+ * ed_lanczos_ritz with the stored matrix as H. */
+typedef struct ed_hmat ed_hmat;
+int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* stream);
+int ed_hmat_destroy(ed_hmat* m);
+int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots, int* n_values);
+int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream);
+int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
+                        int precision, void* stream);
+int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const double* ritz, double* d_x,
+                         int precision, void* stream);
+
 /* Batched Lanczos over many sectors of uneven size (instances), each with
  * its own weights (8 doubles per instance as in ed_hv).  Replaces the
  * one-sector-at-a-time loops of get_all_k_eigenpairs
