@@ -129,6 +129,9 @@ def allq_section():
     log(f"all-q plain in {t_plain:.2f}s")
     t_fast, s_fast = allq.allq_fast(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
     log(f"all-q fast in {t_fast:.2f}s")
+    t_ps, s_ps = allq.allq_fast(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_PLAIN, precision=0,
+                                terminator=False)
+    log(f"all-q plain method on the stored H in {t_ps:.2f}s")
     err_plain = {f"{q[0]},{q[1]}": allq.spectral_l1(s_plain[q], ref[q], ws) for q in ref}
     err_fast = {f"{q[0]},{q[1]}": allq.spectral_l1(s_fast[q], ref[q], ws) for q in ref}
     return {
@@ -141,6 +144,10 @@ def allq_section():
         "fast": {"seconds": t_fast, "steps": ALLQ_STEPS_FAST, "max_l1": max(err_fast.values()),
                  "method": "stored sliced-ELL H per sector, complex64 vectors with FP64 coefficient reductions, "
                            "square-root terminator"},
+        "plain_stored": {"seconds": t_ps, "steps": ALLQ_STEPS_PLAIN,
+                         "max_l1": max(allq.spectral_l1(s_ps[q], ref[q], ws) for q in ref),
+                         "method": "the plain method (complex128, truncated fraction) on the stored H: the H·v "
+                                   "kernel's share of the speed-up"},
         "speedup_vs_plain": t_plain / t_fast,
         "l1_plain": err_plain,
         "l1_fast": err_fast,
@@ -161,6 +168,34 @@ def cpu_allq_estimate(cpu, section, D):
                       "measured build rate of the sample rows (oracle restatement of the reference's column "
                       "functions), disk I/O and Python overhead of the reference not counted",
             "speedup_fast_vs_cpu": total / section["fast"]["seconds"]}
+
+
+def run_form(form):
+    import torch
+
+    from exactdiag_amd import forms
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    if form == "batching":
+        res = forms.batching_form(log=log)
+        line = {"metric": "batched all-k lowest eigenpairs + all-k A(k,w) (seconds, batched vs one sector at a time)",
+                "value": res["gs_batched_s"] + res["spectra_batched_s"], "unit": "s", "n_gpus": 1,
+                "higher_is_better": False, "dtype": "complex128", "data": "synthetic (model built from its definition)",
+                "config": {"workload": "2x10 ladder, 2 holes, Sz=0, t=1, J in {0.3,0.4,0.5}, all 20 k sectors "
+                                       "(60 instances); A+(k,w), A-(k,w) for all 20 k, 250 steps, FP64"},
+                "form": res}
+    else:
+        res = forms.precision_form(log=log)
+        line = {"metric": "complex64 Lanczos: ground-state energy and S(q,w) error vs complex128",
+                "value": res["spectrum_l1_error_c64"], "unit": "spectral L1", "n_gpus": 1, "higher_is_better": False,
+                "dtype": "complex64", "data": "synthetic (model built from its definition)",
+                "config": {"workload": "2x14 ladder, 2 holes, Sz=0, k=(pi,0), t=1, J=0.4; S(q=(pi,pi),w), "
+                                       "300 steps, eta=0.05"},
+                "form": res}
+    line["seconds_total"] = time.perf_counter() - t0
+    print(json.dumps(line), flush=True)
 
 
 def time_steps(step, steps, warmup, stream):
@@ -190,11 +225,16 @@ def main():
     ap.add_argument("--no-allq", action="store_true", help="skip the all-q time-to-accuracy section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy one vector (known bytes) for PMC calibration")
+    ap.add_argument("--form", choices=["hv", "batching", "precision"], default="hv",
+                    help="hv: the headline H·v + all-q line; batching / precision: those forms' own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
               f"this build", file=sys.stderr)
         sys.exit(2)
+    if args.form != "hv":
+        run_form(args.form)
+        return
 
     import ctypes as ct
 
@@ -218,6 +258,8 @@ def main():
     t_store = time.perf_counter() - t0
     log(f"stored H: {hm.nnz} elements ({hm.nnz / D:.2f} per row), {hm.slots} slots, {hm.n_values} unique values, "
         f"{hm.bytes / 1e9:.3f} GB, built in {t_store:.3f}s")
+    tuned = hm.tune(0 if args.precision == "c128" else 1)
+    log(f"stored product tuned: tiles={tuned[0]} gathers in flight={tuned[1]} ({tuned[2]:.3f} ms)")
     rng = np.random.default_rng(0)
     dt = torch.complex128 if args.precision == "c128" else torch.complex64
     x = torch.as_tensor(rng.normal(size=D) + 1j * rng.normal(size=D), device="cuda").to(dt)
@@ -269,7 +311,8 @@ def main():
         "config": {"workload": "t-J ladder 2x12, 2 holes, Sz=0, k=(0,0), t=1, J=0.4, one H·v per step "
                                "(stored sliced-ELL H, built once before the timed region)",
                    "dim": D, "basis_build_s": round(t_basis, 4), "stored_build_s": round(t_store, 4),
-                   "stored_bytes": hm.bytes, "stored_elements_per_row": round(hm.nnz / D, 3)},
+                   "stored_bytes": hm.bytes, "stored_elements_per_row": round(hm.nnz / D, 3),
+                   "stored_tuned": {"tiles_per_workgroup": tuned[0], "gathers_in_flight": tuned[1]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("hm_spmv") if prec == 0 else None,
                      "kernel": "hm_spmv_kernel (+ reduce_partials_kernel), per launch (HIP events)",

@@ -200,9 +200,11 @@ def allq_plain(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=
     return time.perf_counter() - t0, out
 
 
-def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL, precision=1):
+def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL, precision=1,
+              terminator=True):
     """The fast form: stored H (built once per sector on the GPU), complex64 Lanczos vectors with FP64
-    coefficient reductions, fewer steps closed by the square-root terminator.  Returns (seconds, spectra)."""
+    coefficient reductions, fewer steps closed by the square-root terminator.  With precision=0 and
+    terminator=False it is the plain method on the stored H.  Returns (seconds, spectra)."""
     from exactdiag_amd.closure import sqrt_terminator
 
     _sync()
@@ -223,7 +225,7 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
             hm = sec.store(weights)
             a, b = hm.lanczos(phi, nsteps, precision=precision)
             hm.close()
-            at, bt, tail = sqrt_terminator(a, b, len(a)) if len(a) == nsteps else (a, b, None)
+            at, bt, tail = sqrt_terminator(a, b, len(a)) if (terminator and len(a) == nsteps) else (a, b, None)
             out[(qx, qy)] = cf_eval(at, bt, ws, e0, eta, -n2 / np.pi, tail=tail)
         sec.close()
     s0.close()

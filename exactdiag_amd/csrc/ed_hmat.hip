@@ -110,8 +110,12 @@ __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, 
     walk_row<4>(s, T, w, f, sink);
 }
 
-// y_f = sqrt(n_f) sum_k (+-V[code]) x_col + V[diag_f] x_f; partial per workgroup = sum conj(x_f) y_f.
-template <class VT>
+// y_f = sqrt(n_f) sum_k (+-V[code]) x_col + V[diag_f] x_f.  Persistent
+// workgroups walk `tiles` consecutive 256-row tiles (consecutive workgroups of
+// one XCD take neighbouring ranges, so their gathers share that XCD's L2);
+// U elements per lane are loaded, then their U gathers issued together.
+// partial[workgroup] = sum conj(x_f) y_f over its rows.
+template <class VT, int U>
 __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col,
                                                      const uint16_t* __restrict__ code,
@@ -120,32 +124,36 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
                                                      const uint8_t* __restrict__ orbit,
                                                      const double* __restrict__ sq, const double* __restrict__ vals,
                                                      int n_vals, const VT* __restrict__ x, VT* __restrict__ y,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial, int64_t tiles) {
     extern __shared__ __align__(16) unsigned char smem[];
     double* V = reinterpret_cast<double*>(smem);
     for (int i = threadIdx.x; i < 2 * n_vals; i += BS) V[i] = vals[i];
     __syncthreads();
-    const int64_t f = int64_t(blockIdx.x) * BS + threadIdx.x;
+    const int nwg = gridDim.x;
+    int wg = blockIdx.x;
+    const int per = nwg / 8;
+    if (per > 0 && wg < per * 8) wg = (wg & 7) * per + (wg >> 3);
+    const int64_t f0 = int64_t(wg) * tiles * BS;
+    const int64_t f1 = f0 + tiles * BS < D ? f0 + tiles * BS : D;
     C2 dot{0.0, 0.0};
-    if (f < D) {
-        const int64_t sl = f >> 6;
-        const int64_t base = sptr[sl] + (f & 63);
+    for (int64_t f = f0 + threadIdx.x; f < f1; f += BS) {
+        const int64_t base = sptr[f >> 6] + (f & 63);
         const int n = cnt[f];
         C2 acc{0.0, 0.0};
         int k = 0;
-        for (; k + 4 <= n; k += 4) {
-            int32_t j[4];
-            uint16_t c[4];
+        for (; k + U <= n; k += U) {
+            int32_t j[U];
+            uint16_t c[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 j[u] = col[base + int64_t(k + u) * 64];
                 c[u] = code[base + int64_t(k + u) * 64];
             }
-            C2 xv[4];
+            C2 xv[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = ldx(x, j[u]);
+            for (int u = 0; u < U; ++u) xv[u] = ldx(x, j[u]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const int vi = c[u] & 0x7fff;
                 const double sg = (c[u] & 0x8000) ? -1.0 : 1.0;
                 const C2 v{sg * V[2 * vi], sg * V[2 * vi + 1]};
@@ -168,7 +176,7 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
         o.x = decltype(o.x)(yf.re);
         o.y = decltype(o.y)(yf.im);
         y[f] = o;
-        dot = cmul(C2{xf.re, -xf.im}, yf);
+        dot = cadd(dot, cmul(C2{xf.re, -xf.im}, yf));
     }
     if (partial) {
         __shared__ double red[2 * BS];
@@ -183,8 +191,8 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            partial[2 * blockIdx.x] = red[0];
-            partial[2 * blockIdx.x + 1] = red[1];
+            partial[2 * wg] = red[0];
+            partial[2 * wg + 1] = red[1];
         }
     }
 }
@@ -205,6 +213,8 @@ struct ed_hmat {
     HvWeights w{};
     int64_t D = 0, nslices = 0, slots = 0, nnz = 0;
     int n_vals = 0;
+    int tiles = 0;   // row tiles per workgroup of the product (0 = default)
+    int unroll = 4;  // gathers in flight per lane (4 or 8)
     int64_t* d_sptr = nullptr;
     int32_t* d_col = nullptr;
     uint16_t* d_code = nullptr;
@@ -226,19 +236,43 @@ namespace ed {
 
 const ed_sector* hmat_sector(const ed_hmat* m) { return m->s; }
 
-int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st) {
-    const int64_t nwg = (m->D + BS - 1) / BS;
-    if (nwg == 0) return 0;
+int hmat_default_tiles(int64_t D) {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        ED_HIP(hipGetDevice(&dev));
+        ED_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const int64_t ntiles = (D + BS - 1) / BS;
+    const int64_t target = int64_t(cus) * 8;  // several workgroups per CU
+    return int(std::max<int64_t>(1, (ntiles + target - 1) / target));
+}
+
+template <class VT, int U>
+void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int64_t nwg, int64_t tiles,
+                 hipStream_t st) {
     const size_t lds = size_t(m->n_vals) * 16;
-    const ed_sector* s = m->s;
-    if (precision == 0)
-        hipLaunchKernelGGL(hm_spmv_kernel<double2>, dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col,
-                           m->d_code, m->d_cnt, m->d_diag, s->d_orbit, s->d_sqrt, m->d_vals, m->n_vals,
-                           static_cast<const double2*>(x), static_cast<double2*>(y), partial);
-    else
-        hipLaunchKernelGGL(hm_spmv_kernel<float2>, dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col,
-                           m->d_code, m->d_cnt, m->d_diag, s->d_orbit, s->d_sqrt, m->d_vals, m->n_vals,
-                           static_cast<const float2*>(x), static_cast<float2*>(y), partial);
+    hipLaunchKernelGGL((hm_spmv_kernel<VT, U>), dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col,
+                       m->d_code, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals, m->n_vals,
+                       static_cast<const VT*>(x), static_cast<VT*>(y), partial, tiles);
+}
+
+// Launches the product; returns the number of partials written (one per workgroup).
+int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st) {
+    if (m->D == 0) return 0;
+    const int64_t tiles = m->tiles > 0 ? m->tiles : hmat_default_tiles(m->D);
+    const int64_t nwg = ((m->D + BS - 1) / BS + tiles - 1) / tiles;
+    if (precision == 0) {
+        if (m->unroll == 8)
+            launch_spmv<double2, 8>(m, x, y, partial, nwg, tiles, st);
+        else
+            launch_spmv<double2, 4>(m, x, y, partial, nwg, tiles, st);
+    } else {
+        if (m->unroll == 8)
+            launch_spmv<float2, 8>(m, x, y, partial, nwg, tiles, st);
+        else
+            launch_spmv<float2, 4>(m, x, y, partial, nwg, tiles, st);
+    }
     ED_HIP(hipGetLastError());
     return nwg;
 }
@@ -363,6 +397,47 @@ int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots,
         if (nnz) *nnz = m->nnz;
         if (slots) *slots = m->slots;
         if (n_values) *n_values = m->n_vals;
+    });
+}
+
+int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll,
+                 double* best_ms, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && d_x && d_y, ED_ERR_ARG, "null argument");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        double best = 1e300;
+        int bt = 0, bu = 4;
+        if (m->D > 0) {
+            hipEvent_t e0, e1;
+            ED_HIP(hipEventCreate(&e0));
+            ED_HIP(hipEventCreate(&e1));
+            for (int t : {1, 2, 4, 8, 16, 32}) {
+                for (int u : {4, 8}) {
+                    m->tiles = t;
+                    m->unroll = u;
+                    launch_hmat(m, d_x, d_y, m->d_partial, precision, st);
+                    ED_HIP(hipEventRecord(e0, st));
+                    for (int r = 0; r < 3; ++r) launch_hmat(m, d_x, d_y, m->d_partial, precision, st);
+                    ED_HIP(hipEventRecord(e1, st));
+                    ED_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    ED_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms / 3 < best) {
+                        best = ms / 3;
+                        bt = t;
+                        bu = u;
+                    }
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        m->tiles = bt;
+        m->unroll = bu;
+        if (tiles) *tiles = bt;
+        if (unroll) *unroll = bu;
+        if (best_ms) *best_ms = best < 1e300 ? best : 0.0;
     });
 }
 

@@ -1,0 +1,214 @@
+"""The batching and precision forms, measured end to end.
+
+* :func:`batching_form` -- every momentum sector of a ladder for several
+  couplings as uneven Lanczos instances of one batch (lowest eigenpair each,
+  restarted two-pass Lanczos in HIP-graph launches), then the one-particle
+  spectral functions A+(k, w) and A-(k, w) of each coupling's ground state
+  for every k in one more batch; the same work done one sector at a time
+  (the reference's loops, tJ_spin_half_ladder/api.py:45-59 and 135-161) for
+  the timing and the agreement.
+* :func:`precision_form` -- ground state and S(q, w) of one sector with
+  complex64 Lanczos vectors and H·v (FP64 coefficient reductions,
+  tridiagonal solve and continued fraction) against the same computation in
+  complex128; every energy and spectrum error stated.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+from scipy.linalg import eigh_tridiagonal
+
+from exactdiag_amd import _lib
+from exactdiag_amd.allq import final_momentum, spectral_l1
+from exactdiag_amd.general import krylov
+from exactdiag_amd.sector import Sector, cf_eval, weights_array
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=30, seed=0):
+    """Lowest eigenpair of a stored H by restarted two-pass Lanczos in the given precision.
+
+    Pass one gives the coefficients, the tridiagonal's lowest eigenvector y; pass two reruns the recurrence and
+    accumulates x = sum_n y_n v_n in FP64; restart from x until ||H x - E x|| <= tol * max(1, |E|), E the FP64
+    Rayleigh quotient.  Returns (E, x, restarts, residual).
+    """
+    import torch
+
+    dim = hm.dim
+    v = torch.as_tensor(krylov.start_vector(dim, seed), device="cuda")
+    res = np.inf
+    for restart in range(max_restarts):
+        a, b = hm.lanczos(v, min(nsteps, dim), precision=precision)
+        if len(a) == 1:
+            y = np.ones(1)
+        else:
+            _, Y = eigh_tridiagonal(a, b[1:], select="i", select_range=(0, 0))
+            y = Y[:, 0]
+        x = hm.lanczos_ritz(v, y, precision=precision)
+        krylov.vec_scale(x, 1.0 / krylov.vec_norm(x))
+        hx, e = hm.hv(x, with_dot=True)
+        e = e.real
+        r = hx.clone()
+        krylov.vec_axpys(x.view(1, -1), 1, [e], r)
+        res = krylov.vec_norm(r)
+        if res <= tol * max(1.0, abs(e)) or len(a) < min(nsteps, dim):
+            return e, x, restart, res
+        v = x
+    return e, x, max_restarts, res
+
+
+def precision_form(L=14, h=2, nd=13, k=(7, 0), q=(7, 1), weights=(1.0, 1.0, 0.4, 0.4), nsteps=300,
+                   ws=np.linspace(0, 4, 2001), eta=0.05, log=print):
+    """complex64 vs complex128: ground-state energy and S(q, w) errors of one sector (stored H)."""
+    w = list(weights)
+    t0 = time.perf_counter()
+    s0 = Sector(L, h, nd, k[0], k[1])
+    hm0 = s0.store(w)
+    kx, ky = final_momentum(L, k[0], k[1], q[0], q[1])
+    s1 = Sector(L, h, nd, kx, ky)
+    hm1 = s1.store(w)
+    _sync()
+    t_setup = time.perf_counter() - t0
+    log(f"precision form: sectors {s0.dim} and {s1.dim} states, stored H {(hm0.bytes + hm1.bytes) / 1e9:.1f} GB, "
+        f"set up in {t_setup:.1f}s")
+    out = {"dims": [s0.dim, s1.dim], "setup_s": t_setup, "stored_gb": (hm0.bytes + hm1.bytes) / 1e9}
+    spectra = {}
+    for name, prec, tol in (("c128", 0, 1e-10), ("c64", 1, 1e-5)):
+        t1 = time.perf_counter()
+        e0, psi, restarts, res = two_pass_ground_state(hm0, precision=prec, tol=tol)
+        _sync()
+        t_gs = time.perf_counter() - t1
+        phi = s0.op_apply(s1, _lib.OP_SZQ, q[0], q[1], psi)
+        n2 = krylov.vec_norm(phi) ** 2
+        t2 = time.perf_counter()
+        a, b = hm1.lanczos(phi, nsteps, precision=prec)
+        _sync()
+        t_lz = time.perf_counter() - t2
+        spectra[name] = cf_eval(a, b, ws, e0, eta, -n2 / np.pi)
+        out[name] = {"e0": e0, "gs_restarts": restarts, "gs_residual": res, "gs_seconds": t_gs,
+                     "lanczos_seconds": t_lz, "steps": len(a), "norm2": n2}
+        log(f"  {name}: E0 = {e0:.12f} ({restarts} restarts, residual {res:.1e}) in {t_gs:.1f}s; "
+            f"{len(a)} Lanczos steps in {t_lz:.1f}s")
+    out["e0_abs_error_c64"] = abs(out["c64"]["e0"] - out["c128"]["e0"])
+    out["e0_rel_error_c64"] = out["e0_abs_error_c64"] / abs(out["c128"]["e0"])
+    out["spectrum_l1_error_c64"] = spectral_l1(spectra["c64"], spectra["c128"], ws)
+    out["spectrum_max_rel_error_c64"] = float(np.max(np.abs(spectra["c64"] - spectra["c128"]))
+                                              / np.max(np.abs(spectra["c128"])))
+    for obj in (hm1, hm0, s1, s0):
+        obj.close()
+    return out
+
+
+def _spectral_excitations(L, h, nd, k0):
+    """(name, op, qx, qy, final sector params) for A+ and A- at every k (matrix_setup.pyx:281-330)."""
+    N = 2 * L
+    tsp = (N - h) - 2 * nd
+    out = []
+    for qx in range(L):
+        for qy in range(2):
+            if h > 0:
+                ndf, op = (nd + 1, _lib.OP_APLUS_DOWN) if tsp > 0 else (nd, _lib.OP_APLUS_UP)
+                kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
+                out.append(("A+", op, qx, qy, (L, h - 1, ndf, kx, ky)))
+            if h < N:
+                ndf, op = (nd - 1, _lib.OP_AMINUS_DOWN) if tsp <= 0 else (nd, _lib.OP_AMINUS_UP)
+                kx, ky = final_momentum(L, k0[0], k0[1], -qx, (-qy) % 2)
+                if 0 <= ndf <= N - h - 1:
+                    out.append(("A-", op, -qx, (-qy) % 2, (L, h + 1, ndf, kx, ky)))
+    return out
+
+
+def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspace(-6, 6, 2001), eta=0.05,
+                  tol=1e-9, serial=True, log=print):
+    """All k sectors x couplings as one batch, then A(k, w) for every k as another; optionally the serial runs."""
+    import torch
+
+    from exactdiag_amd.batch import Batch, Excitation, lowest_eigenpairs, spectra
+
+    ks = [(kx, ky) for kx in range(L) for ky in range(2)]
+    t0 = time.perf_counter()
+    sectors = {kk: Sector(L, h, nd, kk[0], kk[1]) for kk in ks}
+    _sync()
+    t_basis = time.perf_counter() - t0
+    weights = [[1.0, 1.0, J, J] for J in Js]
+    inst = [(J, kk) for J in Js for kk in ks]
+    batch = Batch([sectors[kk] for _, kk in inst], [[1.0, 1.0, J, J] for J, _ in inst])
+    tiles, gather, ms = batch.tune()
+    log(f"batching form: {len(inst)} instances, dims {int(batch.dims.min())}..{int(batch.dims.max())}, total "
+        f"{batch.total}; tuned tiles={tiles} gather={gather} ({ms:.2f} ms per batched H·v)")
+    _sync()
+    t1 = time.perf_counter()
+    energies, x, restarts = lowest_eigenpairs(batch, tol=tol)
+    _sync()
+    t_gs_batch = time.perf_counter() - t1
+    log(f"  batched lowest eigenpairs: {t_gs_batch:.2f}s ({restarts} restarts)")
+    out = {"instances": len(inst), "dims": [int(d) for d in batch.dims], "basis_s": t_basis,
+           "tuned": {"tiles": tiles, "gather": gather, "ms_per_batched_hv": ms},
+           "gs_batched_s": t_gs_batch, "gs_restarts": restarts}
+    if serial:
+        t1 = time.perf_counter()
+        e_serial = []
+        for J, kk in inst:
+            s = sectors[kk]
+            wa = weights_array([1.0, 1.0, J, J])
+            vals, _, _ = krylov.lowest_eigenpairs(lambda v, s=s, wa=wa: s.hv(wa, v), s.dim, 1, tol=tol)
+            e_serial.append(float(vals[0]))
+        _sync()
+        out["gs_serial_s"] = time.perf_counter() - t1
+        out["gs_max_abs_diff"] = float(np.max(np.abs(np.array(e_serial) - energies)))
+        log(f"  serial lowest eigenpairs: {out['gs_serial_s']:.2f}s, max |dE| {out['gs_max_abs_diff']:.2e}")
+    # ground state per coupling: the lowest of its sectors
+    exc_params = {}
+    spec_out = {}
+    t_spec_b = t_spec_s = 0.0
+    max_diff = 0.0
+    for ji, J in enumerate(Js):
+        idx = [i for i, (JJ, _) in enumerate(inst) if JJ == J]
+        i0 = idx[int(np.argmin(energies[idx]))]
+        k0 = inst[i0][1]
+        e0 = float(energies[i0])
+        psi = batch.part(x, i0).clone()
+        exc = []
+        for name, op, qx, qy, prm in _spectral_excitations(L, h, nd, k0):
+            if prm not in exc_params:
+                exc_params[prm] = Sector(*prm)
+            exc.append(Excitation(name, op, qx, qy, exc_params[prm]))
+        _sync()
+        t1 = time.perf_counter()
+        got, coeffs = spectra(exc, sectors[k0], psi, e0, weights[ji], ws, eta, nsteps)
+        _sync()
+        t_spec_b += time.perf_counter() - t1
+        spec_out[J] = {"k0": k0, "e0": e0, "n_spectra": len(exc)}
+        if serial:
+            t1 = time.perf_counter()
+            wa = weights_array(weights[ji])
+            for ex, g in zip(exc, got):
+                phi = sectors[k0].op_apply(ex.sector, ex.op, ex.qx, ex.qy, psi)
+                n2 = krylov.vec_norm(phi) ** 2 if ex.sector.dim else 0.0
+                if n2 < 1e-16:
+                    continue
+                a, b = ex.sector.lanczos(wa, phi, nsteps)
+                ref = cf_eval(a, b, ws, e0, eta, -n2 / np.pi)
+                max_diff = max(max_diff, spectral_l1(g, ref, ws))
+            _sync()
+            t_spec_s += time.perf_counter() - t1
+    out["spectra"] = {str(J): v for J, v in spec_out.items()}
+    out["spectra_batched_s"] = t_spec_b
+    if serial:
+        out["spectra_serial_s"] = t_spec_s
+        out["spectra_max_l1_batched_vs_serial"] = max_diff
+    log(f"  A(k,w): batched {t_spec_b:.2f}s" + (f", serial {t_spec_s:.2f}s, max L1 diff {max_diff:.1e}"
+                                                   if serial else ""))
+    batch.close()
+    for s in list(exc_params.values()) + list(sectors.values()):
+        s.close()
+    del x
+    torch.cuda.empty_cache()
+    return out

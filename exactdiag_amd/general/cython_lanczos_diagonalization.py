@@ -26,7 +26,7 @@ def _as_device(v):
 
 def _lanczos_get_anbn(v0, num_lanczos_vecs, hamiltonian, precision=0):
     """(a_n, b_n) with b_0 = 0 (cython_lanczos_diagonalization.pyx:50-66)."""
-    return hamiltonian.sector.lanczos(hamiltonian._w, _as_device(v0), num_lanczos_vecs, precision=precision)
+    return hamiltonian.lanczos(_as_device(v0), num_lanczos_vecs, precision=precision)
 
 
 def _spectral_value_at_z(ans, bns, z):

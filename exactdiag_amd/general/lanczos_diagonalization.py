@@ -89,7 +89,7 @@ def diagonalize_IRLM(num_eigenpairs, operator, is_symmetric=False, num_added_vec
     D = operator.shape[1]
 
     def apply(v):
-        return operator.sector.hv(operator._w, v)
+        return operator.apply(v)
 
     ncv = max(2 * num_eigenpairs + 24, 40, num_eigenpairs + num_added_vecs)
     vals, vecs, it = krylov.lowest_eigenpairs(apply, D, num_eigenpairs, tol=tol, ncv=ncv, v0=initial_guess)

@@ -2,9 +2,12 @@
 
 The reference stores each Hamiltonian term as a CSR on disk and sums them in
 ``Added_Sparse_Matrices.dot`` (general/sparse_matrices.py:210-243).  Here the
-Hamiltonian is applied matrix-free by the HIP kernel; the objects keep the
-``.dot``, ``.shape``, ``.weights``, ``.toarray()`` surface.  ``dot`` accepts
-numpy arrays (copied to and from the device) or device tensors (kept there).
+weighted sum is stored once in HBM (sliced ELL, built on the GPU the first
+time the operator is applied) when the sector is large enough for that to
+pay and the device has room for it; otherwise it is applied matrix-free.
+Both are HIP kernels of the library.  The objects keep the ``.dot``,
+``.shape``, ``.weights``, ``.toarray()`` surface.  ``dot`` accepts numpy
+arrays (copied to and from the device) or device tensors (kept there).
 """
 
 from __future__ import annotations
@@ -14,6 +17,14 @@ import numpy as np
 from exactdiag_amd.sector import TERMS, Sector, weights_array
 
 VALUE_TYPE = np.complex128
+STORED_MIN_DIM = 2048  # smaller sectors: the build does not pay for itself
+
+
+def stored_bytes_bound(sector) -> int:
+    """Upper bound of the stored H's size: every row as long as the longest possible one."""
+    N, L, h = sector.N, sector.L, sector.h
+    longest = N + L + 3 * min(h, N - h)
+    return int(sector.dim * (3 + 6 * longest) + 8 * (sector.dim // 64 + 1))
 
 
 def _to_device(vec):
@@ -49,22 +60,48 @@ class Hamiltonian_Operator:
         self._w = weights_array(self.weights)
         self.shape = sector.shape
         self.num_matrices = len(self.names)
+        self._stored = None
+        self.use_stored = True
+
+    @property
+    def stored(self):
+        """The stored form, built on first use, or None (small sector, or not enough free HBM)."""
+        if self._stored is None and self.use_stored and self.shape[0] >= STORED_MIN_DIM:
+            import torch
+
+            free, _ = torch.cuda.mem_get_info()
+            if stored_bytes_bound(self.sector) < 0.5 * free:
+                self._stored = self.sector.store(self._w)
+            else:
+                self.use_stored = False
+        return self._stored
+
+    def apply(self, x):
+        """H x for a device vector (complex128 or complex64)."""
+        m = self.stored
+        return m.hv(x) if m is not None else self.sector.hv(self._w, x)
+
+    def lanczos(self, v0, nsteps, precision=0):
+        """Plain Lanczos coefficients (a, b) from the device vector v0."""
+        m = self.stored
+        if m is not None:
+            return m.lanczos(v0, nsteps, precision=precision)
+        return self.sector.lanczos(self._w, v0, nsteps, precision=precision)
 
     def dot(self, vec):
         if self.shape[0] == 0:
             return np.zeros(0, VALUE_TYPE)
         x, kind = _to_device(vec)
-        if x.dtype not in (np.complex64, np.complex128):
-            import torch
+        import torch
 
-            if x.dtype not in (torch.complex128, torch.complex64):
-                x = x.to(torch.complex128)
-        y = self.sector.hv(self._w, x)
-        return _from_device(y, kind)
+        if x.dtype not in (torch.complex128, torch.complex64):
+            x = x.to(torch.complex128)
+        return _from_device(self.apply(x.contiguous()), kind)
 
     def dot_with_inner(self, x):
         """(H x, <x, H x>) with the inner product fused into the kernel."""
-        return self.sector.hv(self._w, x, with_dot=True)
+        m = self.stored
+        return m.hv(x, with_dot=True) if m is not None else self.sector.hv(self._w, x, with_dot=True)
 
     def matrix(self):
         """Dense matrix of ``dot`` (small sectors only): columns H e_k from the GPU kernel."""

@@ -255,6 +255,17 @@ class StoredH:
             return y, complex(dot.cpu().numpy()[0])
         return y
 
+    def tune(self, precision: int = 0):
+        """Autotune the stored product (tiles per workgroup, gathers in flight); returns (tiles, unroll, ms)."""
+        torch = _torch()
+        dt = torch.complex128 if precision == 0 else torch.complex64
+        x = torch.ones(max(self.dim, 1), dtype=dt, device=self.device)
+        y = torch.empty_like(x)
+        t, u, ms = ct.c_int(0), ct.c_int(0), ct.c_double(0.0)
+        check(lib().ed_hmat_tune(self._h, _ptr(x), _ptr(y), int(precision), ct.byref(t), ct.byref(u), ct.byref(ms),
+                                 _stream()))
+        return t.value, u.value, ms.value
+
     def lanczos(self, v0, nsteps: int, precision: int = 0):
         torch = _torch()
         v0 = v0.to(torch.complex128).contiguous()

@@ -133,3 +133,17 @@ def test_two_pass_ground_state(torch, oracle_mod):
     assert abs(th[0] - ev[0]) < 1e-10
     r = m.hv(x) - th[0] * x
     assert float(r.norm()) < 1e-8
+
+
+def test_stored_tuner_keeps_results(torch):
+    from exactdiag_amd.sector import Sector
+
+    s = Sector(8, 2, 7, 1, 0)
+    m = s.store(WEIGHTS[3])
+    x = _rand(torch, s.dim, 8)
+    y0, d0 = m.hv(x, with_dot=True)
+    t, u, ms = m.tune()
+    assert t in (1, 2, 4, 8, 16, 32) and u in (4, 8) and ms > 0
+    y1, d1 = m.hv(x, with_dot=True)
+    assert torch.equal(y0, y1)  # rows are computed the same way whatever the tiling
+    assert abs(d1 - d0) <= 1e-12 * max(1.0, abs(d0))
