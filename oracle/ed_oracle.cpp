@@ -821,9 +821,13 @@ int orc_build_term_csr_rows(void* p, int term, int64_t lo, int64_t hi, int threa
     }
 }
 
-int64_t orc_term_csr_sizes(void* p, int term, int64_t* n_unique) {
+// Sizes of the stored CSR of one term: returns the stored elements; n_unique
+// receives the unique values and n_rows the rows it covers (all rows, or the
+// rows of the last orc_build_term_csr_rows call).
+int64_t orc_term_csr_sizes(void* p, int term, int64_t* n_unique, int64_t* n_rows) {
     auto& m = static_cast<Handle*>(p)->ham.term[term];
     *n_unique = int64_t(m.values.size());
+    *n_rows = m.major.empty() ? 0 : int64_t(m.major.size()) - 1;
     return int64_t(m.minor.size());
 }
 

@@ -72,7 +72,7 @@ def lib():
         L.orc_build_term_csr.argtypes = [ct.c_void_p, ct.c_int, ct.c_int]
         L.orc_build_term_csr_rows.argtypes = [ct.c_void_p, ct.c_int, ct.c_int64, ct.c_int64, ct.c_int]
         L.orc_term_csr_sizes.restype = ct.c_int64
-        L.orc_term_csr_sizes.argtypes = [ct.c_void_p, ct.c_int, _i64p]
+        L.orc_term_csr_sizes.argtypes = [ct.c_void_p, ct.c_int, _i64p, _i64p]
         L.orc_term_csr_copy.argtypes = [ct.c_void_p, ct.c_int, _dp, _i16p, _i64p, _i64p]
         L.orc_hv.argtypes = [ct.c_void_p, _dp, _dp, _dp, ct.c_int]
         L.orc_cf.argtypes = [_dp, _dp, ct.c_int, _dp, ct.c_int, _dp]
@@ -206,11 +206,12 @@ class OracleSector:
     def term_csr(self, term):
         self.build_csr((term,))
         nu = np.zeros(1, np.int64)
-        nnz = self._lib.orc_term_csr_sizes(self._h, term, _ptr(nu, _i64p))
+        nr = np.zeros(1, np.int64)
+        nnz = self._lib.orc_term_csr_sizes(self._h, term, _ptr(nu, _i64p), _ptr(nr, _i64p))
         values = np.empty(int(nu[0]), np.complex128)
         vinds = np.empty(nnz, np.int16)
         minor = np.empty(nnz, np.int64)
-        major = np.empty(self.dim + 1, np.int64)
+        major = np.empty(int(nr[0]) + 1, np.int64)  # the rows the stored CSR covers
         self._lib.orc_term_csr_copy(self._h, term, _ptr(values.view(np.float64), _dp), _ptr(vinds, _i16p),
                                     _ptr(minor, _i64p), _ptr(major, _i64p))
         return values, vinds, minor, major
