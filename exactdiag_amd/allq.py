@@ -135,6 +135,7 @@ def run_allq(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), precision=0, limited
 # -- time to all-q S(q, w) at a given accuracy ----------------------------------
 
 REFERENCE_EIG_TOL = 2e-7  # the reference's ARPACK tolerance (general/lanczos_diagonalization.py:89)
+LAST_FAST_TIMINGS = {}  # seconds per stage of the last allq_fast run
 
 
 def _sync():
@@ -144,7 +145,7 @@ def _sync():
 
 
 def _gs(apply, dim, tol, seed=0):
-    vals, vecs, _ = krylov.lowest_eigenpairs(apply, dim, 1, tol=tol, v0=krylov.start_vector(dim, seed))
+    vals, vecs, _ = krylov.lowest_eigenpairs(apply, dim, 1, tol=tol, v0=krylov.device_start_vector(dim, seed))
     return float(vals[0]), vecs[0]
 
 
@@ -201,35 +202,60 @@ def allq_plain(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=
 
 
 def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL, precision=1,
-              terminator=True):
+              terminator=True, two_pass_gs=True):
     """The fast form: stored H (built once per sector on the GPU), complex64 Lanczos vectors with FP64
-    coefficient reductions, fewer steps closed by the square-root terminator.  With precision=0 and
-    terminator=False it is the plain method on the stored H.  Returns (seconds, spectra)."""
+    coefficient reductions, fewer steps closed by the square-root terminator; the ground state by restarted
+    two-pass Lanczos in complex128 (no re-orthogonalisation, the same residual criterion as the plain run).  With
+    precision=0, terminator=False and two_pass_gs=False it is the plain method on the stored H.
+    Returns (seconds, spectra)."""
     from exactdiag_amd.closure import sqrt_terminator
 
+    tm = dict.fromkeys(("ground_state", "bases", "stored_h", "operator", "lanczos", "continued_fraction"), 0.0)
+
+    def lap(key, t):
+        _sync()
+        now = time.perf_counter()
+        tm[key] += now - t
+        return now
+
     _sync()
-    t0 = time.perf_counter()
+    t0 = t = time.perf_counter()
     s0 = Sector(L, h, nd, k0[0], k0[1])
+    t = lap("bases", t)
     hm0 = s0.store(weights)
-    e0, psi = _gs(hm0.hv, s0.dim, tol)
+    t = lap("stored_h", t)
+    if two_pass_gs:
+        from exactdiag_amd.forms import two_pass_ground_state
+
+        e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, tol=tol)
+    else:
+        e0, psi = _gs(hm0.hv, s0.dim, tol)
     hm0.close()
+    t = lap("ground_state", t)
     out = {}
     for qx, qy in distinct_qs(L, limited):
         kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
         sec = Sector(L, h, nd, kx, ky)
+        t = lap("bases", t)
         phi = s0.op_apply(sec, _lib.OP_SZQ, qx, qy, psi)
         n2 = krylov.vec_norm(phi) ** 2 if sec.dim else 0.0
+        t = lap("operator", t)
         if n2 < 1e-16:
             out[(qx, qy)] = np.zeros(len(ws))
         else:
             hm = sec.store(weights)
+            t = lap("stored_h", t)
             a, b = hm.lanczos(phi, nsteps, precision=precision)
             hm.close()
+            t = lap("lanczos", t)
             at, bt, tail = sqrt_terminator(a, b, len(a)) if (terminator and len(a) == nsteps) else (a, b, None)
             out[(qx, qy)] = cf_eval(at, bt, ws, e0, eta, -n2 / np.pi, tail=tail)
+            t = lap("continued_fraction", t)
         sec.close()
     s0.close()
     _sync()
+    LAST_FAST_TIMINGS.clear()
+    LAST_FAST_TIMINGS.update(tm)
     return time.perf_counter() - t0, out
 
 

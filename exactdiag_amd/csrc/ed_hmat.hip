@@ -236,17 +236,9 @@ namespace ed {
 
 const ed_sector* hmat_sector(const ed_hmat* m) { return m->s; }
 
-int hmat_default_tiles(int64_t D) {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        ED_HIP(hipGetDevice(&dev));
-        ED_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    }
-    const int64_t ntiles = (D + BS - 1) / BS;
-    const int64_t target = int64_t(cus) * 8;  // several workgroups per CU
-    return int(std::max<int64_t>(1, (ntiles + target - 1) / target));
-}
+// One 256-row tile per workgroup: the fastest setting measured for the 2x12
+// sector (ed_hmat_tune, product plus the reduction of its partials).
+int hmat_default_tiles(int64_t) { return 1; }
 
 template <class VT, int U>
 void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int64_t nwg, int64_t tiles,
@@ -258,6 +250,8 @@ void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int6
 }
 
 // Launches the product; returns the number of partials written (one per workgroup).
+int64_t hmat_max_partials(const ed_hmat* m) { return std::max<int64_t>((m->D + BS - 1) / BS, 1); }
+
 int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st) {
     if (m->D == 0) return 0;
     const int64_t tiles = m->tiles > 0 ? m->tiles : hmat_default_tiles(m->D);
@@ -338,7 +332,8 @@ int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* str
             m->nslices = (D + 63) / 64;
             m->d_cnt = dev_alloc<uint8_t>(size_t(std::max<int64_t>(D, 1)));
             m->d_diag = dev_alloc<uint16_t>(size_t(std::max<int64_t>(D, 1)));
-            m->d_partial = dev_alloc<double>(size_t(2 * std::max<int64_t>((D + BS - 1) / BS, 1)));
+            // partials of the product (one per workgroup, at most one per tile) + one dot for the tuner
+            m->d_partial = dev_alloc<double>(size_t(2 * std::max<int64_t>((D + BS - 1) / BS, 1) + 2));
             std::vector<int64_t> sptr(static_cast<size_t>(m->nslices + 1), 0);
             if (D > 0) {
                 uint16_t* d_map = dev_upload(map, st);
@@ -416,9 +411,14 @@ int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int*
                 for (int u : {4, 8}) {
                     m->tiles = t;
                     m->unroll = u;
-                    launch_hmat(m, d_x, d_y, m->d_partial, precision, st);
+                    // the product with its fused dot, as ed_hmat_hv runs it
+                    auto run = [&] {
+                        const int64_t np = launch_hmat(m, d_x, d_y, m->d_partial, precision, st);
+                        launch_reduce_partials(m->d_partial, np, m->d_partial + 2 * hmat_max_partials(m), st);
+                    };
+                    run();
                     ED_HIP(hipEventRecord(e0, st));
-                    for (int r = 0; r < 3; ++r) launch_hmat(m, d_x, d_y, m->d_partial, precision, st);
+                    for (int r = 0; r < 3; ++r) run();
                     ED_HIP(hipEventRecord(e1, st));
                     ED_HIP(hipEventSynchronize(e1));
                     float ms = 0.f;

@@ -42,12 +42,16 @@ __device__ inline void stc(VT* p, int64_t i, double re, double im) {
     p[i] = o;
 }
 
+constexpr int RBS = 1024;     // threads of the one-workgroup reductions
+constexpr int UPD_WGS = 2048;  // workgroups of the grid-stride vector kernels (fewer partials to reduce)
+
+template <int NT = BS>
 __device__ inline void block_sum2(double& re, double& im) {
-    __shared__ double red[2 * BS];
+    __shared__ double red[2 * NT];
     red[2 * threadIdx.x] = re;
     red[2 * threadIdx.x + 1] = im;
     __syncthreads();
-    for (int st = BS / 2; st > 0; st >>= 1) {
+    for (int st = NT / 2; st > 0; st >>= 1) {
         if (int(threadIdx.x) < st) {
             red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
             red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
@@ -60,24 +64,23 @@ __device__ inline void block_sum2(double& re, double& im) {
 
 template <class VT>
 __global__ void __launch_bounds__(BS) copy_in_kernel(const double* src, VT* dst, int64_t D, double* partial) {
-    const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
     double n2 = 0.0;
-    if (i < D) {
+    for (int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x; i < D; i += int64_t(gridDim.x) * BS) {
         const double re = src[2 * i], im = src[2 * i + 1];
         stc(dst, i, re, im);
         double r2, i2;
         ldc(dst, i, r2, i2);
-        n2 = r2 * r2 + i2 * i2;
+        n2 += r2 * r2 + i2 * i2;
     }
     double z = 0.0;
     block_sum2(n2, z);
     if (threadIdx.x == 0) partial[2 * blockIdx.x] = n2, partial[2 * blockIdx.x + 1] = 0.0;
 }
 
-__global__ void lz_init_kernel(LzDev lz, const double* partial, int64_t n) {
+__global__ void __launch_bounds__(RBS) lz_init_kernel(LzDev lz, const double* partial, int64_t n) {
     double re = 0.0, im = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += BS) re += partial[2 * k];
-    block_sum2(re, im);
+    for (int64_t k = threadIdx.x; k < n; k += RBS) re += partial[2 * k];
+    block_sum2<RBS>(re, im);
     if (threadIdx.x == 0) {
         lz.b[0] = 0.0;
         lz.s[0] = 1.0 / std::sqrt(re);
@@ -86,14 +89,14 @@ __global__ void lz_init_kernel(LzDev lz, const double* partial, int64_t n) {
     }
 }
 
-__global__ void lz_reduce_a_kernel(LzDev lz, const double* partial, int64_t n, int step) {
+__global__ void __launch_bounds__(RBS) lz_reduce_a_kernel(LzDev lz, const double* partial, int64_t n, int step) {
     if (lz.flag[0]) return;
     double re = 0.0, im = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += BS) {
+    for (int64_t k = threadIdx.x; k < n; k += RBS) {
         re += partial[2 * k];
         im += partial[2 * k + 1];
     }
-    block_sum2(re, im);
+    block_sum2<RBS>(re, im);
     if (threadIdx.x == 0) {
         const double sn = lz.s[step];
         lz.a[step] = sn * sn * re;
@@ -106,22 +109,19 @@ __global__ void __launch_bounds__(BS) lz_update_kernel(LzDev lz, int step, int64
                                                        VT* __restrict__ basis, double2* __restrict__ acc,
                                                        const double* __restrict__ ritz, double* partial) {
     if (lz.flag[0]) return;
-    const int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x;
     const double sn = lz.s[step], an = lz.a[step], bn = lz.b[step];
     const double snm = step > 0 ? lz.s[step - 1] : 0.0;
+    const double c = acc ? ritz[step] * sn : 0.0;
     double n2 = 0.0;
-    if (acc && i < D) {  // second pass: x += y_n v_n in FP64
-        double ur, ui;
-        ldc(u, i, ur, ui);
-        const double c = ritz[step] * sn;
-        const double2 av = acc[i];
-        acc[i] = double2{av.x + c * ur, av.y + c * ui};
-    }
-    if (i < D) {
+    for (int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x; i < D; i += int64_t(gridDim.x) * BS) {
         double yr, yi, ur, ui, pr, pi;
         ldc(y, i, yr, yi);
         ldc(u, i, ur, ui);
         ldc(up, i, pr, pi);
+        if (acc) {  // second pass: x += y_n v_n in FP64
+            const double2 av = acc[i];
+            acc[i] = double2{av.x + c * ur, av.y + c * ui};
+        }
         const double vr = sn * ur, vi = sn * ui;
         const double hr = sn * yr, hi = sn * yi;
         const double qr = snm * pr, qi = snm * pi;
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(BS) lz_update_kernel(LzDev lz, int step, int64
         if (basis) stc(basis, i, vr, vi);
         double r2, i2;
         ldc(up, i, r2, i2);
-        n2 = r2 * r2 + i2 * i2;
+        n2 += r2 * r2 + i2 * i2;
     }
     double z = 0.0;
     block_sum2(n2, z);
@@ -141,11 +141,11 @@ __global__ void __launch_bounds__(BS) lz_update_kernel(LzDev lz, int step, int64
     }
 }
 
-__global__ void lz_reduce_b_kernel(LzDev lz, const double* partial, int64_t n, int step) {
+__global__ void __launch_bounds__(RBS) lz_reduce_b_kernel(LzDev lz, const double* partial, int64_t n, int step) {
     if (lz.flag[0]) return;
     double re = 0.0, im = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += BS) re += partial[2 * k];
-    block_sum2(re, im);
+    for (int64_t k = threadIdx.x; k < n; k += RBS) re += partial[2 * k];
+    block_sum2<RBS>(re, im);
     if (threadIdx.x == 0) {
         const double bnp = std::sqrt(re);
         lz.b[step + 1] = bnp;
@@ -263,6 +263,7 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
                    const double* ritz = nullptr, const ed_hmat* m = nullptr) {
     const int64_t D = s->D;
     const int64_t nwg = (D + BS - 1) / BS;
+    const int64_t nupd = std::min<int64_t>(nwg, UPD_WGS);  // grid-stride update: at most UPD_WGS partials
     const HvWeights hw = make_weights(w);
     VT* buf[3] = {dev_alloc<VT>(size_t(D)), dev_alloc<VT>(size_t(D)), dev_alloc<VT>(size_t(D))};
     double* partial = dev_alloc<double>(size_t(2 * std::max<int64_t>(nwg, hv_partials(s))));
@@ -288,17 +289,17 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
             hipLaunchKernelGGL(zero2_kernel, dim3(unsigned(nwg)), dim3(BS), 0, st, acc, D);
         }
         ED_HIP(hipMemsetAsync(up, 0, size_t(D) * sizeof(VT), st));
-        hipLaunchKernelGGL((copy_in_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, d_v0, u, D, partial);
-        hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(BS), 0, st, lz, partial, nwg);
+        hipLaunchKernelGGL((copy_in_kernel<VT>), dim3(unsigned(nupd)), dim3(BS), 0, st, d_v0, u, D, partial);
+        hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(RBS), 0, st, lz, partial, nupd);
         ED_HIP(hipGetLastError());
         int flag[2] = {0, 0};
         for (int n = 0; n < nsteps; ++n) {
             const int64_t np =
                 m ? launch_hmat(m, u, y, partial, precision, st) : launch_hv(s, hw, u, y, partial, precision, 1, st);
-            hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(BS), 0, st, lz, partial, np, n);
-            hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nwg)), dim3(BS), 0, st, lz, n, D, y, u, up,
+            hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(RBS), 0, st, lz, partial, np, n);
+            hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nupd)), dim3(BS), 0, st, lz, n, D, y, u, up,
                                d_basis ? d_basis + int64_t(n) * D : nullptr, acc, d_ritz, partial);
-            hipLaunchKernelGGL(lz_reduce_b_kernel, dim3(1), dim3(BS), 0, st, lz, partial, nwg, n);
+            hipLaunchKernelGGL(lz_reduce_b_kernel, dim3(1), dim3(RBS), 0, st, lz, partial, nupd, n);
             ED_HIP(hipGetLastError());
             VT* t = up;  // w (u_{n+1}) now lives in up
             up = u;

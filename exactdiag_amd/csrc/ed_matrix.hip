@@ -616,17 +616,18 @@ __global__ void zero_kernel(double* p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
-__global__ void reduce_partials_kernel(const double* partial, int64_t n, double* out) {
-    __shared__ double red[512];
+// Sum of n (re, im) partials in a fixed order: 1024 threads, then a tree.
+__global__ void __launch_bounds__(1024) reduce_partials_kernel(const double* partial, int64_t n, double* out) {
+    __shared__ double red[2048];
     double re = 0.0, im = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += 256) {
+    for (int64_t k = threadIdx.x; k < n; k += 1024) {
         re += partial[2 * k];
         im += partial[2 * k + 1];
     }
     red[2 * threadIdx.x] = re;
     red[2 * threadIdx.x + 1] = im;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
+    for (int st = 512; st > 0; st >>= 1) {
         if (int(threadIdx.x) < st) {
             red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
             red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
@@ -719,7 +720,7 @@ int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y
 int64_t hv_partials(const ed_sector* s) { return (s->D + 255) / 256; }
 
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, partial, n, out);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, st, partial, n, out);
     ED_HIP(hipGetLastError());
 }
 

@@ -42,7 +42,7 @@ def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=3
     import torch
 
     dim = hm.dim
-    v = torch.as_tensor(krylov.start_vector(dim, seed), device="cuda")
+    v = krylov.device_start_vector(dim, seed)
     res = np.inf
     for restart in range(max_restarts):
         a, b = hm.lanczos(v, min(nsteps, dim), precision=precision)

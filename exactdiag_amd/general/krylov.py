@@ -79,6 +79,16 @@ def start_vector(D, seed=0):
     return v / np.linalg.norm(v)
 
 
+def device_start_vector(D, seed=0):
+    """Seeded complex normal start vector drawn on the device (no host round trip for large sectors)."""
+    torch = _torch()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    v = torch.randn(D, dtype=torch.complex128, device="cuda", generator=g)
+    vec_scale(v, 1.0 / vec_norm(v))
+    return v
+
+
 def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=500):
     """Lowest ``nev`` eigenpairs of the Hermitian operator ``apply`` (device tensor in, out).
 
@@ -89,7 +99,10 @@ def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=
     m = ncv or max(2 * nev + 24, 40)
     m = min(m, D)
     V = torch.empty((m + 1, D), dtype=torch.complex128, device="cuda")
-    v = torch.as_tensor(start_vector(D) if v0 is None else np.asarray(v0, np.complex128), device="cuda")
+    if isinstance(v0, torch.Tensor):
+        v = v0.to("cuda", torch.complex128)
+    else:
+        v = torch.as_tensor(start_vector(D) if v0 is None else np.asarray(v0, np.complex128), device="cuda")
     V[0].copy_(v)
     vec_scale(V[0], 1.0 / vec_norm(V[0]))
     T = np.zeros((m, m))
