@@ -115,7 +115,18 @@ __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, 
 // one XCD take neighbouring ranges, so their gathers share that XCD's L2);
 // U elements per lane are loaded, then their U gathers issued together.
 // partial[workgroup] = sum conj(x_f) y_f over its rows.
-template <class VT, int U>
+// Streamed, read-once data (the stored elements, counts, diagonal codes) and
+// the output can be moved with non-temporal hints (NT), so that L2 keeps the
+// gathered x_j instead of the matrix stream.
+template <bool NT, class T>
+__device__ inline T ld_stream(const T* p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <class VT, int U, bool NT>
 __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col,
                                                      const uint16_t* __restrict__ code,
@@ -138,7 +149,7 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
     C2 dot{0.0, 0.0};
     for (int64_t f = f0 + threadIdx.x; f < f1; f += BS) {
         const int64_t base = sptr[f >> 6] + (f & 63);
-        const int n = cnt[f];
+        const int n = ld_stream<NT>(cnt + f);
         C2 acc{0.0, 0.0};
         int k = 0;
         for (; k + U <= n; k += U) {
@@ -146,8 +157,8 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
             uint16_t c[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                j[u] = col[base + int64_t(k + u) * 64];
-                c[u] = code[base + int64_t(k + u) * 64];
+                j[u] = ld_stream<NT>(col + base + int64_t(k + u) * 64);
+                c[u] = ld_stream<NT>(code + base + int64_t(k + u) * 64);
             }
             C2 xv[U];
 #pragma unroll
@@ -161,8 +172,8 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
             }
         }
         for (; k < n; ++k) {
-            const int32_t j = col[base + int64_t(k) * 64];
-            const uint16_t c = code[base + int64_t(k) * 64];
+            const int32_t j = ld_stream<NT>(col + base + int64_t(k) * 64);
+            const uint16_t c = ld_stream<NT>(code + base + int64_t(k) * 64);
             const C2 xv = ldx(x, j);
             const int vi = c & 0x7fff;
             const double sg = (c & 0x8000) ? -1.0 : 1.0;
@@ -170,12 +181,21 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
         }
         const double sqf = sq[orbit[f]];
         const C2 xf = ldx(x, f);
-        const int dv = diag[f];
+        const int dv = ld_stream<NT>(diag + f);
         const C2 yf = cadd(C2{sqf * acc.re, sqf * acc.im}, cmul(C2{V[2 * dv], V[2 * dv + 1]}, xf));
         VT o;
         o.x = decltype(o.x)(yf.re);
         o.y = decltype(o.y)(yf.im);
-        y[f] = o;
+        if constexpr (NT) {
+            using S = decltype(o.x);
+            typedef S S2 __attribute__((ext_vector_type(2)));
+            S2 v;
+            v.x = o.x;
+            v.y = o.y;
+            __builtin_nontemporal_store(v, reinterpret_cast<S2*>(y + f));
+        } else {
+            y[f] = o;
+        }
         dot = cadd(dot, cmul(C2{xf.re, -xf.im}, yf));
     }
     if (partial) {
@@ -215,6 +235,7 @@ struct ed_hmat {
     int n_vals = 0;
     int tiles = 0;   // row tiles per workgroup of the product (0 = default)
     int unroll = 4;  // gathers in flight per lane (4 or 8)
+    int nontemporal = 0;  // stream the stored elements and the output with non-temporal hints
     int64_t* d_sptr = nullptr;
     int32_t* d_col = nullptr;
     uint16_t* d_code = nullptr;
@@ -244,9 +265,14 @@ template <class VT, int U>
 void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int64_t nwg, int64_t tiles,
                  hipStream_t st) {
     const size_t lds = size_t(m->n_vals) * 16;
-    hipLaunchKernelGGL((hm_spmv_kernel<VT, U>), dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col,
-                       m->d_code, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals, m->n_vals,
-                       static_cast<const VT*>(x), static_cast<VT*>(y), partial, tiles);
+    if (m->nontemporal)
+        hipLaunchKernelGGL((hm_spmv_kernel<VT, U, true>), dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr,
+                           m->d_col, m->d_code, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals,
+                           m->n_vals, static_cast<const VT*>(x), static_cast<VT*>(y), partial, tiles);
+    else
+        hipLaunchKernelGGL((hm_spmv_kernel<VT, U, false>), dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr,
+                           m->d_col, m->d_code, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals,
+                           m->n_vals, static_cast<const VT*>(x), static_cast<VT*>(y), partial, tiles);
 }
 
 // Launches the product; returns the number of partials written (one per workgroup).
@@ -395,22 +421,35 @@ int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots,
     });
 }
 
+int ed_hmat_set_config(ed_hmat* m, int tiles, int unroll, int nontemporal) {
+    return guarded([&] {
+        ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
+        ED_REQUIRE(tiles >= 0 && tiles <= 1024, ED_ERR_ARG, "tiles must be in [0, 1024] (0 = default)");
+        ED_REQUIRE(unroll == 4 || unroll == 8, ED_ERR_ARG, "unroll must be 4 or 8");
+        m->tiles = tiles;
+        m->unroll = unroll;
+        m->nontemporal = nontemporal ? 1 : 0;
+    });
+}
+
 int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll,
-                 double* best_ms, void* stream) {
+                 int* nontemporal, double* best_ms, void* stream) {
     return guarded([&] {
         ED_REQUIRE(m && d_x && d_y, ED_ERR_ARG, "null argument");
         ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
         hipStream_t st = static_cast<hipStream_t>(stream);
         double best = 1e300;
-        int bt = 0, bu = 4;
+        int bt = 0, bu = 4, bn = 0;
         if (m->D > 0) {
             hipEvent_t e0, e1;
             ED_HIP(hipEventCreate(&e0));
             ED_HIP(hipEventCreate(&e1));
             for (int t : {1, 2, 4, 8, 16, 32}) {
-                for (int u : {4, 8}) {
+                for (int u : {4, 8})
+                for (int nt : {0, 1}) {
                     m->tiles = t;
                     m->unroll = u;
+                    m->nontemporal = nt;
                     // the product with its fused dot, as ed_hmat_hv runs it
                     auto run = [&] {
                         const int64_t np = launch_hmat(m, d_x, d_y, m->d_partial, precision, st);
@@ -427,6 +466,7 @@ int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int*
                         best = ms / 3;
                         bt = t;
                         bu = u;
+                        bn = nt;
                     }
                 }
             }
@@ -435,8 +475,10 @@ int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int*
         }
         m->tiles = bt;
         m->unroll = bu;
+        m->nontemporal = bn;
         if (tiles) *tiles = bt;
         if (unroll) *unroll = bu;
+        if (nontemporal) *nontemporal = bn;
         if (best_ms) *best_ms = best < 1e300 ? best : 0.0;
     });
 }

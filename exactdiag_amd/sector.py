@@ -256,15 +256,18 @@ class StoredH:
         return y
 
     def tune(self, precision: int = 0):
-        """Autotune the stored product (tiles per workgroup, gathers in flight); returns (tiles, unroll, ms)."""
+        """Autotune the stored product; returns (tiles per workgroup, gathers in flight, non-temporal, ms)."""
         torch = _torch()
         dt = torch.complex128 if precision == 0 else torch.complex64
         x = torch.ones(max(self.dim, 1), dtype=dt, device=self.device)
         y = torch.empty_like(x)
-        t, u, ms = ct.c_int(0), ct.c_int(0), ct.c_double(0.0)
-        check(lib().ed_hmat_tune(self._h, _ptr(x), _ptr(y), int(precision), ct.byref(t), ct.byref(u), ct.byref(ms),
-                                 _stream()))
-        return t.value, u.value, ms.value
+        t, u, nt, ms = ct.c_int(0), ct.c_int(0), ct.c_int(0), ct.c_double(0.0)
+        check(lib().ed_hmat_tune(self._h, _ptr(x), _ptr(y), int(precision), ct.byref(t), ct.byref(u), ct.byref(nt),
+                                 ct.byref(ms), _stream()))
+        return t.value, u.value, nt.value, ms.value
+
+    def set_config(self, tiles=0, unroll=4, nontemporal=False):
+        check(lib().ed_hmat_set_config(self._h, int(tiles), int(unroll), int(bool(nontemporal))))
 
     def lanczos(self, v0, nsteps: int, precision: int = 0):
         torch = _torch()

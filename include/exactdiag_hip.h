@@ -137,10 +137,13 @@ int ed_hmat_destroy(ed_hmat* m);
 int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots, int* n_values);
 int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream);
 /* Autotuner for the stored product (no counterpart in the reference): row
- * tiles of 256 per workgroup (1..32) x gathers in flight per lane (4, 8);
- * keeps the fastest in the matrix. */
-int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll, double* best_ms,
-                 void* stream);
+ * tiles of 256 per workgroup (1..32) x gathers in flight per lane (4, 8) x
+ * plain or non-temporal streaming of the stored elements and the output;
+ * times the product with its fused dot and keeps the fastest in the matrix.
+ * ed_hmat_set_config sets the same three choices (tiles 0 = default). */
+int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll,
+                 int* nontemporal, double* best_ms, void* stream);
+int ed_hmat_set_config(ed_hmat* m, int tiles, int unroll, int nontemporal);
 int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
                         int precision, void* stream);
 int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const double* ritz, double* d_x,

@@ -142,8 +142,14 @@ def test_stored_tuner_keeps_results(torch):
     m = s.store(WEIGHTS[3])
     x = _rand(torch, s.dim, 8)
     y0, d0 = m.hv(x, with_dot=True)
-    t, u, ms = m.tune()
-    assert t in (1, 2, 4, 8, 16, 32) and u in (4, 8) and ms > 0
+    t, u, nt, ms = m.tune()
+    assert t in (1, 2, 4, 8, 16, 32) and u in (4, 8) and nt in (0, 1) and ms > 0
     y1, d1 = m.hv(x, with_dot=True)
     assert torch.equal(y0, y1)  # rows are computed the same way whatever the tiling
     assert abs(d1 - d0) <= 1e-12 * max(1.0, abs(d0))
+    for cfg in ((1, 8, True), (16, 4, True), (3, 4, False), (0, 8, False)):
+        m.set_config(*cfg)
+        y2 = m.hv(x)
+        assert torch.equal(y0, y2), cfg
+    with pytest.raises(ValueError):
+        m.set_config(1, 5, False)
