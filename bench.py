@@ -315,6 +315,7 @@ def main():
                    "dim": D, "basis_build_s": round(t_basis, 4), "stored_build_s": round(t_store, 4),
                    "stored_bytes": hm.bytes, "stored_elements_per_row": round(hm.nnz / D, 3),
                    "stored_config": {"tiles_per_workgroup": 1, "gathers_in_flight": 4,
+                                     "nontemporal_stream": prec == 0,
                                      "note": "the autotuner's choice for this sector (ed_hmat_tune), built in as "
                                              "the default"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

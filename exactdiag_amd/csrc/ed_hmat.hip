@@ -11,8 +11,10 @@
 //  * sliced ELL with slices of 64 consecutive rows: element k of the rows of
 //    slice s sits at sptr[s] + 64 k + lane, so the 64 lanes of a wavefront
 //    read consecutive words (coalesced) at every k;
-//  * per element an int32 column and a uint16 code = index into a table of
-//    exact unique values (bit 15 = negate), 6 bytes per element; per row the
+//  * per element a column and a code = index into a table of exact unique
+//    values plus a negate bit: packed into one 32-bit word when the column
+//    and the code fit (4 bytes per element, e.g. 23 + 9 bits for the 2x12
+//    sector), else an int32 column and a uint16 code (6 bytes); per row the
 //    element count (uint8) and the diagonal's value index (uint16);
 //  * the common factor sqrt(n_f) of row f's off-diagonal elements is applied
 //    once per row (from the orbit table), so the value table holds
@@ -22,6 +24,7 @@
 // order, in two passes (count, fill).
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <utility>
@@ -54,8 +57,10 @@ struct CountSink {
 };
 
 struct FillSink {
-    int32_t* col;
-    uint16_t* code;
+    int32_t* col;         // 6-byte form: column ...
+    uint16_t* code;       // ... and value index | negate << 15
+    uint32_t* pack;       // 4-byte form: column << shift | value index << 1 | negate
+    int shift;            // 0: 6-byte form
     const uint16_t* map;  // combo -> value index
     int64_t base;         // sptr[slice] + lane
     int k;
@@ -68,8 +73,14 @@ struct FillSink {
         for (int q = 0; q < G; ++q) {
             if (e[q].code == NO_ELEM) continue;
             const int64_t at = base + int64_t(k) * 64;
-            col[at] = int32_t(e[q].j);
-            code[at] = uint16_t(map[e[q].code & 0x7fffu] | (e[q].code & 0x8000u));
+            const uint32_t vi = map[e[q].code & 0x7fffu];
+            const uint32_t neg = (e[q].code >> 15) & 1u;
+            if (shift) {
+                pack[at] = (uint32_t(e[q].j) << shift) | (vi << 1) | neg;
+            } else {
+                col[at] = int32_t(e[q].j);
+                code[at] = uint16_t(vi | (neg << 15));
+            }
             ++k;
         }
     }
@@ -100,13 +111,14 @@ __global__ void __launch_bounds__(BS) hm_width_kernel(const uint8_t* cnt, int64_
 
 __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, LdsLayout lay, int lds,
                                                      const uint16_t* __restrict__ map,
-                                                     const int64_t* __restrict__ sptr, int32_t* col, uint16_t* code) {
+                                                     const int64_t* __restrict__ sptr, int32_t* col, uint16_t* code,
+                                                     uint32_t* pack, int shift) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Tabs T = lds ? stage_tabs<BS>(s, lay, smem) : global_tabs(s);
     __syncthreads();
     const int64_t f = int64_t(blockIdx.x) * BS + threadIdx.x;
     if (f >= s.D) return;
-    FillSink sink{col, code, map, sptr[f >> 6] + (f & 63), 0};
+    FillSink sink{col, code, pack, shift, map, sptr[f >> 6] + (f & 63), 0};
     walk_row<4>(s, T, w, f, sink);
 }
 
@@ -126,10 +138,29 @@ __device__ inline T ld_stream(const T* p) {
         return *p;
 }
 
-template <class VT, int U, bool NT>
+// One stored element: column j, value index vi, sign sg.
+template <bool PACKED, bool NT>
+__device__ inline void ld_elem(const int32_t* __restrict__ col, const uint16_t* __restrict__ code,
+                               const uint32_t* __restrict__ pack, int shift, int64_t at, int32_t& j, int& vi,
+                               double& sg) {
+    if constexpr (PACKED) {
+        const uint32_t e = ld_stream<NT>(pack + at);
+        j = int32_t(e >> shift);
+        vi = int((e & ((1u << shift) - 1u)) >> 1);
+        sg = (e & 1u) ? -1.0 : 1.0;
+    } else {
+        j = ld_stream<NT>(col + at);
+        const uint16_t c = ld_stream<NT>(code + at);
+        vi = c & 0x7fff;
+        sg = (c & 0x8000) ? -1.0 : 1.0;
+    }
+}
+
+template <class VT, int U, bool NT, bool PACKED>
 __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col,
                                                      const uint16_t* __restrict__ code,
+                                                     const uint32_t* __restrict__ pack, int shift,
                                                      const uint8_t* __restrict__ cnt,
                                                      const uint16_t* __restrict__ diag,
                                                      const uint8_t* __restrict__ orbit,
@@ -154,29 +185,26 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
         int k = 0;
         for (; k + U <= n; k += U) {
             int32_t j[U];
-            uint16_t c[U];
+            int vi[U];
+            double sg[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                j[u] = ld_stream<NT>(col + base + int64_t(k + u) * 64);
-                c[u] = ld_stream<NT>(code + base + int64_t(k + u) * 64);
-            }
+            for (int u = 0; u < U; ++u)
+                ld_elem<PACKED, NT>(col, code, pack, shift, base + int64_t(k + u) * 64, j[u], vi[u], sg[u]);
             C2 xv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) xv[u] = ldx(x, j[u]);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int vi = c[u] & 0x7fff;
-                const double sg = (c[u] & 0x8000) ? -1.0 : 1.0;
-                const C2 v{sg * V[2 * vi], sg * V[2 * vi + 1]};
+                const C2 v{sg[u] * V[2 * vi[u]], sg[u] * V[2 * vi[u] + 1]};
                 acc = cadd(acc, cmul(v, xv[u]));
             }
         }
         for (; k < n; ++k) {
-            const int32_t j = ld_stream<NT>(col + base + int64_t(k) * 64);
-            const uint16_t c = ld_stream<NT>(code + base + int64_t(k) * 64);
+            int32_t j;
+            int vi;
+            double sg;
+            ld_elem<PACKED, NT>(col, code, pack, shift, base + int64_t(k) * 64, j, vi, sg);
             const C2 xv = ldx(x, j);
-            const int vi = c & 0x7fff;
-            const double sg = (c & 0x8000) ? -1.0 : 1.0;
             acc = cadd(acc, cmul(C2{sg * V[2 * vi], sg * V[2 * vi + 1]}, xv));
         }
         const double sqf = sq[orbit[f]];
@@ -235,20 +263,23 @@ struct ed_hmat {
     int n_vals = 0;
     int tiles = 0;   // row tiles per workgroup of the product (0 = default)
     int unroll = 4;  // gathers in flight per lane (4 or 8)
-    int nontemporal = 0;  // stream the stored elements and the output with non-temporal hints
+    int nontemporal = -1;  // stream elements and output non-temporally: 1 yes, 0 no, -1 for complex128 only
     int64_t* d_sptr = nullptr;
-    int32_t* d_col = nullptr;
-    uint16_t* d_code = nullptr;
+    int32_t* d_col = nullptr;    // 6-byte form
+    uint16_t* d_code = nullptr;  // 6-byte form
+    uint32_t* d_pack = nullptr;  // 4-byte form
+    int shift = 0;               // bits of value index + sign in the 4-byte form; 0 = 6-byte form
     uint8_t* d_cnt = nullptr;
     uint16_t* d_diag = nullptr;
     double* d_vals = nullptr;
     double* d_partial = nullptr;
 
-    int64_t bytes() const { return slots * 6 + D * 3 + (nslices + 1) * 8 + int64_t(n_vals) * 16; }
+    int element_bytes() const { return shift ? 4 : 6; }
+    int64_t bytes() const { return slots * element_bytes() + D * 3 + (nslices + 1) * 8 + int64_t(n_vals) * 16; }
 
     ~ed_hmat() {
-        for (void* p : {(void*)d_sptr, (void*)d_col, (void*)d_code, (void*)d_cnt, (void*)d_diag, (void*)d_vals,
-                        (void*)d_partial})
+        for (void* p : {(void*)d_sptr, (void*)d_col, (void*)d_code, (void*)d_pack, (void*)d_cnt, (void*)d_diag,
+                        (void*)d_vals, (void*)d_partial})
             dev_free(p);
     }
 };
@@ -265,14 +296,25 @@ template <class VT, int U>
 void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int64_t nwg, int64_t tiles,
                  hipStream_t st) {
     const size_t lds = size_t(m->n_vals) * 16;
-    if (m->nontemporal)
-        hipLaunchKernelGGL((hm_spmv_kernel<VT, U, true>), dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr,
-                           m->d_col, m->d_code, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals,
+    // measured on the 2x12 sector: non-temporal streaming gains 4 % with complex128 vectors, nothing with
+    // complex64 (ed_hmat_tune)
+    const bool nt = m->nontemporal < 0 ? sizeof(VT) == 16 : m->nontemporal != 0;
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col, m->d_code,
+                           m->d_pack, m->shift, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals,
                            m->n_vals, static_cast<const VT*>(x), static_cast<VT*>(y), partial, tiles);
-    else
-        hipLaunchKernelGGL((hm_spmv_kernel<VT, U, false>), dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr,
-                           m->d_col, m->d_code, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals,
-                           m->n_vals, static_cast<const VT*>(x), static_cast<VT*>(y), partial, tiles);
+    };
+    if (m->shift) {
+        if (nt)
+            go(hm_spmv_kernel<VT, U, true, true>);
+        else
+            go(hm_spmv_kernel<VT, U, false, true>);
+    } else {
+        if (nt)
+            go(hm_spmv_kernel<VT, U, true, false>);
+        else
+            go(hm_spmv_kernel<VT, U, false, false>);
+    }
 }
 
 // Launches the product; returns the number of partials written (one per workgroup).
@@ -343,6 +385,7 @@ int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* str
                             uint16_t(intern(cmul(m->w.w[t], C2{st_ * ar, -(st_ * ai)})));
                     }
             }
+            const int n_off = int(vals.size() / 2);  // off-diagonal values come first
             std::vector<uint16_t> dmap(size_t((N + 1) * (L + 1)), 0);
             for (int a = 0; a <= N; ++a)
                 for (int b = 0; b <= L; ++b) {
@@ -353,6 +396,16 @@ int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* str
                 }
             m->n_vals = int(vals.size() / 2);
             ED_REQUIRE(m->n_vals < 0x8000, ED_ERR_CAPACITY, "more than 32767 unique matrix values");
+            // 4-byte elements when column, value index and sign fit 32 bits (ED_HMAT_ELEMENT_BYTES=6 forces the
+            // 6-byte form)
+            {
+                int col_bits = 1;
+                while (col_bits < 31 && (int64_t(1) << col_bits) < s->D) ++col_bits;
+                const int code_bits = 32 - col_bits;
+                const char* force = std::getenv("ED_HMAT_ELEMENT_BYTES");
+                const bool six = force && std::atoi(force) == 6;
+                if (!six && code_bits >= 2 && int64_t(n_off) <= (int64_t(1) << (code_bits - 1))) m->shift = code_bits;
+            }
             m->d_vals = dev_upload(vals, st);
             const int64_t D = s->D;
             m->nslices = (D + 63) / 64;
@@ -386,10 +439,14 @@ int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* str
                 for (uint8_t c : cnt) m->nnz += c;
                 m->slots = sptr.back();
                 m->d_sptr = dev_upload(sptr, st);
-                m->d_col = dev_alloc<int32_t>(size_t(std::max<int64_t>(m->slots, 1)));
-                m->d_code = dev_alloc<uint16_t>(size_t(std::max<int64_t>(m->slots, 1)));
+                if (m->shift) {
+                    m->d_pack = dev_alloc<uint32_t>(size_t(std::max<int64_t>(m->slots, 1)));
+                } else {
+                    m->d_col = dev_alloc<int32_t>(size_t(std::max<int64_t>(m->slots, 1)));
+                    m->d_code = dev_alloc<uint16_t>(size_t(std::max<int64_t>(m->slots, 1)));
+                }
                 hipLaunchKernelGGL(hm_fill_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
-                                   d_map, m->d_sptr, m->d_col, m->d_code);
+                                   d_map, m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift);
                 ED_HIP(hipGetLastError());
                 ED_HIP(hipStreamSynchronize(st));
             } else {
@@ -421,14 +478,17 @@ int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots,
     });
 }
 
+int ed_hmat_element_bytes(const ed_hmat* m) { return m ? m->element_bytes() : ED_ERR_ARG; }
+
 int ed_hmat_set_config(ed_hmat* m, int tiles, int unroll, int nontemporal) {
     return guarded([&] {
         ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
         ED_REQUIRE(tiles >= 0 && tiles <= 1024, ED_ERR_ARG, "tiles must be in [0, 1024] (0 = default)");
         ED_REQUIRE(unroll == 4 || unroll == 8, ED_ERR_ARG, "unroll must be 4 or 8");
+        ED_REQUIRE(nontemporal >= -1 && nontemporal <= 1, ED_ERR_ARG, "nontemporal must be -1, 0 or 1");
         m->tiles = tiles;
         m->unroll = unroll;
-        m->nontemporal = nontemporal ? 1 : 0;
+        m->nontemporal = nontemporal;
     });
 }
 

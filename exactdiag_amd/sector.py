@@ -229,6 +229,7 @@ class StoredH:
         b, nnz, slots, nv = ct.c_int64(0), ct.c_int64(0), ct.c_int64(0), ct.c_int(0)
         check(lib().ed_hmat_info(self._h, ct.byref(b), ct.byref(nnz), ct.byref(slots), ct.byref(nv)))
         self.bytes, self.nnz, self.slots, self.n_values = b.value, nnz.value, slots.value, nv.value
+        self.element_bytes = int(lib().ed_hmat_element_bytes(self._h))
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -266,8 +267,9 @@ class StoredH:
                                  ct.byref(ms), _stream()))
         return t.value, u.value, nt.value, ms.value
 
-    def set_config(self, tiles=0, unroll=4, nontemporal=False):
-        check(lib().ed_hmat_set_config(self._h, int(tiles), int(unroll), int(bool(nontemporal))))
+    def set_config(self, tiles=0, unroll=4, nontemporal=-1):
+        """tiles 0 = default; nontemporal -1 = default (complex128 only), 0 or 1 (bools accepted)."""
+        check(lib().ed_hmat_set_config(self._h, int(tiles), int(unroll), int(nontemporal)))
 
     def lanczos(self, v0, nsteps: int, precision: int = 0):
         torch = _torch()

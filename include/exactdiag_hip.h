@@ -135,12 +135,17 @@ typedef struct ed_hmat ed_hmat;
 int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* stream);
 int ed_hmat_destroy(ed_hmat* m);
 int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots, int* n_values);
+/* Bytes per stored element: 4 when column, value index and sign fit one
+ * 32-bit word, else 6 (the environment variable ED_HMAT_ELEMENT_BYTES=6,
+ * read at ed_hmat_create, forces 6). */
+int ed_hmat_element_bytes(const ed_hmat* m);
 int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream);
 /* Autotuner for the stored product (no counterpart in the reference): row
  * tiles of 256 per workgroup (1..32) x gathers in flight per lane (4, 8) x
  * plain or non-temporal streaming of the stored elements and the output;
  * times the product with its fused dot and keeps the fastest in the matrix.
- * ed_hmat_set_config sets the same three choices (tiles 0 = default). */
+ * ed_hmat_set_config sets the same three choices (tiles 0 = default;
+ * nontemporal -1 = the default, non-temporal for complex128 vectors only). */
 int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll,
                  int* nontemporal, double* best_ms, void* stream);
 int ed_hmat_set_config(ed_hmat* m, int tiles, int unroll, int nontemporal);

@@ -39,7 +39,17 @@ def _rand(torch, n, seed, dtype=None):
     return torch.as_tensor(v, device="cuda", dtype=dtype or torch.complex128)
 
 
-def test_stored_matches_matrix_free(torch):
+@pytest.fixture(params=[4, 6], ids=["4-byte", "6-byte"])
+def element_bytes(request, monkeypatch):
+    """Both stored element forms: packed 32-bit words and int32 column + uint16 code."""
+    if request.param == 6:
+        monkeypatch.setenv("ED_HMAT_ELEMENT_BYTES", "6")
+    else:
+        monkeypatch.delenv("ED_HMAT_ELEMENT_BYTES", raising=False)
+    return request.param
+
+
+def test_stored_matches_matrix_free(torch, element_bytes):
     from exactdiag_amd.sector import Sector
 
     for p in SECTORS:
@@ -48,6 +58,7 @@ def test_stored_matches_matrix_free(torch):
             print("sector", p, "weights", wi, "dim", s.dim, flush=True)
             m = s.store(w)
             assert m.nnz <= m.slots and m.bytes > 0
+            assert m.element_bytes == element_bytes
             if s.dim == 0:
                 continue
             x = _rand(torch, s.dim, wi)
@@ -59,7 +70,7 @@ def test_stored_matches_matrix_free(torch):
             m.close()
 
 
-def test_stored_matches_oracle(torch, oracle_mod):
+def test_stored_matches_oracle(torch, oracle_mod, element_bytes):
     from exactdiag_amd.sector import Sector
 
     for p in SECTORS[:12]:
@@ -76,7 +87,7 @@ def test_stored_matches_oracle(torch, oracle_mod):
             np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
-def test_stored_c64(torch):
+def test_stored_c64(torch, element_bytes):
     from exactdiag_amd.sector import Sector
 
     s = Sector(8, 2, 7, 1, 0)
