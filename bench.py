@@ -295,8 +295,8 @@ def main():
     s_bytes = 16 if prec == 0 else 8
     alg = (2 * s_bytes + 12) * D
     achieved = alg / (ms_per_step * 1e-3) / 1e9
-    # compulsory bytes of the stored form: x, y, 6 bytes per stored element, count + diagonal index + orbit size
-    alg_stored = (2 * s_bytes + 4) * D + 6 * hm.nnz
+    # compulsory bytes of the stored form: x, y, the stored elements (4 or 6 bytes), count + diagonal index + orbit
+    alg_stored = (2 * s_bytes + 4) * D + hm.element_bytes * hm.nnz
     line = {
         "metric": "t-J H·v Gstates/s (% HBM peak); time to all-q S(q,ω) at 1e-3 vs plain",
         "value": value,
@@ -314,6 +314,7 @@ def main():
                                "(stored sliced-ELL H, built once before the timed region)",
                    "dim": D, "basis_build_s": round(t_basis, 4), "stored_build_s": round(t_store, 4),
                    "stored_bytes": hm.bytes, "stored_elements_per_row": round(hm.nnz / D, 3),
+                   "stored_element_bytes": hm.element_bytes,
                    "stored_config": {"tiles_per_workgroup": 1, "gathers_in_flight": 4,
                                      "nontemporal_stream": prec == 0,
                                      "note": "the autotuner's choice for this sector (ed_hmat_tune), built in as "
