@@ -147,10 +147,25 @@ class Basis_Map:
         return self.get_num_sparse_in_dense(d) if d > -1 else 0
 
 
-def py_get_ladder_translators(config: "configs.Hamiltonian_Config"):
-    """(translator, basis map, symmetries), matrix_setup.pyx:377-406."""
+def py_get_ladder_translators(config: "configs.Hamiltonian_Config", basis_file=True):
+    """(translator, basis map, symmetries), matrix_setup.pyx:377-406.
+
+    As the reference's get_basis_map (general/basis_indexing.h:245-253) does with
+    <calc folder>/Sz_local-momentum_map.dat: an existing file is read and must equal the
+    GPU-built basis (ValueError otherwise); a missing one is written.
+    """
+    from exactdiag_amd.general import basis_indexing
+
     s = get_sector(config)
-    return Translator(s, config), Basis_Map(s), Symmetries(config.num_rungs, config.symmetry_qs.to_npint32())
+    bmap = Basis_Map(s)
+    if basis_file:
+        path = config.get_calc_folder() / basis_indexing.BASIS_MAP_NAME
+        sparse, orbit = bmap._arrays()
+        if path.exists():
+            basis_indexing.check_basis_map(path, sparse, orbit)
+        else:
+            basis_indexing.save_basis_map(path, sparse, orbit)
+    return Translator(s, config), bmap, Symmetries(config.num_rungs, config.symmetry_qs.to_npint32())
 
 
 def _weights_dict(config):

@@ -161,3 +161,25 @@ def test_all_q_szq_spectra_equal_lehmann_sums(torch, oracle_mod):
             exact /= len(gs)
             got = res.spectrum[qx, qy]
             assert np.max(np.abs(got - exact)) < 1e-6 * max(1.0, np.max(exact)), (qx, qy)
+
+
+def test_translators_write_and_check_the_basis_file(torch, oracle_mod):
+    """get_translators() keeps <calc folder>/Sz_local-momentum_map.dat as get_basis_map does
+    (general/basis_indexing.h:245-253): written from the GPU basis, then read back and checked."""
+    from exactdiag_amd.general import basis_indexing as bi
+    from exactdiag_amd.tJ_spin_half_ladder import configs
+
+    ham = configs.Hamiltonian_Config(num_rungs=6, num_holes=2, weights={"tl": 1, "tr": 1, "jl": 0.4, "jr": 0.4},
+                                     symmetry_qs={"leg": 1, "rung": 1})
+    translator, bmap, sym = ham.get_translators()
+    path = ham.get_calc_folder() / bi.BASIS_MAP_NAME
+    sparse, orbit = bi.load_basis_map(path)
+    so, co = oracle_mod.OracleSector(6, 2, 5, 1, 1).basis()
+    assert np.array_equal(sparse, so) and np.array_equal(orbit, co)
+    assert bmap.get_num_states() == len(so)
+    ham.get_translators()  # an existing, matching file is accepted
+    bad = sparse.copy()
+    bad[3], bad[4] = bad[4], bad[3]
+    bi.save_basis_map(path, bad, orbit)
+    with pytest.raises(ValueError):
+        ham.get_translators()
