@@ -187,7 +187,8 @@ def run_form(form):
                 "value": res["gs_batched_s"] + res["spectra_batched_s"], "unit": "s", "n_gpus": 1,
                 "higher_is_better": False, "dtype": "complex128", "data": "synthetic (model built from its definition)",
                 "config": {"workload": "2x10 ladder, 2 holes, Sz=0, t=1, J in {0.3,0.4,0.5}, all 20 k sectors "
-                                       "(60 instances); A+(k,w), A-(k,w) for all 20 k, 250 steps, FP64"},
+                                       "(60 instances); A+(k,w), A-(k,w) for all 20 k, 250 steps, FP64; stored H on "
+                                       "both sides"},
                 "form": res}
     else:
         res = forms.precision_form(log=log)

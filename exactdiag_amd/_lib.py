@@ -26,7 +26,7 @@ EXPORTED = [
     "ed_batch_create", "ed_batch_destroy", "ed_batch_offsets", "ed_batch_hv", "ed_batch_lanczos", "ed_batch_tune",
     "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune", "ed_lanczos_ritz",
     "ed_hmat_create", "ed_hmat_destroy", "ed_hmat_info", "ed_hmat_hv", "ed_hmat_lanczos_run", "ed_hmat_lanczos_ritz",
-    "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes",
+    "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_batch_create_stored",
 ]
 
 _vp = ct.c_void_p
@@ -84,6 +84,7 @@ _SIGNATURES = {
     "ed_hmat_tune": (ct.c_int, [_vp, _vp, _vp, ct.c_int, _ip, _ip, _ip, _dp, _vp]),
     "ed_hmat_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int]),
     "ed_hmat_element_bytes": (ct.c_int, [_vp]),
+    "ed_batch_create_stored": (ct.c_int, [ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp)]),
 }
 
 _lib = None

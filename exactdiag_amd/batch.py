@@ -29,7 +29,9 @@ from exactdiag_amd.sector import Sector, _dbl, _ptr, _stream, _torch, cf_eval, w
 class Batch:
     """A set of (sector, weights) instances sharing one vector pool."""
 
-    def __init__(self, sectors, weights):
+    def __init__(self, sectors, weights, stored=False):
+        """``stored``: every instance's H·v reads a stored H (built here, one per distinct sector and
+        weights) instead of recomputing its elements."""
         torch = _torch()
         self.sectors = list(sectors)
         n = len(self.sectors)
@@ -39,9 +41,20 @@ class Batch:
                         and all(isinstance(w, (dict, list, tuple, np.ndarray)) for w in weights))
         wl = [weights_array(w) for w in weights] if per_instance else [weights_array(weights)] * n
         self._w = np.ascontiguousarray(np.concatenate(wl), np.float64)
-        handles = (ct.c_void_p * n)(*[s.handle.value for s in self.sectors])
         h = ct.c_void_p()
-        check(lib().ed_batch_create(handles, self._w.ctypes.data_as(ct.POINTER(ct.c_double)), n, ct.byref(h)))
+        self._stored = []
+        if stored:
+            built = {}
+            for s, w in zip(self.sectors, wl):
+                key = (id(s), w.tobytes())
+                if key not in built:
+                    built[key] = s.store(w)
+                self._stored.append(built[key])
+            mats = (ct.c_void_p * n)(*[m._h.value for m in self._stored])
+            check(lib().ed_batch_create_stored(mats, n, ct.byref(h)))
+        else:
+            handles = (ct.c_void_p * n)(*[s.handle.value for s in self.sectors])
+            check(lib().ed_batch_create(handles, self._w.ctypes.data_as(ct.POINTER(ct.c_double)), n, ct.byref(h)))
         self._h = h
         off = np.zeros(n + 1, np.int64)
         check(lib().ed_batch_offsets(self._h, off.ctypes.data_as(ct.POINTER(ct.c_int64))))
@@ -58,6 +71,9 @@ class Batch:
         if h is not None and h.value:
             lib().ed_batch_destroy(h)
             self._h = None
+        for m in getattr(self, "_stored", []):
+            m.close()
+        self._stored = []
 
     def __del__(self):
         try:
@@ -222,7 +238,7 @@ class Excitation:
     norm2: float = 0.0
 
 
-def spectra(excitations, gs_sector, psi, e0, weights, ws, eta, nsteps, graph=True):
+def spectra(excitations, gs_sector, psi, e0, weights, ws, eta, nsteps, graph=True, stored=False):
     """All excitations' spectra from one ground state in one batched Lanczos run.
 
     Follows lanczos_dynamical_response (general/cython_lanczos_diagonalization.pyx:18-40)
@@ -241,7 +257,7 @@ def spectra(excitations, gs_sector, psi, e0, weights, ws, eta, nsteps, graph=Tru
     coeffs = [(np.zeros(0), np.zeros(0)) for _ in excitations]
     if not live:
         return out, coeffs
-    batch = Batch([excitations[i].sector for i in live], weights)
+    batch = Batch([excitations[i].sector for i in live], weights, stored=stored)
     try:
         v0 = batch.pack([phis[i] for i in live])
         a, b, _ = batch.lanczos(v0, nsteps, graph=graph)

@@ -28,8 +28,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
+#include "ed_hmatrow.h"
 #include "ed_hvrow.h"
 
 namespace ed {
@@ -41,10 +43,11 @@ struct BInst {
     SectorView s;
     LdsLayout lay;
     HvWeights w;
+    StoredView m;  // the instance's stored H when `stored`
     int64_t voff;  // first pool row of the instance
     int job0, job1;
-    int lds;  // tables staged in LDS
-    int pad;
+    int lds;     // tables staged in LDS (matrix-free instances)
+    int stored;  // rows from the stored H instead of the matrix-free walk
 };
 
 struct BJob {
@@ -103,18 +106,37 @@ __global__ void __launch_bounds__(BS) bhv_kernel(const BInst* __restrict__ insts
         if (threadIdx.x == 0 && partial) partial[2 * jb] = partial[2 * jb + 1] = 0.0;
         return;
     }
-    const SectorView s = I.s;
-    const Tabs T = I.lds ? stage_tabs<BS>(s, I.lay, lds) : global_tabs(s);
-    __syncthreads();
-    const HvWeights w = I.w;
     const double2* xi = x + I.voff;
     double2* yi = y + I.voff;
     C2 dot{0.0, 0.0};
-    for (int64_t f = job.r0 + threadIdx.x; f < job.r1; f += BS) {
-        C2 xf;
-        const C2 acc = hv_row<double2, G>(s, T, w, xi, f, xf);
-        yi[f] = double2{acc.re, acc.im};
-        dot = cadd(dot, cmul(C2{xf.re, -xf.im}, acc));
+    if (I.stored) {
+        const StoredView m = I.m;
+        double* V = reinterpret_cast<double*>(lds);
+        for (int i = threadIdx.x; i < 2 * m.n_vals; i += BS) V[i] = m.vals[i];
+        __syncthreads();
+        auto rows = [&](auto packed) {
+            for (int64_t f = job.r0 + threadIdx.x; f < job.r1; f += BS) {
+                C2 xf;
+                const C2 acc = stored_row<double2, 4, false, decltype(packed)::value>(m, V, xi, f, xf);
+                yi[f] = double2{acc.re, acc.im};
+                dot = cadd(dot, cmul(C2{xf.re, -xf.im}, acc));
+            }
+        };
+        if (m.shift)
+            rows(std::true_type{});
+        else
+            rows(std::false_type{});
+    } else {
+        const SectorView s = I.s;
+        const Tabs T = I.lds ? stage_tabs<BS>(s, I.lay, lds) : global_tabs(s);
+        __syncthreads();
+        const HvWeights w = I.w;
+        for (int64_t f = job.r0 + threadIdx.x; f < job.r1; f += BS) {
+            C2 xf;
+            const C2 acc = hv_row<double2, G>(s, T, w, xi, f, xf);
+            yi[f] = double2{acc.re, acc.im};
+            dot = cadd(dot, cmul(C2{xf.re, -xf.im}, acc));
+        }
     }
     if (partial) {
         wg_sum2(dot.re, dot.im);
@@ -276,7 +298,7 @@ constexpr int TUNE_GATHER[] = {2, 4, 8};
 using namespace ed;
 
 struct ed_batch {
-    std::vector<ed_sector*> sectors;
+    std::vector<const ed_sector*> sectors;
     std::vector<BInst> inst;
     std::vector<BJob> jobs;
     std::vector<int64_t> offsets;  // n + 1
@@ -359,6 +381,39 @@ int ed_batch_create(ed_sector* const* sectors, const double* weights, int n, ed_
                 I.lds = I.lay.bytes <= LDS_CAP;
                 if (I.lds) b->lds_bytes = std::max(b->lds_bytes, I.lay.bytes);
                 I.w = make_weights(weights + 8 * i);
+                I.voff = off;
+                b->offsets.push_back(off);
+                off += s->D;
+                b->inst.push_back(I);
+            }
+            b->offsets.push_back(off);
+            b->build_jobs();
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int ed_batch_create_stored(const ed_hmat* const* mats, int n, ed_batch** out) {
+    return guarded([&] {
+        ED_REQUIRE(mats && out && n >= 1, ED_ERR_ARG, "bad batch arguments");
+        *out = nullptr;
+        auto* b = new ed_batch();
+        try {
+            int64_t off = 0;
+            for (int i = 0; i < n; ++i) {
+                ED_REQUIRE(mats[i], ED_ERR_ARG, "null stored matrix");
+                const ed_sector* s = hmat_sector(mats[i]);
+                b->sectors.push_back(s);
+                BInst I{};
+                I.s = s->view();
+                I.lay = lds_layout(s);
+                I.w = hmat_weights(mats[i]);
+                I.m = hmat_view(mats[i]);
+                I.stored = 1;
+                b->lds_bytes = std::max(b->lds_bytes, I.m.n_vals * 16);
                 I.voff = off;
                 b->offsets.push_back(off);
                 off += s->D;

@@ -30,6 +30,7 @@
 #include <utility>
 #include <vector>
 
+#include "ed_hmatrow.h"
 #include "ed_hvrow.h"
 
 namespace ed {
@@ -122,54 +123,18 @@ __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, 
     walk_row<4>(s, T, w, f, sink);
 }
 
-// y_f = sqrt(n_f) sum_k (+-V[code]) x_col + V[diag_f] x_f.  Persistent
+// The product over all rows (stored_row, ed_hmatrow.h).  Persistent
 // workgroups walk `tiles` consecutive 256-row tiles (consecutive workgroups of
 // one XCD take neighbouring ranges, so their gathers share that XCD's L2);
-// U elements per lane are loaded, then their U gathers issued together.
-// partial[workgroup] = sum conj(x_f) y_f over its rows.
-// Streamed, read-once data (the stored elements, counts, diagonal codes) and
-// the output can be moved with non-temporal hints (NT), so that L2 keeps the
-// gathered x_j instead of the matrix stream.
-template <bool NT, class T>
-__device__ inline T ld_stream(const T* p) {
-    if constexpr (NT)
-        return __builtin_nontemporal_load(p);
-    else
-        return *p;
-}
-
-// One stored element: column j, value index vi, sign sg.
-template <bool PACKED, bool NT>
-__device__ inline void ld_elem(const int32_t* __restrict__ col, const uint16_t* __restrict__ code,
-                               const uint32_t* __restrict__ pack, int shift, int64_t at, int32_t& j, int& vi,
-                               double& sg) {
-    if constexpr (PACKED) {
-        const uint32_t e = ld_stream<NT>(pack + at);
-        j = int32_t(e >> shift);
-        vi = int((e & ((1u << shift) - 1u)) >> 1);
-        sg = (e & 1u) ? -1.0 : 1.0;
-    } else {
-        j = ld_stream<NT>(col + at);
-        const uint16_t c = ld_stream<NT>(code + at);
-        vi = c & 0x7fff;
-        sg = (c & 0x8000) ? -1.0 : 1.0;
-    }
-}
-
+// with NT the output is stored non-temporally as well.  partial[workgroup] =
+// sum conj(x_f) y_f over its rows.
 template <class VT, int U, bool NT, bool PACKED>
-__global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* __restrict__ sptr,
-                                                     const int32_t* __restrict__ col,
-                                                     const uint16_t* __restrict__ code,
-                                                     const uint32_t* __restrict__ pack, int shift,
-                                                     const uint8_t* __restrict__ cnt,
-                                                     const uint16_t* __restrict__ diag,
-                                                     const uint8_t* __restrict__ orbit,
-                                                     const double* __restrict__ sq, const double* __restrict__ vals,
-                                                     int n_vals, const VT* __restrict__ x, VT* __restrict__ y,
-                                                     double* __restrict__ partial, int64_t tiles) {
+__global__ void __launch_bounds__(BS) hm_spmv_kernel(StoredView m, int64_t D, const VT* __restrict__ x,
+                                                     VT* __restrict__ y, double* __restrict__ partial,
+                                                     int64_t tiles) {
     extern __shared__ __align__(16) unsigned char smem[];
     double* V = reinterpret_cast<double*>(smem);
-    for (int i = threadIdx.x; i < 2 * n_vals; i += BS) V[i] = vals[i];
+    for (int i = threadIdx.x; i < 2 * m.n_vals; i += BS) V[i] = m.vals[i];
     __syncthreads();
     const int nwg = gridDim.x;
     int wg = blockIdx.x;
@@ -179,38 +144,8 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(int64_t D, const int64_t* _
     const int64_t f1 = f0 + tiles * BS < D ? f0 + tiles * BS : D;
     C2 dot{0.0, 0.0};
     for (int64_t f = f0 + threadIdx.x; f < f1; f += BS) {
-        const int64_t base = sptr[f >> 6] + (f & 63);
-        const int n = ld_stream<NT>(cnt + f);
-        C2 acc{0.0, 0.0};
-        int k = 0;
-        for (; k + U <= n; k += U) {
-            int32_t j[U];
-            int vi[U];
-            double sg[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                ld_elem<PACKED, NT>(col, code, pack, shift, base + int64_t(k + u) * 64, j[u], vi[u], sg[u]);
-            C2 xv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = ldx(x, j[u]);
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const C2 v{sg[u] * V[2 * vi[u]], sg[u] * V[2 * vi[u] + 1]};
-                acc = cadd(acc, cmul(v, xv[u]));
-            }
-        }
-        for (; k < n; ++k) {
-            int32_t j;
-            int vi;
-            double sg;
-            ld_elem<PACKED, NT>(col, code, pack, shift, base + int64_t(k) * 64, j, vi, sg);
-            const C2 xv = ldx(x, j);
-            acc = cadd(acc, cmul(C2{sg * V[2 * vi], sg * V[2 * vi + 1]}, xv));
-        }
-        const double sqf = sq[orbit[f]];
-        const C2 xf = ldx(x, f);
-        const int dv = ld_stream<NT>(diag + f);
-        const C2 yf = cadd(C2{sqf * acc.re, sqf * acc.im}, cmul(C2{V[2 * dv], V[2 * dv + 1]}, xf));
+        C2 xf;
+        const C2 yf = stored_row<VT, U, NT, PACKED>(m, V, x, f, xf);
         VT o;
         o.x = decltype(o.x)(yf.re);
         o.y = decltype(o.y)(yf.im);
@@ -288,6 +223,13 @@ namespace ed {
 
 const ed_sector* hmat_sector(const ed_hmat* m) { return m->s; }
 
+HvWeights hmat_weights(const ed_hmat* m) { return m->w; }
+
+StoredView hmat_view(const ed_hmat* m) {
+    return StoredView{m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift, m->n_vals, m->d_cnt, m->d_diag,
+                      m->s->d_orbit, m->s->d_sqrt, m->d_vals};
+}
+
 // One 256-row tile per workgroup: the fastest setting measured for the 2x12
 // sector (ed_hmat_tune, product plus the reduction of its partials).
 int hmat_default_tiles(int64_t) { return 1; }
@@ -299,10 +241,10 @@ void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int6
     // measured on the 2x12 sector: non-temporal streaming gains 4 % with complex128 vectors, nothing with
     // complex64 (ed_hmat_tune)
     const bool nt = m->nontemporal < 0 ? sizeof(VT) == 16 : m->nontemporal != 0;
+    const StoredView v = hmat_view(m);
     auto go = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, dim3(unsigned(nwg)), dim3(BS), lds, st, m->D, m->d_sptr, m->d_col, m->d_code,
-                           m->d_pack, m->shift, m->d_cnt, m->d_diag, m->s->d_orbit, m->s->d_sqrt, m->d_vals,
-                           m->n_vals, static_cast<const VT*>(x), static_cast<VT*>(y), partial, tiles);
+        hipLaunchKernelGGL(kernel, dim3(unsigned(nwg)), dim3(BS), lds, st, v, m->D, static_cast<const VT*>(x),
+                           static_cast<VT*>(y), partial, tiles);
     };
     if (m->shift) {
         if (nt)

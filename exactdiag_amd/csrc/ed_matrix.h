@@ -21,6 +21,9 @@ int64_t hv_partials(const ed_sector* s);
 int hv_default_tiles(const ed_sector* s);
 int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st);
 const ed_sector* hmat_sector(const ed_hmat* m);
+HvWeights hmat_weights(const ed_hmat* m);
+struct StoredView;
+StoredView hmat_view(const ed_hmat* m);
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st);
 
 }  // namespace ed

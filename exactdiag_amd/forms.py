@@ -126,8 +126,11 @@ def _spectral_excitations(L, h, nd, k0):
 
 
 def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspace(-6, 6, 2001), eta=0.05,
-                  tol=1e-9, serial=True, log=print):
-    """All k sectors x couplings as one batch, then A(k, w) for every k as another; optionally the serial runs."""
+                  tol=1e-9, serial=True, stored=True, log=print):
+    """All k sectors x couplings as one batch, then A(k, w) for every k as another; optionally the serial runs.
+
+    With ``stored`` both the batch and the one-sector-at-a-time runs use stored H (built inside the timed
+    regions of the spectra; for the ground states the builds are timed separately, ``stored_build_s``)."""
     import torch
 
     from exactdiag_amd.batch import Batch, Excitation, lowest_eigenpairs, spectra
@@ -139,7 +142,10 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
     t_basis = time.perf_counter() - t0
     weights = [[1.0, 1.0, J, J] for J in Js]
     inst = [(J, kk) for J in Js for kk in ks]
-    batch = Batch([sectors[kk] for _, kk in inst], [[1.0, 1.0, J, J] for J, _ in inst])
+    t0 = time.perf_counter()
+    batch = Batch([sectors[kk] for _, kk in inst], [[1.0, 1.0, J, J] for J, _ in inst], stored=stored)
+    _sync()
+    t_build = time.perf_counter() - t0
     tiles, gather, ms = batch.tune()
     log(f"batching form: {len(inst)} instances, dims {int(batch.dims.min())}..{int(batch.dims.max())}, total "
         f"{batch.total}; tuned tiles={tiles} gather={gather} ({ms:.2f} ms per batched H·v)")
@@ -151,17 +157,23 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
     log(f"  batched lowest eigenpairs: {t_gs_batch:.2f}s ({restarts} restarts)")
     out = {"instances": len(inst), "dims": [int(d) for d in batch.dims], "basis_s": t_basis,
            "tuned": {"tiles": tiles, "gather": gather, "ms_per_batched_hv": ms},
-           "gs_batched_s": t_gs_batch, "gs_restarts": restarts}
+           "gs_batched_s": t_gs_batch, "gs_restarts": restarts, "stored": stored,
+           "stored_build_s": t_build if stored else 0.0}
     if serial:
+        ops = {}
+        for J, kk in inst:
+            wa = weights_array([1.0, 1.0, J, J])
+            ops[(J, kk)] = (sectors[kk].store(wa).hv if stored
+                            else (lambda v, s=sectors[kk], wa=wa: s.hv(wa, v)))
+        _sync()
         t1 = time.perf_counter()
         e_serial = []
         for J, kk in inst:
-            s = sectors[kk]
-            wa = weights_array([1.0, 1.0, J, J])
-            vals, _, _ = krylov.lowest_eigenpairs(lambda v, s=s, wa=wa: s.hv(wa, v), s.dim, 1, tol=tol)
+            vals, _, _ = krylov.lowest_eigenpairs(ops[(J, kk)], sectors[kk].dim, 1, tol=tol)
             e_serial.append(float(vals[0]))
         _sync()
         out["gs_serial_s"] = time.perf_counter() - t1
+        del ops
         out["gs_max_abs_diff"] = float(np.max(np.abs(np.array(e_serial) - energies)))
         log(f"  serial lowest eigenpairs: {out['gs_serial_s']:.2f}s, max |dE| {out['gs_max_abs_diff']:.2e}")
     # ground state per coupling: the lowest of its sectors
@@ -182,7 +194,7 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
             exc.append(Excitation(name, op, qx, qy, exc_params[prm]))
         _sync()
         t1 = time.perf_counter()
-        got, coeffs = spectra(exc, sectors[k0], psi, e0, weights[ji], ws, eta, nsteps)
+        got, coeffs = spectra(exc, sectors[k0], psi, e0, weights[ji], ws, eta, nsteps, stored=stored)
         _sync()
         t_spec_b += time.perf_counter() - t1
         spec_out[J] = {"k0": k0, "e0": e0, "n_spectra": len(exc)}
@@ -194,7 +206,12 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
                 n2 = krylov.vec_norm(phi) ** 2 if ex.sector.dim else 0.0
                 if n2 < 1e-16:
                     continue
-                a, b = ex.sector.lanczos(wa, phi, nsteps)
+                if stored:
+                    m = ex.sector.store(wa)
+                    a, b = m.lanczos(phi, nsteps)
+                    m.close()
+                else:
+                    a, b = ex.sector.lanczos(wa, phi, nsteps)
                 ref = cf_eval(a, b, ws, e0, eta, -n2 / np.pi)
                 max_diff = max(max_diff, spectral_l1(g, ref, ws))
             _sync()

@@ -166,6 +166,9 @@ int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const
  * The sectors must outlive the batch. */
 typedef struct ed_batch ed_batch;
 int ed_batch_create(ed_sector* const* sectors, const double* weights, int n, ed_batch** out);
+/* The same batch over stored matrices (ed_hmat_create): each instance's H·v
+ * reads its stored H.  The matrices and their sectors must outlive the batch. */
+int ed_batch_create_stored(const ed_hmat* const* mats, int n, ed_batch** out);
 int ed_batch_destroy(ed_batch* b);
 int ed_batch_offsets(const ed_batch* b, int64_t* offsets);
 /* d_y = H d_x for every instance; dots (host, 2n doubles, optional) gets

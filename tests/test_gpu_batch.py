@@ -206,3 +206,26 @@ def test_autotuner_keeps_results(torch, mixed):
     t, g, ms = s.tune(W)
     assert t in (1, 2, 4, 8, 16, 32) and g in (2, 4, 8)
     assert float((s.hv(W, xs) - ys0).abs().max()) <= 1e-14 * float(ys0.abs().max())
+
+
+def test_stored_batch_matches_matrix_free_batch(torch, mixed):
+    """Instances reading their stored H give the matrix-free batch's products and Lanczos coefficients."""
+    from exactdiag_amd.batch import Batch
+
+    ws = [[1.0, 1.0, 0.4, 0.4], [0.45, 0.36, 0.15, 0.12], [1.0 + 0.3j, 0.7, 0.4j, 0.25]]
+    inst = [(s, ws[i % 3]) for i, s in enumerate(mixed)] + [(mixed[0], ws[1]), (mixed[0], ws[1])]
+    b0 = Batch([s for s, _ in inst], [w for _, w in inst])
+    b1 = Batch([s for s, _ in inst], [w for _, w in inst], stored=True)
+    x = b0.random(9)
+    y0, d0 = b0.hv(x, with_dots=True)
+    y1, d1 = b1.hv(x, with_dots=True)
+    assert float((y1 - y0).abs().max()) <= 1e-13 * float(y0.abs().max())
+    np.testing.assert_allclose(d1, d0, rtol=1e-12, atol=1e-12)
+    a0, bb0, _ = b0.lanczos(x, 30)
+    a1, bb1, _ = b1.lanczos(x, 30)
+    for i in range(len(inst)):
+        assert len(a0[i]) == len(a1[i])
+        np.testing.assert_allclose(a1[i], a0[i], rtol=0, atol=1e-10)
+    tiles, gather, ms = b1.tune()
+    assert float((b1.hv(x) - y1).abs().max()) <= 1e-14 * float(y1.abs().max())
+    b1.close()
