@@ -27,6 +27,8 @@ EXPORTED = [
     "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune", "ed_lanczos_ritz",
     "ed_hmat_create", "ed_hmat_destroy", "ed_hmat_info", "ed_hmat_hv", "ed_hmat_lanczos_run", "ed_hmat_lanczos_ritz",
     "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_batch_create_stored",
+    "ed_csr_from_sector", "ed_csr_from_operator", "ed_csr_create", "ed_csr_info", "ed_csr_copy_out", "ed_csr_destroy",
+    "ed_csr_sum_create", "ed_csr_sum_info", "ed_csr_sum_hv", "ed_csr_sum_destroy",
 ]
 
 _vp = ct.c_void_p
@@ -85,6 +87,17 @@ _SIGNATURES = {
     "ed_hmat_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int]),
     "ed_hmat_element_bytes": (ct.c_int, [_vp]),
     "ed_batch_create_stored": (ct.c_int, [ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp)]),
+    "ed_csr_from_sector": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(_vp), _vp]),
+    "ed_csr_from_operator": (ct.c_int, [_vp, _vp, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(_vp), _vp]),
+    "ed_csr_create": (ct.c_int, [_i64, _i64, ct.c_int, ct.c_int, _i64, ct.c_int, _vp, _vp, _vp, _vp,
+                                 ct.POINTER(_vp), _vp]),
+    "ed_csr_info": (ct.c_int, [_vp, ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64), _ip, _ip]),
+    "ed_csr_copy_out": (ct.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ed_csr_destroy": (ct.c_int, [_vp]),
+    "ed_csr_sum_create": (ct.c_int, [ct.POINTER(_vp), _dp, ct.c_int, ct.POINTER(_vp), _vp]),
+    "ed_csr_sum_info": (ct.c_int, [_vp, ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64)]),
+    "ed_csr_sum_hv": (ct.c_int, [_vp, _vp, _vp, _vp]),
+    "ed_csr_sum_destroy": (ct.c_int, [_vp]),
 }
 
 _lib = None

@@ -25,5 +25,10 @@ HvWeights hmat_weights(const ed_hmat* m);
 struct StoredView;
 StoredView hmat_view(const ed_hmat* m);
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st);
+// Exact per-column elements of columns [lo, lo + n) (the reference's
+// calc_column_matrix_elements); count[t] > column_capacity() flags overflow.
+void launch_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, int triangle, int max_per_col,
+                    int64_t lo, int64_t n, int32_t* d_count, int64_t* d_rows, double* d_vals, hipStream_t st);
+int column_capacity();
 
 }  // namespace ed

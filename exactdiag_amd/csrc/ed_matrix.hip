@@ -77,10 +77,14 @@ __device__ inline int commutation_parity(uint32_t E, int arity, int a, int b) {
 
 __device__ inline bool cabs_gt(C2 v, double thr2) { return v.re * v.re + v.im * v.im > thr2; }
 
+// Columns [lo, lo + n) of the initial sector; column lo + t is written at
+// slot t of count / rows / vals.
 __global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView fin, RuleView r, int triangle,
-                                                      int32_t* count, int64_t* rows, double* vals) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= ini.D) return;
+                                                      int64_t lo, int64_t n_cols, int32_t* count, int64_t* rows,
+                                                      double* vals) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_cols) return;
+    const int64_t i = lo + t;
     const uint64_t key = ini.key[i];
     const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
     const uint32_t E = ~H & ini.full;
@@ -190,12 +194,12 @@ __global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView 
         ci[q + 1] = ki;
         cv[q + 1] = kv;
     }
-    count[i] = overflow ? MAXC + 1 : m;
+    count[t] = overflow ? MAXC + 1 : m;
     const int keep = m < r.max_per_col ? m : r.max_per_col;
     for (int k = 0; k < keep; ++k) {
-        rows[i * r.max_per_col + k] = ci[k];
-        vals[2 * (i * r.max_per_col + k)] = cv[k].re;
-        vals[2 * (i * r.max_per_col + k) + 1] = cv[k].im;
+        rows[t * r.max_per_col + k] = ci[k];
+        vals[2 * (t * r.max_per_col + k)] = cv[k].re;
+        vals[2 * (t * r.max_per_col + k) + 1] = cv[k].im;
     }
 }
 
@@ -667,19 +671,28 @@ struct RuleDev {
 void run_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, int triangle, int max_per_col,
                  int32_t* d_count, int64_t* d_rows, double* d_vals, hipStream_t st) {
     ED_REQUIRE(ini && ini->D >= 0 && fin && fin->D >= 0, ED_ERR_STATE, "basis not built");
+    launch_columns(ini, fin, rule, triangle, max_per_col, 0, ini->D, d_count, d_rows, d_vals, st);
+}
+
+}  // namespace
+
+void launch_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, int triangle, int max_per_col,
+                    int64_t lo, int64_t n, int32_t* d_count, int64_t* d_rows, double* d_vals, hipStream_t st) {
+    ED_REQUIRE(ini && ini->D >= 0 && fin && fin->D >= 0, ED_ERR_STATE, "basis not built");
     ED_REQUIRE(ini->L == fin->L, ED_ERR_ARG, "sectors of different ladders");
     ED_REQUIRE(max_per_col >= 1 && max_per_col <= MAXC, ED_ERR_ARG, "max_per_col out of range");
-    if (ini->D == 0) return;
+    ED_REQUIRE(lo >= 0 && n >= 0 && lo + n <= ini->D, ED_ERR_ARG, "column range outside the sector");
+    if (n == 0) return;
     RuleDev rd(rule, st);
     rd.v.max_per_col = max_per_col;
     const int T = 128;
-    hipLaunchKernelGGL(column_kernel, dim3(unsigned((ini->D + T - 1) / T)), dim3(T), 0, st, ini->view(),
-                       fin->view(), rd.v, triangle, d_count, d_rows, d_vals);
+    hipLaunchKernelGGL(column_kernel, dim3(unsigned((n + T - 1) / T)), dim3(T), 0, st, ini->view(), fin->view(),
+                       rd.v, triangle, lo, n, d_count, d_rows, d_vals);
     ED_HIP(hipGetLastError());
     ED_HIP(hipStreamSynchronize(st));
 }
 
-}  // namespace
+int column_capacity() { return MAXC; }
 
 // Launch helper shared with the Lanczos driver; returns the number of
 // workgroup partials written.  ED_HV_KERNEL selects a kernel for

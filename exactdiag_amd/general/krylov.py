@@ -89,6 +89,36 @@ def device_start_vector(D, seed=0):
     return v
 
 
+def plain_lanczos(apply, v0, nsteps):
+    """(a_n, b_n) of the plain recurrence for any device operator (cython_lanczos_diagonalization.pyx:50-77).
+
+    v_0 = v0 / |v0|; a_n = Re<v_n, H v_n>; w = H v_n - a_n v_n - b_n v_{n-1}; b_{n+1} = |w|; stop after
+    the step whose b_{n+1} < 1e-8.  Returns a[:n], b[:n] with b[0] = 0, as the reference's
+    _lanczos_get_anbn.  One host round trip per coefficient: for operators without a fused driver.
+    """
+    torch = _torch()
+    D = v0.numel()
+    V = torch.zeros((2, D), dtype=torch.complex128, device=v0.device)
+    V[0].copy_(v0)
+    vec_scale(V[0], 1.0 / vec_norm(V[0]))
+    a = np.zeros(nsteps)
+    b = np.zeros(nsteps + 1)
+    cur, n = 0, 0
+    for i in range(nsteps):
+        w = apply(V[cur])
+        a[i] = float(np.real(vec_dots(V[cur : cur + 1], 1, w)[0]))
+        coef = [a[i], b[i]] if cur == 0 else [b[i], a[i]]  # rows in memory order
+        vec_axpys(V, 2, coef, w)
+        b[i + 1] = vec_norm(w)
+        n += 1
+        if b[i + 1] < 1e-8:
+            break
+        V[1 - cur].copy_(w)
+        vec_scale(V[1 - cur], 1.0 / b[i + 1])
+        cur = 1 - cur
+    return a[:n], b[:n]
+
+
 def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=500):
     """Lowest ``nev`` eigenpairs of the Hermitian operator ``apply`` (device tensor in, out).
 

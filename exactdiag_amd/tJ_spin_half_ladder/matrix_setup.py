@@ -244,3 +244,122 @@ def setup_excitation_operator(initial_config):
 
 # keep functools imported for callers that build partials like the reference
 partial = functools.partial
+
+
+# ---------------------------------------------------------------------------
+# The reference's matrix folders under the calc folder
+# (general/sparse_matrices.py:246-274, matrix_setup.pyx:41-129 and :195-374).
+
+TERM_IDS = {"tl": 0, "tr": 1, "jl": 2, "jr": 3}
+
+
+def term_max_values_per_column(config, name):
+    """max_num_values_per_column of a term, as the reference sets it (matrix_setup.pyx:80-108)."""
+    h, L = config.num_holes, config.num_rungs
+    spins = 2 * L - h
+    return {"tl": 2 * h, "tr": min(h, L), "jl": min(spins, L) + 1, "jr": min(spins, L) + 1}[name]
+
+
+def hamiltonian_term_signature(config, name):
+    """The Signature setup_hamiltonian gives term `name` (matrix_setup.pyx:54-71, :176-190)."""
+    from exactdiag_amd.general.sparse_matrices import FILE_NAMES, Signature
+
+    D = get_sector(config).dim
+    strings = ["hermitian", "sorted"] + (["triangle_only"] if config.triangle_only else [])
+    return Signature(matrix_name=name, shape=(D, D), symmetry_strings=tuple(strings),
+                     max_values_per_column=term_max_values_per_column(config, name), major=config.major,
+                     initial_system_info=config.get_symmetry_info(), final_system_info=None,
+                     folder_name=str(config.get_calc_folder() / name), **FILE_NAMES)
+
+
+def write_hamiltonian_term(config, name):
+    """Build term `name` on the GPU in the config's layout, write its folder, return it opened."""
+    from exactdiag_amd.csr import StoredCSR
+    from exactdiag_amd.general.sparse_matrices import Sparse_Matrix
+
+    if name not in TERM_IDS:
+        raise ValueError(f"{name} is not a Hamiltonian term")
+    m = StoredCSR.from_sector(get_sector(config), TERM_IDS[name], triangle_only=config.triangle_only,
+                              row_major=config.major == "row")
+    limit = term_max_values_per_column(config, name)
+    if m.max_count > limit:
+        _logger.warning(f"{name}: a column has {m.max_count} elements, more than the {limit} the reference "
+                        f"allows for it (its build raises IndexError for this sector); written anyway.")
+    try:
+        return Sparse_Matrix.from_device(m, hamiltonian_term_signature(config, name))
+    finally:
+        m.close()
+
+
+def get_hamiltonian_matrices(config, overwrite=False):
+    """setup_hamiltonian(config)() of the reference, on disk (general/sparse_matrices.py:246-274).
+
+    Every term with a nonzero weight is opened from <calc folder>/<term> when saved there, otherwise built
+    on the GPU and saved; returns their weighted sum (``Added_Sparse_Matrices``, applied on the GPU).
+    """
+    from exactdiag_amd.general.sparse_matrices import FILE_NAMES, Added_Sparse_Matrices, Sparse_Matrix
+
+    mats, weights = [], []
+    for name, w in _weights_dict(config).items():
+        if w == 0:
+            continue
+        folder = config.get_calc_folder() / name
+        if overwrite or not (folder / FILE_NAMES["info_name"]).exists():
+            mats.append(write_hamiltonian_term(config, name))
+        else:
+            mats.append(Sparse_Matrix(hamiltonian_term_signature(config, name), num_threads=config.num_threads))
+        weights.append(w)
+    if not mats:
+        raise ValueError("all Hamiltonian weights are zero")
+    return Added_Sparse_Matrices(mats, weights)
+
+
+def _operator_max_values_per_column(initial_config, op):
+    """max_num_values_per_column of an excitation operator (matrix_setup.pyx:240-323)."""
+    ham = initial_config.hamiltonian
+    h, nodes = ham.num_holes, 2 * ham.num_rungs
+    if op in (_lib.OP_SZQ, _lib.OP_NQ):
+        return 1
+    if op == _lib.OP_CURRENT_RUNG:
+        return h
+    if op == _lib.OP_CURRENT_LEG:
+        return 2 * h
+    if op in (_lib.OP_APLUS_DOWN, _lib.OP_APLUS_UP):
+        return h
+    if op == _lib.OP_AMINUS_DOWN:
+        return ham.num_down_spins
+    return nodes - h - ham.num_down_spins
+
+
+def excitation_operator_signature(initial_config):
+    """The Signature setup_excitation_operator gives the operator (matrix_setup.pyx:350-364)."""
+    from exactdiag_amd.general.sparse_matrices import FILE_NAMES, Signature
+
+    final, op, _, _ = excitation_setup(initial_config)
+    qs = initial_config.spectrum.operator_symmetry_qs
+    q_string = f"_{qs.to_npint32()}" if qs is not None else ""
+    full_name = f"{initial_config.spectrum.name}{q_string}"
+    shape = (get_sector(final.hamiltonian).dim, get_sector(initial_config.hamiltonian).dim)
+    return Signature(matrix_name=full_name, shape=shape, symmetry_strings=("sorted",),
+                     max_values_per_column=_operator_max_values_per_column(initial_config, op), major="column",
+                     initial_system_info=initial_config.hamiltonian.get_symmetry_info(),
+                     final_system_info=final.hamiltonian.get_symmetry_info(),
+                     folder_name=str(initial_config.hamiltonian.get_calc_folder() / full_name), **FILE_NAMES)
+
+
+def get_excitation_operator_matrix(initial_config, overwrite=False):
+    """get_operator() of the reference's setup_excitation_operator, on disk: opened when saved, else built
+    on the GPU and saved (column-major, final x initial)."""
+    from exactdiag_amd.csr import StoredCSR
+    from exactdiag_amd.general.sparse_matrices import FILE_NAMES, Sparse_Matrix
+
+    sig = excitation_operator_signature(initial_config)
+    folder = initial_config.hamiltonian.get_calc_folder() / sig.matrix_name
+    if not overwrite and (folder / FILE_NAMES["info_name"]).exists():
+        return Sparse_Matrix(sig)
+    final, op, qx, qy = excitation_setup(initial_config)
+    m = StoredCSR.from_operator(get_sector(initial_config.hamiltonian), get_sector(final.hamiltonian), op, qx, qy)
+    try:
+        return Sparse_Matrix.from_device(m, sig)
+    finally:
+        m.close()

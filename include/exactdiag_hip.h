@@ -154,6 +154,56 @@ int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double
 int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const double* ritz, double* d_x,
                          int precision, void* stream);
 
+/* The reference's on-disk matrices, held in HBM.  An ed_csr is one stored
+ * matrix in the reference's layout: values complex128[n_values] (re, im
+ * pairs), value_inds int16[nnz], minor int64[nnz], major int64[n_major + 1],
+ * n_major = rows for row-major matrices, cols for column-major ones.
+ *
+ * ed_csr_from_sector builds Hamiltonian term `term` (0 tl, 1 tr, 2 jl, 3 jr)
+ * of a sector as calc_sparse_matrix writes it for setup_hamiltonian
+ * (general/cython_sparse_matrices.pyx:58-241 with the rules of
+ * tJ_spin_half_ladder/matrix_setup.pyx:41-129): sorted, unique values
+ * joined within 1e-6 in order of first appearance; triangle = the
+ * config's triangle_only (upper triangle, read back as a Hermitian
+ * triangle), row_major = config.major == "row" (value table conjugated,
+ * pyx:139-145).  The reference's default is triangle = row_major = 1.  ed_csr_from_operator builds an
+ * excitation operator from sector init to sector fin as
+ * setup_excitation_operator stores it (matrix_setup.pyx:195-374):
+ * column-major, shape (fin rows, init cols).  Both return ED_ERR_CAPACITY
+ * where the reference raises IndexError (more than 32767 unique values).
+ *
+ * ed_csr_create uploads arrays read from a matrix folder
+ * (Sparse_Matrix.__init__, general/sparse_matrices.py:27-80); it checks the
+ * pointers and indices and rejects arrays that do not form a matrix.
+ * ed_csr_info / ed_csr_copy_out read a matrix back (host pointers; any may
+ * be null).  max_count = the longest major slice. */
+typedef struct ed_csr ed_csr;
+int ed_csr_from_sector(const ed_sector* s, int term, int triangle, int row_major, ed_csr** out, void* stream);
+int ed_csr_from_operator(const ed_sector* init, const ed_sector* fin, int op, int qx, int qy, ed_csr** out,
+                         void* stream);
+int ed_csr_create(int64_t rows, int64_t cols, int row_major, int hermitian_triangle, int64_t nnz, int n_values,
+                  const double* values, const int16_t* value_inds, const int64_t* minor, const int64_t* major,
+                  ed_csr** out, void* stream);
+int ed_csr_info(const ed_csr* m, int64_t* rows, int64_t* cols, int64_t* nnz, int* n_values, int* max_count);
+int ed_csr_copy_out(const ed_csr* m, double* values, int16_t* value_inds, int64_t* minor, int64_t* major,
+                    void* stream);
+int ed_csr_destroy(ed_csr* m);
+
+/* Weighted sum of stored matrices of one shape (weights: n complex values
+ * as 2n doubles, host memory), applied as Added_Sparse_Matrices.dot over
+ * Sparse_Matrix.dot / sparse_matrix_dot (general/sparse_matrices.py:83-105,
+ * :233-237; general/cython_sparse_dot.pyx:40-100;
+ * general/cpp_sparse_dot.h:59-138): explicit elements times w, elements of
+ * a Hermitian triangle applied transposed with conj(value) times w (w not
+ * conjugated), terms with |w| < 1e-6 left out.  ed_csr_sum_hv: d_y = sum
+ * (complex128 device vectors, d_y of `rows`, d_x of `cols` entries).  The
+ * matrices may be destroyed after ed_csr_sum_create. */
+typedef struct ed_csr_sum ed_csr_sum;
+int ed_csr_sum_create(const ed_csr* const* terms, const double* weights, int n, ed_csr_sum** out, void* stream);
+int ed_csr_sum_info(const ed_csr_sum* m, int64_t* rows, int64_t* cols, int64_t* nnz);
+int ed_csr_sum_hv(const ed_csr_sum* m, const double* d_x, double* d_y, void* stream);
+int ed_csr_sum_destroy(ed_csr_sum* m);
+
 /* Batched Lanczos over many sectors of uneven size (instances), each with
  * its own weights (8 doubles per instance as in ed_hv).  Replaces the
  * one-sector-at-a-time loops of get_all_k_eigenpairs

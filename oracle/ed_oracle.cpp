@@ -661,9 +661,12 @@ void csr_dot_add(const Csr& m, const cplx* x, cplx* y, int64_t D, cplx w, bool f
     }
 }
 
+// term[0..3]: the Hamiltonian terms in the default layout; term[EXTRA]: the
+// last matrix built by orc_build_layout_csr or orc_build_op_csr.
+constexpr int EXTRA = 4;
 struct Hamiltonian {
-    Csr term[4];
-    bool built[4] = {false, false, false, false};
+    Csr term[5];
+    bool built[5] = {false, false, false, false, false};
 };
 
 struct Handle {
@@ -786,6 +789,42 @@ int orc_build_term_csr(void* p, int term, int threads) {
         Columns c = all_columns(*hd->sec, *hd->sec, r, true, threads);
         hd->ham.term[term] = to_csr(c, true);
         hd->ham.built[term] = true;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// A Hamiltonian term in any of the reference's layouts (config.triangle_only,
+// config.major; cython_sparse_matrices.pyx:110-184): the column arrays are
+// kept and the value table is conjugated for row-major storage.  Kept in the
+// EXTRA slot.
+int orc_build_layout_csr(void* p, int term, int triangle, int row_major, int threads) {
+    try {
+        auto* hd = static_cast<Handle*>(p);
+        Rule r = hamiltonian_rule(hd->sec->N, term);
+        Columns c = all_columns(*hd->sec, *hd->sec, r, triangle != 0, threads);
+        hd->ham.term[EXTRA] = to_csr(c, row_major != 0);
+        hd->ham.built[EXTRA] = true;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// An excitation operator as setup_excitation_operator stores it
+// (column-major, tJ_spin_half_ladder/matrix_setup.pyx:340-371), kept in the
+// EXTRA slot of the initial sector's handle.
+int orc_build_op_csr(void* pi, void* pf, int op, int qx, int qy, int threads) {
+    try {
+        auto* hi = static_cast<Handle*>(pi);
+        auto& sf = *static_cast<Handle*>(pf)->sec;
+        Rule r = operator_rule(hi->sec->L, op, qx, qy);
+        Columns c = all_columns(*hi->sec, sf, r, false, threads);
+        hi->ham.term[EXTRA] = to_csr(c, false);
+        hi->ham.built[EXTRA] = true;
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

@@ -74,6 +74,8 @@ def lib():
         L.orc_term_csr_sizes.restype = ct.c_int64
         L.orc_term_csr_sizes.argtypes = [ct.c_void_p, ct.c_int, _i64p, _i64p]
         L.orc_term_csr_copy.argtypes = [ct.c_void_p, ct.c_int, _dp, _i16p, _i64p, _i64p]
+        L.orc_build_layout_csr.argtypes = [ct.c_void_p] + [ct.c_int] * 4
+        L.orc_build_op_csr.argtypes = [ct.c_void_p, ct.c_void_p] + [ct.c_int] * 4
         L.orc_hv.argtypes = [ct.c_void_p, _dp, _dp, _dp, ct.c_int]
         L.orc_cf.argtypes = [_dp, _dp, ct.c_int, _dp, ct.c_int, _dp]
         L.orc_lanczos.argtypes = [ct.c_void_p, _dp, _dp, ct.c_int, _dp, _dp, ct.c_int]
@@ -203,8 +205,24 @@ class OracleSector:
             raise IndexError(self._lib.orc_last_error().decode())
         self._terms_built.add(term)
 
+    def layout_csr(self, term, triangle=True, row_major=True, threads=None):
+        """Stored CSR of one term in any of the reference's layouts (triangle_only, major)."""
+        if self._lib.orc_build_layout_csr(self._h, term, int(triangle), int(row_major),
+                                          threads or default_threads()) != 0:
+            raise IndexError(self._lib.orc_last_error().decode())
+        return self._slot_csr(4)
+
+    def op_csr(self, final: "OracleSector", op, qx, qy, threads=None):
+        """Stored CSR of an excitation operator (column-major, final rows x initial columns)."""
+        if self._lib.orc_build_op_csr(self._h, final._h, op, qx, qy, threads or default_threads()) != 0:
+            raise IndexError(self._lib.orc_last_error().decode())
+        return self._slot_csr(4)
+
     def term_csr(self, term):
         self.build_csr((term,))
+        return self._slot_csr(term)
+
+    def _slot_csr(self, term):
         nu = np.zeros(1, np.int64)
         nr = np.zeros(1, np.int64)
         nnz = self._lib.orc_term_csr_sizes(self._h, term, _ptr(nu, _i64p), _ptr(nr, _i64p))
