@@ -173,7 +173,41 @@ def cpu_allq_estimate(cpu, section, D):
             "speedup_fast_vs_cpu": total / section["fast"]["seconds"]}
 
 
-def run_form(form):
+CSR_CPU_SAMPLE_ROWS = None  # None: every row (about 15 s on 16 cores for the 2x12 sector)
+
+
+def cpu_csr_build(mats, threads):
+    """The oracle's restatement of calc_sparse_matrix over rows [0, CSR_CPU_SAMPLE_ROWS) of each term (all
+    rows by default): its time (scaled to the whole sector when sampled) and whether the GPU's arrays
+    reproduce it (the whole arrays, or their prefix for a sample)."""
+    import numpy as np
+
+    from oracle import oracle
+
+    o = oracle.OracleSector(L, HOLES, NDOWN, KX, KY, threads=threads)
+    rows = o.dim if CSR_CPU_SAMPLE_ROWS is None else min(CSR_CPU_SAMPLE_ROWS, o.dim)
+    t_terms, same = [], True
+    for t in range(4):
+        t1 = time.perf_counter()
+        o.build_csr_rows(t, 0, rows, threads=threads)
+        t_terms.append(time.perf_counter() - t1)
+        vo, io, mo, po = o.term_csr(t)
+        vg, ig, mg, pg = mats[t].arrays()
+        if po[-1] == 1 and not po[:-1].any() and len(vo) == 1 and vo[0] == 0:
+            same &= bool(pg[rows] == 0)  # no elements in the sample rows (the oracle's empty-matrix marker)
+        else:
+            n = int(po[-1])
+            same &= bool(np.array_equal(pg[: rows + 1], po) and np.array_equal(mg[:n], mo)
+                         and np.array_equal(ig[:n], io) and np.array_equal(vg[: len(vo)], vo))
+        log(f"cpu csr build: term {t} rows [0, {rows}) in {t_terms[-1]:.1f}s; GPU prefix equal: {same}")
+    est = sum(t_terms) * o.dim / rows
+    scaled = "" if rows == o.dim else f", scaled by D / {rows}"
+    return {"value": est, "unit": "s", "cores": threads, "kind": "port",
+            "sample": (f"oracle restatement of calc_sparse_matrix (columns, unique-value walk, CSR) for rows "
+                       f"[0, {rows}) of each of the four terms (D = {o.dim}), {sum(t_terms):.1f}s{scaled}")}, same
+
+
+def run_form(form, args=None):
     import torch
 
     from exactdiag_amd import forms
@@ -181,7 +215,23 @@ def run_form(form):
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    if form == "batching":
+    if form == "csr":
+        res, mats = forms.csr_form(L, HOLES, NDOWN, (KX, KY), WEIGHTS, log=log)
+        line = {"metric": "GPU build of the reference's stored CSR matrices (four Hamiltonian terms), seconds",
+                "value": res["build_s"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                "dtype": "complex128", "data": "synthetic (model built from its definition)",
+                "config": {"workload": "2x12 ladder, 2 holes, Sz=0, k=(0,0), t=1, J=0.4: tl, tr, jl, jr as "
+                                       "row-major Hermitian upper triangles with unique-value tables"},
+                "form": res}
+        if args is None or not args.no_cpu_baseline:
+            threads = min(16, os.cpu_count() or 1)
+            cpu, same = cpu_csr_build(mats, threads)
+            line["cpu_baseline"] = cpu
+            line["arrays_match_oracle"] = same
+            line["speedup_vs_cpu"] = cpu["value"] / res["build_s"]
+        for m in mats:
+            m.close()
+    elif form == "batching":
         res = forms.batching_form(log=log)
         line = {"metric": "batched all-k lowest eigenpairs + all-k A(k,w) (seconds, batched vs one sector at a time)",
                 "value": res["gs_batched_s"] + res["spectra_batched_s"], "unit": "s", "n_gpus": 1,
@@ -229,15 +279,15 @@ def main():
     ap.add_argument("--no-allq", action="store_true", help="skip the all-q time-to-accuracy section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy one vector (known bytes) for PMC calibration")
-    ap.add_argument("--form", choices=["hv", "batching", "precision"], default="hv",
-                    help="hv: the headline H·v + all-q line; batching / precision: those forms' own line")
+    ap.add_argument("--form", choices=["hv", "batching", "precision", "csr"], default="hv",
+                    help="hv: the headline H·v + all-q line; batching / precision / csr: their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
               f"this build", file=sys.stderr)
         sys.exit(2)
     if args.form != "hv":
-        run_form(args.form)
+        run_form(args.form, args)
         return
 
     import ctypes as ct

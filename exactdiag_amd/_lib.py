@@ -95,7 +95,7 @@ _SIGNATURES = {
     "ed_csr_copy_out": (ct.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "ed_csr_destroy": (ct.c_int, [_vp]),
     "ed_csr_sum_create": (ct.c_int, [ct.POINTER(_vp), _dp, ct.c_int, ct.POINTER(_vp), _vp]),
-    "ed_csr_sum_info": (ct.c_int, [_vp, ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64)]),
+    "ed_csr_sum_info": (ct.c_int, [_vp, ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64)]),
     "ed_csr_sum_hv": (ct.c_int, [_vp, _vp, _vp, _vp]),
     "ed_csr_sum_destroy": (ct.c_int, [_vp]),
 }

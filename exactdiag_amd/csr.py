@@ -113,10 +113,10 @@ class CSRSum:
         check(self._lib.ed_csr_sum_create(handles, w.ctypes.data_as(ct.POINTER(ct.c_double)), len(mats),
                                           ct.byref(h), _stream()))
         self._h = h
-        rows, cols, nnz = ct.c_int64(), ct.c_int64(), ct.c_int64()
-        check(self._lib.ed_csr_sum_info(self._h, ct.byref(rows), ct.byref(cols), ct.byref(nnz)))
+        rows, cols, nnz, slots = ct.c_int64(), ct.c_int64(), ct.c_int64(), ct.c_int64()
+        check(self._lib.ed_csr_sum_info(self._h, ct.byref(rows), ct.byref(cols), ct.byref(nnz), ct.byref(slots)))
         self.shape = (rows.value, cols.value)
-        self.nnz = nnz.value
+        self.nnz, self.slots = nnz.value, slots.value  # elements; slots with the slice padding
         self.device = _torch().device("cuda", _torch().cuda.current_device())
 
     def hv(self, x, out=None):

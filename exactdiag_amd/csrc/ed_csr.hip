@@ -224,26 +224,74 @@ __device__ inline uint64_t pattern_key(double2 v) {
     return k ? k : 1;  // 0 marks an empty slot
 }
 
-// Insert every element's pattern; slot[e] = its slot, first[slot] = smallest element index.
-__global__ void hash_kernel(int64_t nnz, const double2* ev, uint32_t mask, unsigned long long* key,
-                           unsigned long long* first, int32_t* slot, int* err) {
-    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < nnz; e += int64_t(gridDim.x) * blockDim.x) {
-        const unsigned long long k = pattern_key(ev[e]);
-        uint32_t h = uint32_t(k ^ (k >> 32)) & mask;
-        bool placed = false;
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
-            unsigned long long cur = key[h];
-            if (cur == 0) cur = atomicCAS(&key[h], 0ull, k);
-            if (cur == 0 || cur == k) {
-                if (first[h] > (unsigned long long)e) atomicMin(&first[h], (unsigned long long)e);
-                slot[e] = int32_t(h);
-                placed = true;
-                break;
-            }
-            h = (h + 1) & mask;
-        }
-        if (!placed) atomicOr(err, 1);
+constexpr int HASH_LDS = 512;     // per-workgroup table of patterns (power of two)
+constexpr int HASH_ROUNDS = 64;   // elements per lane: a workgroup takes 256 * HASH_ROUNDS consecutive elements
+
+__device__ inline int global_insert(unsigned long long k, uint32_t mask, unsigned long long* key, int* err) {
+    uint32_t h = uint32_t(k ^ (k >> 32)) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        unsigned long long cur = key[h];
+        if (cur == 0) cur = atomicCAS(&key[h], 0ull, k);
+        if (cur == 0 || cur == k) return int(h);
+        h = (h + 1) & mask;
     }
+    atomicOr(err, 1);
+    return -1;
+}
+
+// Insert every element's pattern into the global set: slot[e] = its slot,
+// first[slot] = the smallest element index with that pattern.  Each
+// workgroup first collects its patterns in LDS (with their smallest index),
+// so the global set sees one insertion per pattern and workgroup; patterns
+// that do not fit the LDS table go to the global set directly.
+__global__ void __launch_bounds__(256) hash_kernel(int64_t nnz, const double2* ev, uint32_t mask,
+                                                   unsigned long long* key, unsigned long long* first,
+                                                   int32_t* slot, int* err) {
+    __shared__ unsigned long long lkey[HASH_LDS];
+    __shared__ unsigned long long lfirst[HASH_LDS];
+    __shared__ int lglob[HASH_LDS];
+    for (int i = threadIdx.x; i < HASH_LDS; i += 256) {
+        lkey[i] = 0;
+        lfirst[i] = ~0ull;
+        lglob[i] = -1;
+    }
+    __syncthreads();
+    const int64_t chunk0 = int64_t(blockIdx.x) * 256 * HASH_ROUNDS;
+    for (int round = 0; round < HASH_ROUNDS; ++round) {
+        const int64_t e = chunk0 + int64_t(round) * 256 + threadIdx.x;
+        int ls = -1;
+        unsigned long long k = 0;
+        if (e < nnz) {
+            k = pattern_key(ev[e]);
+            uint32_t h = uint32_t(k ^ (k >> 32)) & (HASH_LDS - 1);
+            for (int probe = 0; probe < HASH_LDS; ++probe) {
+                const unsigned long long cur = atomicCAS(&lkey[h], 0ull, k);
+                if (cur == 0 || cur == k) {
+                    ls = int(h);
+                    break;
+                }
+                h = (h + 1) & (HASH_LDS - 1);
+            }
+            if (ls >= 0) atomicMin(&lfirst[ls], (unsigned long long)e);
+        }
+        __syncthreads();
+        // one lane per new LDS entry inserts its pattern into the global set
+        if (ls >= 0 && lglob[ls] == -1 && atomicCAS(&lglob[ls], -1, -2) == -1)
+            lglob[ls] = global_insert(k, mask, key, err);
+        __syncthreads();
+        if (e < nnz) {
+            if (ls >= 0) {
+                slot[e] = lglob[ls];
+            } else {
+                const int g = global_insert(k, mask, key, err);
+                slot[e] = g;
+                if (g >= 0) atomicMin(&first[g], (unsigned long long)e);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < HASH_LDS; i += 256)
+        if (lkey[i] != 0 && lglob[i] >= 0) atomicMin(&first[lglob[i]], lfirst[i]);
 }
 
 __global__ void list_kernel(uint32_t cap, const unsigned long long* key, const unsigned long long* first,
@@ -379,7 +427,9 @@ ed_csr* build_csr(const ed_sector* ini, const ed_sector* fin, const RuleHost& ru
             list_val = dev_alloc<double2>(hcap);
             ED_HIP(hipMemsetAsync(key, 0, hcap * sizeof(unsigned long long), st));
             ED_HIP(hipMemsetAsync(first, 0xff, hcap * sizeof(unsigned long long), st));
-            hipLaunchKernelGGL(hash_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, st, nnz, ev, hcap - 1, key, first,
+            const int64_t hblocks = (nnz + 256 * HASH_ROUNDS - 1) / (256 * HASH_ROUNDS);
+            ED_REQUIRE(hblocks < (int64_t(1) << 31), ED_ERR_ARG, "too many elements");
+            hipLaunchKernelGGL(hash_kernel, dim3(unsigned(hblocks)), dim3(256), 0, st, nnz, ev, hcap - 1, key, first,
                                slot, flags + 1);
             hipLaunchKernelGGL(list_kernel, dim3((hcap + 255) / 256), dim3(256), 0, st, hcap, key, first, ev,
                                flags + 2, slot_idx, list_first, list_val);
@@ -484,22 +534,42 @@ __global__ void sum_fill_kernel(int64_t n_major, const int64_t* major, const int
     }
 }
 
-// Each output row's elements in (column, code) order.
-__global__ void sum_sort_kernel(int64_t rows, const int64_t* ptr, int32_t* col, uint32_t* code) {
-    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const int64_t s = ptr[r], e = ptr[r + 1];
-    for (int64_t k = s + 1; k < e; ++k) {
-        const int32_t ck = col[k];
-        const uint32_t dk = code[k];
-        int64_t q = k - 1;
-        while (q >= s && (col[q] > ck || (col[q] == ck && code[q] > dk))) {
-            col[q + 1] = col[q];
-            code[q + 1] = code[q];
-            --q;
+// Each output row's elements in (column, code) order: one wavefront per row,
+// a bitonic sort of the 64-bit keys (column << 32 | code) across the lanes;
+// rows longer than a wavefront are sorted by one lane in place.
+__global__ void __launch_bounds__(256) sum_sort_kernel(int64_t rows, const int64_t* ptr, int32_t* col,
+                                                       uint32_t* code) {
+    const int64_t r = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    if (r >= rows) return;  // uniform across the wavefront
+    const int64_t s = ptr[r], n = ptr[r + 1] - s;
+    if (n <= 1) return;
+    if (n > 64) {
+        if (lane != 0) return;
+        for (int64_t k = s + 1; k < s + n; ++k) {
+            const int32_t ck = col[k];
+            const uint32_t dk = code[k];
+            int64_t q = k - 1;
+            while (q >= s && (col[q] > ck || (col[q] == ck && code[q] > dk))) {
+                col[q + 1] = col[q];
+                code[q + 1] = code[q];
+                --q;
+            }
+            col[q + 1] = ck;
+            code[q + 1] = dk;
         }
-        col[q + 1] = ck;
-        code[q + 1] = dk;
+        return;
+    }
+    unsigned long long key = lane < n ? (uint64_t(uint32_t(col[s + lane])) << 32) | code[s + lane] : ~0ull;
+    for (int k = 2; k <= 64; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const unsigned long long other = __shfl_xor(key, j, 64);
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
+        }
+    if (lane < n) {
+        col[s + lane] = int32_t(key >> 32);
+        code[s + lane] = uint32_t(key);
     }
 }
 
@@ -748,8 +818,8 @@ int ed_csr_sum_create(const ed_csr* const* terms, const double* weights, int n, 
                                        d_col, d_code);
             }
             if (rows > 0)
-                hipLaunchKernelGGL(sum_sort_kernel, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, st, rows,
-                                   d_ptr, d_col, d_code);
+                hipLaunchKernelGGL(sum_sort_kernel, dim3(unsigned((rows + 3) / 4)), dim3(256), 0, st, rows, d_ptr,
+                                   d_col, d_code);
             ED_HIP(hipGetLastError());
             // sliced layout (rowcnt reused for the slice sizes)
             const int64_t n_slices = (rows + SLICE - 1) / SLICE;
@@ -775,12 +845,13 @@ int ed_csr_sum_create(const ed_csr* const* terms, const double* weights, int n, 
     });
 }
 
-int ed_csr_sum_info(const ed_csr_sum* m, int64_t* rows, int64_t* cols, int64_t* nnz) {
+int ed_csr_sum_info(const ed_csr_sum* m, int64_t* rows, int64_t* cols, int64_t* nnz, int64_t* slots) {
     return guarded([&] {
         ED_REQUIRE(m, ED_ERR_ARG, "null operator");
         if (rows) *rows = m->rows;
         if (cols) *cols = m->cols;
         if (nnz) *nnz = m->nnz;
+        if (slots) *slots = m->slots;
     });
 }
 

@@ -229,3 +229,58 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
     del x
     torch.cuda.empty_cache()
     return out
+
+
+def csr_form(L=12, h=2, nd=11, k=(0, 0), weights=(1.0, 1.0, 0.4, 0.4), reps=20, log=print):
+    """The reference's stored matrices of one sector built on the GPU, and their summed product.
+
+    calc_sparse_matrix (general/cython_sparse_matrices.pyx:185-241) for the four Hamiltonian terms, in the
+    default layout (row-major Hermitian upper triangle), then the weighted sum of the four applied as
+    Added_Sparse_Matrices.dot; the product is checked against the stored-H product.  Returns (results,
+    matrices) so that a caller can compare the arrays.
+    """
+    import torch
+
+    from exactdiag_amd.csr import CSRSum, StoredCSR
+
+    t0 = time.perf_counter()
+    s = Sector(L, h, nd, *k)
+    _sync()
+    out = {"dim": s.dim, "basis_s": time.perf_counter() - t0, "terms": []}
+    mats = []
+    for t in range(4):
+        _sync()
+        t1 = time.perf_counter()
+        m = StoredCSR.from_sector(s, t)
+        _sync()
+        dt = time.perf_counter() - t1
+        mats.append(m)
+        out["terms"].append({"term": ("tl", "tr", "jl", "jr")[t], "nnz": m.nnz, "n_values": m.n_values,
+                             "max_count": m.max_count, "build_s": dt})
+        log(f"  term {t}: {m.nnz} elements, {m.n_values} unique values, built in {dt:.3f}s")
+    out["build_s"] = sum(d["build_s"] for d in out["terms"])
+    t1 = time.perf_counter()
+    op = CSRSum(mats, list(weights))
+    _sync()
+    out["sum_build_s"] = time.perf_counter() - t1
+    x = krylov.device_start_vector(s.dim, 1)
+    y = op.hv(x)
+    hm = s.store(weights_array(list(weights)))
+    y_ref = hm.hv(x)
+    out["rel_diff_vs_stored_h"] = float((y - y_ref).abs().max() / y_ref.abs().max())
+    hm.close()
+    for _ in range(3):
+        op.hv(x, out=y)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        op.hv(x, out=y)
+    e1.record(stream)
+    _sync()
+    ms = e0.elapsed_time(e1) / reps
+    out.update({"sum_nnz": op.nnz, "sum_slots": op.slots, "hv_ms": ms, "hv_gstates_per_s": s.dim / ms / 1e6})
+    log(f"  sum of the four: {op.nnz} elements ({op.slots} slots), H·v {ms:.3f} ms, "
+        f"rel. diff vs stored H {out['rel_diff_vs_stored_h']:.1e}")
+    op.close()
+    return out, mats

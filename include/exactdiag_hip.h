@@ -197,10 +197,12 @@ int ed_csr_destroy(ed_csr* m);
  * a Hermitian triangle applied transposed with conj(value) times w (w not
  * conjugated), terms with |w| < 1e-6 left out.  ed_csr_sum_hv: d_y = sum
  * (complex128 device vectors, d_y of `rows`, d_x of `cols` entries).  The
- * matrices may be destroyed after ed_csr_sum_create. */
+ * matrices may be destroyed after ed_csr_sum_create.  Layout: rows in
+ * slices of 64, 8 bytes (column, value code) per element; ed_csr_sum_info
+ * gives the elements (nnz) and the slots with the slice padding. */
 typedef struct ed_csr_sum ed_csr_sum;
 int ed_csr_sum_create(const ed_csr* const* terms, const double* weights, int n, ed_csr_sum** out, void* stream);
-int ed_csr_sum_info(const ed_csr_sum* m, int64_t* rows, int64_t* cols, int64_t* nnz);
+int ed_csr_sum_info(const ed_csr_sum* m, int64_t* rows, int64_t* cols, int64_t* nnz, int64_t* slots);
 int ed_csr_sum_hv(const ed_csr_sum* m, const double* d_x, double* d_y, void* stream);
 int ed_csr_sum_destroy(ed_csr_sum* m);
 

@@ -266,3 +266,32 @@ def test_excitation_operator_folder(tmp_path, monkeypatch, oracle_mod, torch):
     again = matrix_setup.get_excitation_operator_matrix(cfg)  # reopened from the folder
     x = np.random.default_rng(2).normal(size=oi.dim) + 0j
     assert np.array_equal(again.dot(x), m.dot(x))
+
+
+def test_large_sector_prefix_and_table_invariants(oracle_mod, torch):
+    """2x10, 2 holes, k=(3, pi) (D = 461,890): the first 20,000 rows equal the oracle's build of those rows;
+    over the whole matrix the value table keeps the reference's walk: entries pairwise at least 1e-6 apart,
+    index u first used after index u - 1, every element within 1e-6 of its entry."""
+    from exactdiag_amd.csr import StoredCSR
+    from exactdiag_amd.sector import Sector
+
+    p = (10, 2, 9, 3, 1)
+    s, o = Sector(*p), oracle_mod.OracleSector(*p)
+    rows = 20000
+    for t in range(4):
+        print("term", t, flush=True)
+        vg, ig, mg, pg = StoredCSR.from_sector(s, t).arrays()
+        o.build_csr_rows(t, 0, rows)
+        vo, io, mo, po = o.term_csr(t)
+        if po[-1] == 1 and not po[:-1].any() and len(vo) == 1 and vo[0] == 0:
+            assert pg[rows] == 0  # no elements in these rows: the oracle wrote the empty-matrix marker
+        else:
+            n = int(po[-1])
+            assert _same(pg[: rows + 1], po) and _same(mg[:n], mo) and _same(ig[:n], io)
+            assert _same(vg[: len(vo)], vo)
+        assert np.all(np.abs(vg[:, None] - vg[None, :])[np.triu_indices(len(vg), 1)] >= 1e-6)
+        u, first = np.unique(ig, return_index=True)
+        assert np.array_equal(u, np.arange(len(vg))) and np.all(np.diff(first) > 0)
+        assert np.all(pg[1:] >= pg[:-1]) and pg[-1] == len(mg)
+        rows_of = np.repeat(np.arange(s.dim), np.diff(pg))
+        assert np.all(mg >= rows_of)  # upper triangle
