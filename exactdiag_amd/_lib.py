@@ -16,6 +16,7 @@ ED_OK, ED_ERR_ARG, ED_ERR_HIP, ED_ERR_STATE, ED_ERR_CAPACITY = 0, -1, -2, -3, -4
 
 (OP_SZQ, OP_NQ, OP_APLUS_DOWN, OP_APLUS_UP, OP_AMINUS_DOWN, OP_AMINUS_UP, OP_CURRENT_RUNG,
  OP_CURRENT_LEG) = range(8)
+POS_HOLES, POS_SZ, POS_SINGLET = range(3)
 
 EXPORTED = [
     "ed_last_error", "ed_version", "ed_sector_create", "ed_sector_destroy", "ed_basis_build", "ed_basis_size",
@@ -29,6 +30,8 @@ EXPORTED = [
     "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_batch_create_stored",
     "ed_csr_from_sector", "ed_csr_from_operator", "ed_csr_create", "ed_csr_info", "ed_csr_copy_out", "ed_csr_destroy",
     "ed_csr_sum_create", "ed_csr_sum_info", "ed_csr_sum_hv", "ed_csr_sum_destroy",
+    "ed_position_op_columns", "ed_position_op_apply", "ed_position_expectation", "ed_position_correlations",
+    "ed_csr_from_position_operator",
 ]
 
 _vp = ct.c_void_p
@@ -98,6 +101,11 @@ _SIGNATURES = {
     "ed_csr_sum_info": (ct.c_int, [_vp, ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64)]),
     "ed_csr_sum_hv": (ct.c_int, [_vp, _vp, _vp, _vp]),
     "ed_csr_sum_destroy": (ct.c_int, [_vp]),
+    "ed_position_op_columns": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp]),
+    "ed_position_op_apply": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, _vp, _vp, _vp]),
+    "ed_position_expectation": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, _vp, _dp, _vp]),
+    "ed_position_correlations": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, _vp, _dp, _vp]),
+    "ed_csr_from_position_operator": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, ct.POINTER(_vp), _vp]),
 }
 
 _lib = None

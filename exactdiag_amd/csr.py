@@ -55,6 +55,18 @@ class StoredCSR:
         return m
 
     @classmethod
+    def from_position_operator(cls, sector: Sector, kind: int, shifts):
+        """Position-correlation operator (0 holes, 1 S^z, 2 singlet-singlet) as
+        get_position_correlation_operator stores it (column-major, sorted)."""
+        sh = np.ascontiguousarray(np.asarray(shifts, np.int32).reshape(-1, 2))
+        h = ct.c_void_p()
+        check(lib().ed_csr_from_position_operator(sector._h, int(kind), sh.ctypes.data, len(sh), ct.byref(h),
+                                                  _stream()))
+        m = cls(h, keep=(sector,))
+        m.row_major, m.hermitian_triangle = False, False
+        return m
+
+    @classmethod
     def from_arrays(cls, shape, row_major: bool, hermitian_triangle: bool, values, value_inds, minor, major):
         """Upload arrays in the reference's layout (checked: pointers span the elements, indices in range)."""
         values = np.ascontiguousarray(values, np.complex128)

@@ -648,6 +648,15 @@ int ed_csr_from_operator(const ed_sector* init, const ed_sector* fin, int op, in
     });
 }
 
+int ed_csr_from_position_operator(const ed_sector* s, int kind, const int* shifts, int n_shifts, ed_csr** out,
+                                  void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && out && (shifts || n_shifts == 0), ED_ERR_ARG, "null argument");
+        *out = build_csr(s, s, position_rule(s->L, kind, shifts, n_shifts), 0, 0, 0, 0,
+                         static_cast<hipStream_t>(stream));
+    });
+}
+
 int ed_csr_create(int64_t rows, int64_t cols, int row_major, int hermitian_triangle, int64_t nnz, int n_values,
                   const double* values, const int16_t* value_inds, const int64_t* minor, const int64_t* major,
                   ed_csr** out, void* stream) {

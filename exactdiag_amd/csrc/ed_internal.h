@@ -69,20 +69,29 @@ inline void dev_free(void* p) {
 // Combination list of one rule: reversed site pairs (or single sites) with
 // (diagonal, off-diagonal) weights, as combinations_from_start_end_points
 // returns them (general/matrix_rules_utils.pyx:15-87).
+constexpr int RULE_W = 4;  // sites per combination slot (arity <= 4)
 struct RuleHost {
     int arity = 2;
-    std::vector<int8_t> idx;     // n_combo * 2
+    std::vector<int8_t> idx;     // n_combo * RULE_W (unused slots 0)
     std::vector<double> w;       // n_combo * 4: diag re, im, off re, im
     int valid = 0, make = 0, spin_proj = 0, density = 0, fermion = 0;
+    int singlet = 0;             // get_weights_annihilation_part_singlet_singlet
     int max_per_col = 1;
-    int n_combo() const { return int(idx.size() / 2); }
+    int n_combo() const { return int(idx.size() / RULE_W); }
 };
 
-enum RuleValid { RV_HOP = 0, RV_SWAP = 1, RV_NO_HOLES = 2, RV_ALL_UP = 3, RV_ALL_DOWN = 4, RV_ALL_HOLES = 5 };
-enum RuleMake { RM_SWAP = 0, RM_SAME = 1, RM_ADD_DOWN = 2, RM_ADD_UP = 3, RM_ADD_HOLE = 4 };
+enum RuleValid {
+    RV_HOP = 0, RV_SWAP = 1, RV_NO_HOLES = 2, RV_ALL_UP = 3, RV_ALL_DOWN = 4, RV_ALL_HOLES = 5,
+    RV_UNEQUAL_SPINS_2HOLES = 6
+};
+enum RuleMake { RM_SWAP = 0, RM_SAME = 1, RM_ADD_DOWN = 2, RM_ADD_UP = 3, RM_ADD_HOLE = 4, RM_REMOVE2_ADD_UP_DOWN = 5 };
 
 RuleHost hamiltonian_rule(int L, int term);
 RuleHost operator_rule(int L, int op, int qx, int qy);
+// Position-correlation operators (kind ED_POS_HOLES / ED_POS_SZ / ED_POS_SINGLET);
+// shifts: (leg, rung) pairs, the fixed distances followed by the free shift.
+RuleHost position_rule(int L, int kind, const int* shifts, int n_shifts);
+int shifted_site(int N, int i, int leg, int rung);
 
 }  // namespace ed
 

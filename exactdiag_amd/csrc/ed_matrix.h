@@ -30,5 +30,7 @@ void launch_reduce_partials(const double* partial, int64_t n, double* out, hipSt
 void launch_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, int triangle, int max_per_col,
                     int64_t lo, int64_t n, int32_t* d_count, int64_t* d_rows, double* d_vals, hipStream_t st);
 int column_capacity();
+void apply_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, const double* d_x, double* d_y,
+                   hipStream_t st);
 
 }  // namespace ed

@@ -28,7 +28,7 @@ namespace {
 constexpr int MAXC = 72;  // per-column capacity of the exact kernel
 
 struct RuleView {
-    int n_combo, arity, valid, make, spin_proj, density, fermion, max_per_col;
+    int n_combo, arity, valid, make, spin_proj, density, fermion, singlet, max_per_col;
     const int8_t* idx;
     const double* w;
 };
@@ -38,17 +38,33 @@ __device__ inline int site_value(uint32_t H, uint32_t Dn, int i) {  // 0 hole, 1
     return (H & b) ? 0 : ((Dn & b) ? 1 : 2);
 }
 
-__device__ inline bool rule_valid(const RuleView& r, uint32_t H, uint32_t Dn, int a, int b) {
+// Validity of a combination of r.arity sites c[0..arity)
+// (tJ_spin_half_ladder/matrix_element_functions.pyx:16-62).
+__device__ inline bool rule_valid(const RuleView& r, uint32_t H, uint32_t Dn, const int* c) {
     switch (r.valid) {
-        case RV_HOP: return site_value(H, Dn, a) == 0 && site_value(H, Dn, b) != 0;
+        case RV_HOP: return site_value(H, Dn, c[0]) == 0 && site_value(H, Dn, c[1]) != 0;
         case RV_SWAP: {
-            const int va = site_value(H, Dn, a), vb = site_value(H, Dn, b);
+            const int va = site_value(H, Dn, c[0]), vb = site_value(H, Dn, c[1]);
             return va != vb && va != 0 && vb != 0;
         }
-        case RV_NO_HOLES: return site_value(H, Dn, a) != 0 && (r.arity < 2 || site_value(H, Dn, b) != 0);
-        case RV_ALL_UP: return site_value(H, Dn, a) == 2 && (r.arity < 2 || site_value(H, Dn, b) == 2);
-        case RV_ALL_DOWN: return site_value(H, Dn, a) == 1 && (r.arity < 2 || site_value(H, Dn, b) == 1);
-        case RV_ALL_HOLES: return site_value(H, Dn, a) == 0 && (r.arity < 2 || site_value(H, Dn, b) == 0);
+        case RV_NO_HOLES:
+        case RV_ALL_UP:
+        case RV_ALL_DOWN:
+        case RV_ALL_HOLES: {
+            for (int k = 0; k < r.arity; ++k) {
+                const int v = site_value(H, Dn, c[k]);
+                if (r.valid == RV_NO_HOLES ? v == 0 : v != (r.valid == RV_ALL_UP ? 2 : r.valid == RV_ALL_DOWN ? 1 : 0))
+                    return false;
+            }
+            return true;
+        }
+        case RV_UNEQUAL_SPINS_2HOLES: {
+            if (c[0] == c[1]) return false;
+            const bool first = c[0] == c[2] || c[0] == c[3] || site_value(H, Dn, c[0]) == 0;
+            const bool second = c[1] == c[2] || c[1] == c[3] || site_value(H, Dn, c[1]) == 0;
+            const int v2 = site_value(H, Dn, c[2]), v3 = site_value(H, Dn, c[3]);
+            return first && second && v2 != v3 && v2 != 0 && v3 != 0;
+        }
     }
     return false;
 }
@@ -61,40 +77,33 @@ __device__ inline void set_site(uint32_t& H, uint32_t& Dn, int i, int v) {
 
 // general/column_functions.h:317-346: operators act from the last index to
 // the first; each counts the electrons above it plus the operators already
-// placed above it.
-__device__ inline int commutation_parity(uint32_t E, int arity, int a, int b) {
+// placed above it (a site placed twice counts twice).
+__device__ inline int commutation_parity(uint32_t E, int arity, const int* idx) {
     int count = 0;
-    uint32_t placed = 0;
-    const int idx[2] = {a, b};
     for (int k = arity - 1; k >= 0; --k) {
         const int p = idx[k];
         const uint32_t above = p >= 31 ? 0u : (~0u << (p + 1));
-        count += popc32(E & above) + popc32(placed & above);
-        placed |= 1u << p;
+        count += popc32(E & above);
+        for (int m = k + 1; m < arity; ++m) count += idx[m] > p;
     }
     return count & 1;
 }
 
 __device__ inline bool cabs_gt(C2 v, double thr2) { return v.re * v.re + v.im * v.im > thr2; }
 
-// Columns [lo, lo + n) of the initial sector; column lo + t is written at
-// slot t of count / rows / vals.
-__global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView fin, RuleView r, int triangle,
-                                                      int64_t lo, int64_t n_cols, int32_t* count, int64_t* rows,
-                                                      double* vals) {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n_cols) return;
-    const int64_t i = lo + t;
+// The elements of column i of the rule's matrix (rows of sector fin), merged
+// by row, near-zeros dropped, sorted by row: cv / ci[0..return value);
+// overflow is set when more than MAXC distinct rows were reached.
+__device__ int column_elements(const SectorView& ini, const SectorView& fin, const RuleView& r, int triangle,
+                               int64_t i, C2* cv, int64_t* ci, bool& overflow) {
     const uint64_t key = ini.key[i];
     const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
     const uint32_t E = ~H & ini.full;
     const int n = ini.orbit[i];
     const C2 dw0{1.0, 0.0};
     const C2 ow0{ini.sqrt_tab[n], 0.0};
-    C2 cv[MAXC];
-    int64_t ci[MAXC];
     int cnt = 0;
-    bool overflow = false;
+    overflow = false;
     auto add = [&](C2 v, int64_t j) {
         for (int k = 0; k < cnt; ++k)
             if (ci[k] == j) {
@@ -114,16 +123,22 @@ __global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView 
     int64_t same_j = -1;
     C2 same_c{0, 0};
     for (int c = 0; c < r.n_combo; ++c) {
-        const int a = r.idx[2 * c], b = r.idx[2 * c + 1];
+        int cs[RULE_W];
+        for (int k = 0; k < RULE_W; ++k) cs[k] = r.idx[RULE_W * c + k];
+        const int a = cs[0], b = cs[1];
         C2 dw = cmul(dw0, C2{r.w[4 * c], r.w[4 * c + 1]});
         C2 ow = cmul(ow0, C2{r.w[4 * c + 2], r.w[4 * c + 3]});
-        if (!rule_valid(r, H, Dn, a, b)) continue;
+        if (!rule_valid(r, H, Dn, cs)) continue;
         if (r.spin_proj) {  // tJ_spin_half_ladder/matrix_element_functions.pyx:71-82
             C2 f{1.0, 0.0};
-            if (site_value(H, Dn, a) == 1) f = cmul(f, C2{-1.0, 0.0});
-            if (r.arity > 1 && site_value(H, Dn, b) == 1) f = cmul(f, C2{-1.0, 0.0});
+            for (int k = 0; k < r.arity; ++k)
+                if (site_value(H, Dn, cs[k]) == 1) f = cmul(f, C2{-1.0, 0.0});
             dw = cmul(dw, f);
             ow = cmul(ow, f);
+        }
+        if (r.singlet && site_value(H, Dn, cs[2]) == 1) {  // matrix_element_functions.pyx:84-95
+            dw = cmul(dw, C2{-1.0, 0.0});
+            ow = cmul(ow, C2{-1.0, 0.0});
         }
         if (cabs_gt(dw, 1e-16)) add(dw, i);
         if (ow.re * ow.re + ow.im * ow.im < 1e-16) continue;
@@ -154,6 +169,11 @@ __global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView 
                     set_site(H2, D2, a, 0);
                     if (r.arity > 1) set_site(H2, D2, b, 0);
                     break;
+                case RM_REMOVE2_ADD_UP_DOWN: {  // matrix_element_functions.pyx:125-137, last index first
+                    const int changes[4] = {0, 0, 2, 1};
+                    for (int k = 0; k < r.arity; ++k) set_site(H2, D2, cs[r.arity - 1 - k], changes[k & 3]);
+                    break;
+                }
             }
             const OrbitResult o = orbit_walk(fin, H2, D2, false);
             j = -1;
@@ -171,7 +191,7 @@ __global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView 
         }
         if (j < 0 || (triangle && j < i)) continue;
         C2 v = cmul(ow, cc);
-        if (r.fermion && commutation_parity(E, r.arity, a, b)) v = C2{v.re * -1.0, v.im * -1.0};
+        if (r.fermion && commutation_parity(E, r.arity, cs)) v = C2{v.re * -1.0, v.im * -1.0};
         add(v, j);
     }
     // drop near-zero entries, sort by row (general/column_functions.h:79-97)
@@ -194,6 +214,20 @@ __global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView 
         ci[q + 1] = ki;
         cv[q + 1] = kv;
     }
+    return m;
+}
+
+// Columns [lo, lo + n) of the initial sector; column lo + t is written at
+// slot t of count / rows / vals.
+__global__ void __launch_bounds__(128) column_kernel(SectorView ini, SectorView fin, RuleView r, int triangle,
+                                                      int64_t lo, int64_t n_cols, int32_t* count, int64_t* rows,
+                                                      double* vals) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_cols) return;
+    C2 cv[MAXC];
+    int64_t ci[MAXC];
+    bool overflow;
+    const int m = column_elements(ini, fin, r, triangle, lo + t, cv, ci, overflow);
     count[t] = overflow ? MAXC + 1 : m;
     const int keep = m < r.max_per_col ? m : r.max_per_col;
     for (int k = 0; k < keep; ++k) {
@@ -576,7 +610,7 @@ __global__ void diag_op_gather_kernel(SectorView ini, SectorView fin, RuleView r
     C2 acc{0.0, 0.0};
     bool first = true;
     for (int c = 0; c < r.n_combo; ++c) {
-        const int a = r.idx[2 * c];
+        const int a = r.idx[RULE_W * c];
         C2 ow = cmul(ow0, C2{r.w[4 * c + 2], r.w[4 * c + 3]});
         const int va = site_value(H, Dn, a);
         if (va == 0) continue;
@@ -601,12 +635,93 @@ __global__ void diag_op_gather_kernel(SectorView ini, SectorView fin, RuleView r
     y[2 * j + 1] = o.im;
 }
 
+// <psi|O|psi> = sum_i sum_j conj(psi_j) O_ji psi_i, one column per lane in a
+// grid-stride loop; per-workgroup partials (re, im) in a fixed order.
+__global__ void __launch_bounds__(128) expect_kernel(SectorView s, RuleView r, const double* __restrict__ psi,
+                                                     double* partial) {
+    __shared__ double red[2 * 128];
+    double re = 0.0, im = 0.0;
+    C2 cv[MAXC];
+    int64_t ci[MAXC];
+    for (int64_t i = int64_t(blockIdx.x) * 128 + threadIdx.x; i < s.D; i += int64_t(gridDim.x) * 128) {
+        bool overflow;
+        const int m = column_elements(s, s, r, 0, i, cv, ci, overflow);
+        const C2 xi{psi[2 * i], psi[2 * i + 1]};
+        for (int k = 0; k < m; ++k) {
+            const C2 xj{psi[2 * ci[k]], -psi[2 * ci[k] + 1]};
+            const C2 v = cmul(cmul(xj, cv[k]), xi);
+            re += v.re;
+            im += v.im;
+        }
+    }
+    red[2 * threadIdx.x] = re;
+    red[2 * threadIdx.x + 1] = im;
+    __syncthreads();
+    for (int st = 64; st > 0; st >>= 1) {
+        if (int(threadIdx.x) < st) {
+            red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
+            red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = red[0];
+        partial[2 * blockIdx.x + 1] = red[1];
+    }
+}
+
+// calculate_hole_spin_projection_correlations (position_correlations.pyx:84-127,
+// :186-230): for every node index d, sum over dense states of |psi|^2 times
+// sum over reference sites r of the value at positions(d, r).  masks[d * N + r]
+// = (all positions, positions listed an odd number of times).  Holes: 1 when
+// every position holds a hole.  S^z: 0 when any position holds a hole, else
+// (-1)^(down spins at the odd-listed positions).  NP >= N, a multiple of 8.
+template <int NP>
+__global__ void __launch_bounds__(256) position_corr_kernel(SectorView s, int kind, const uint2* __restrict__ masks,
+                                                            int N, const double* __restrict__ psi, double* partial) {
+    __shared__ uint2 mk[NP * NP];
+    __shared__ double red[256];
+    for (int k = threadIdx.x; k < N * N; k += 256) mk[k] = masks[k];
+    __syncthreads();
+    double acc[NP];
+#pragma unroll
+    for (int d = 0; d < NP; ++d) acc[d] = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < s.D; i += int64_t(gridDim.x) * 256) {
+        const uint64_t key = s.key[i];
+        const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
+        const double p = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+#pragma unroll
+        for (int d = 0; d < NP; ++d) {
+            if (d >= N) break;
+            int v = 0;
+            for (int r = 0; r < N; ++r) {
+                const uint2 m = mk[d * N + r];
+                if (kind == ED_POS_HOLES) v += (H & m.x) == m.x;
+                else if (!(H & m.x)) v += (popc32(Dn & m.y) & 1) ? -1 : 1;
+            }
+            acc[d] += p * double(v);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < NP; ++d) {
+        if (d >= N) break;
+        red[threadIdx.x] = acc[d];
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (int(threadIdx.x) < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[int64_t(blockIdx.x) * N + d] = red[0];
+        __syncthreads();
+    }
+}
+
 __global__ void scatter_columns_kernel(int64_t D, int max_per_col, const int32_t* count, const int64_t* rows,
                                        const double* vals, const double* __restrict__ x, double* y) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= D) return;
     const C2 xi{x[2 * i], x[2 * i + 1]};
-    const int c = count[i];
+    const int c = count[i] < max_per_col ? count[i] : max_per_col;  // a column beyond capacity is cut
     for (int k = 0; k < c; ++k) {
         const int64_t e = i * max_per_col + k;
         const C2 v = cmul(xi, C2{vals[2 * e], vals[2 * e + 1]});
@@ -658,6 +773,7 @@ struct RuleDev {
         v.spin_proj = h.spin_proj;
         v.density = h.density;
         v.fermion = h.fermion;
+        v.singlet = h.singlet;
         v.max_per_col = h.max_per_col;
         v.idx = d_idx;
         v.w = d_w;
@@ -693,6 +809,34 @@ void launch_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& 
 }
 
 int column_capacity() { return MAXC; }
+
+// y = O x from the rule's columns: padded columns, then a scatter of x_i times
+// column i (atomic adds into y).
+void apply_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, const double* d_x, double* d_y,
+                   hipStream_t st) {
+    const int T = 256;
+    const int mpc = std::min(rule.max_per_col, MAXC);
+    int32_t* cnt = dev_alloc<int32_t>(size_t(std::max<int64_t>(ini->D, 1)));
+    int64_t* rows = dev_alloc<int64_t>(size_t(std::max<int64_t>(ini->D, 1) * mpc));
+    double* vals = dev_alloc<double>(size_t(std::max<int64_t>(ini->D, 1) * mpc * 2));
+    try {
+        launch_columns(ini, fin, rule, 0, mpc, 0, ini->D, cnt, rows, vals, st);
+        hipLaunchKernelGGL(zero_kernel, dim3(unsigned((2 * fin->D + T - 1) / T)), dim3(T), 0, st, d_y, 2 * fin->D);
+        if (ini->D > 0)
+            hipLaunchKernelGGL(scatter_columns_kernel, dim3(unsigned((ini->D + T - 1) / T)), dim3(T), 0, st, ini->D,
+                               mpc, cnt, rows, vals, d_x, d_y);
+        ED_HIP(hipGetLastError());
+        ED_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        dev_free(cnt);
+        dev_free(rows);
+        dev_free(vals);
+        throw;
+    }
+    dev_free(cnt);
+    dev_free(rows);
+    dev_free(vals);
+}
 
 // Launch helper shared with the Lanczos driver; returns the number of
 // workgroup partials written.  ED_HV_KERNEL selects a kernel for
@@ -801,28 +945,111 @@ int ed_op_apply(const ed_sector* ini, const ed_sector* fin, int op, int qx, int 
             ED_HIP(hipStreamSynchronize(st));
             return;
         }
-        const int mpc = rule.max_per_col;
-        int32_t* cnt = dev_alloc<int32_t>(size_t(std::max<int64_t>(ini->D, 1)));
-        int64_t* rows = dev_alloc<int64_t>(size_t(std::max<int64_t>(ini->D, 1) * mpc));
-        double* vals = dev_alloc<double>(size_t(std::max<int64_t>(ini->D, 1) * mpc * 2));
+        apply_columns(ini, fin, rule, d_x, d_y, st);
+    });
+}
+
+int ed_position_op_columns(const ed_sector* s, int kind, const int* shifts, int n_shifts, int max_per_col,
+                           int32_t* d_count, int64_t* d_rows, double* d_vals, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && (shifts || n_shifts == 0), ED_ERR_ARG, "null argument");
+        run_columns(s, s, position_rule(s->L, kind, shifts, n_shifts), 0, max_per_col, d_count, d_rows, d_vals,
+                    static_cast<hipStream_t>(stream));
+    });
+}
+
+int ed_position_op_apply(const ed_sector* s, int kind, const int* shifts, int n_shifts, const double* d_x,
+                         double* d_y, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
+        ED_REQUIRE(shifts || n_shifts == 0, ED_ERR_ARG, "null shifts");
+        if (s->D == 0) return;
+        apply_columns(s, s, position_rule(s->L, kind, shifts, n_shifts), d_x, d_y, static_cast<hipStream_t>(stream));
+    });
+}
+
+int ed_position_expectation(const ed_sector* s, int kind, const int* shifts, int n_shifts, const double* d_psi,
+                            double* out, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
+        ED_REQUIRE(out && (shifts || n_shifts == 0), ED_ERR_ARG, "null argument");
+        out[0] = out[1] = 0.0;
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const RuleHost rule = position_rule(s->L, kind, shifts, n_shifts);
+        ED_REQUIRE(rule.n_combo() + 1 <= MAXC, ED_ERR_CAPACITY, "too many combinations per column");
+        RuleDev rd(rule, st);
+        const int64_t nwg = std::min<int64_t>((s->D + 127) / 128, 4096);
+        double* partial = dev_alloc<double>(size_t(2 * nwg + 2));
         try {
-            run_columns(ini, fin, rule, 0, mpc, cnt, rows, vals, st);
-            hipLaunchKernelGGL(zero_kernel, dim3(unsigned((2 * fin->D + T - 1) / T)), dim3(T), 0, st, d_y,
-                               2 * fin->D);
-            if (ini->D > 0)
-                hipLaunchKernelGGL(scatter_columns_kernel, dim3(unsigned((ini->D + T - 1) / T)), dim3(T), 0, st,
-                                   ini->D, mpc, cnt, rows, vals, d_x, d_y);
-            ED_HIP(hipGetLastError());
+            hipLaunchKernelGGL(expect_kernel, dim3(unsigned(nwg)), dim3(128), 0, st, s->view(), rd.v, d_psi, partial);
+            launch_reduce_partials(partial, nwg, partial + 2 * nwg, st);
+            ED_HIP(hipMemcpyAsync(out, partial + 2 * nwg, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
             ED_HIP(hipStreamSynchronize(st));
         } catch (...) {
-            dev_free(cnt);
-            dev_free(rows);
-            dev_free(vals);
+            dev_free(partial);
             throw;
         }
-        dev_free(cnt);
-        dev_free(rows);
-        dev_free(vals);
+        dev_free(partial);
+    });
+}
+
+int ed_position_correlations(const ed_sector* s, int kind, const int* fixed, int n_fixed, const double* d_psi,
+                             double* out, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
+        ED_REQUIRE(out && (fixed || n_fixed == 0), ED_ERR_ARG, "null argument");
+        ED_REQUIRE(kind == ED_POS_HOLES || kind == ED_POS_SZ, ED_ERR_ARG, "kind must be holes or S^z");
+        ED_REQUIRE(n_fixed >= 0 && n_fixed <= 8, ED_ERR_ARG, "at most 8 fixed distances");
+        const int N = s->N;
+        // position lists (r, r + unit shifts of d, r + each fixed distance) as bit masks
+        std::vector<uint32_t> masks(size_t(2 * N * N));
+        for (int d = 0; d < N; ++d)
+            for (int r = 0; r < N; ++r) {
+                uint32_t all = 0, odd = 0;
+                auto put = [&](int q) {
+                    all |= 1u << q;
+                    odd ^= 1u << q;
+                };
+                put(r);
+                put(shifted_site(N, r, d / 2, d % 2));
+                for (int k = 0; k < n_fixed; ++k) put(shifted_site(N, r, fixed[2 * k], fixed[2 * k + 1]));
+                masks[2 * (size_t(d) * N + r)] = all;
+                masks[2 * (size_t(d) * N + r) + 1] = odd;
+            }
+        for (int d = 0; d < N; ++d) out[d] = 0.0;
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const int64_t nwg = std::min<int64_t>((s->D + 255) / 256, 2048);
+        uint32_t* d_masks = dev_upload(masks, st);
+        double* partial = dev_alloc<double>(size_t(nwg * N));
+        try {
+            const auto* mk = reinterpret_cast<const uint2*>(d_masks);
+            if (N <= 8)
+                hipLaunchKernelGGL(position_corr_kernel<8>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind, mk,
+                                   N, d_psi, partial);
+            else if (N <= 16)
+                hipLaunchKernelGGL(position_corr_kernel<16>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind,
+                                   mk, N, d_psi, partial);
+            else if (N <= 24)
+                hipLaunchKernelGGL(position_corr_kernel<24>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind,
+                                   mk, N, d_psi, partial);
+            else
+                hipLaunchKernelGGL(position_corr_kernel<32>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind,
+                                   mk, N, d_psi, partial);
+            ED_HIP(hipGetLastError());
+            std::vector<double> h(size_t(nwg * N));
+            ED_HIP(hipMemcpyAsync(h.data(), partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+            for (int64_t b = 0; b < nwg; ++b)
+                for (int d = 0; d < N; ++d) out[d] += h[size_t(b * N + d)];
+        } catch (...) {
+            dev_free(d_masks);
+            dev_free(partial);
+            throw;
+        }
+        dev_free(d_masks);
+        dev_free(partial);
     });
 }
 

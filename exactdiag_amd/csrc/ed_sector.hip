@@ -192,6 +192,7 @@ int shifted_index(int N, int i, int s0, int s1) {  // symmetry_ladder.h:96-101
 void add_combo(RuleHost& r, int a, int b, std::complex<double> w0, std::complex<double> w1) {
     r.idx.push_back(int8_t(a));
     r.idx.push_back(int8_t(b));
+    for (int k = 2; k < RULE_W; ++k) r.idx.push_back(0);
     r.w.push_back(w0.real());
     r.w.push_back(w0.imag());
     r.w.push_back(w1.real());
@@ -301,6 +302,57 @@ RuleHost operator_rule(int L, int op, int qx, int qy) {
         r.max_per_col = 2 * N;
     } else {
         throw Fail{ED_ERR_ARG, "unknown operator id"};
+    }
+    return r;
+}
+
+int shifted_site(int N, int i, int leg, int rung) { return shifted_index(N, i, leg, rung); }
+
+// get_position_correlation_operator (tJ_spin_half_ladder/matrix_setup.pyx:409-480):
+// combinations start at every site (combinations_from_start_end_points with
+// reverse=False).  Holes / S^z: (first, first + each shift), weights (1, 0),
+// diagonal only.  Singlet-singlet, shifts = (y, x, j) for
+// Delta_i^dagger(y) Delta_{i+j}(x): end points (y, j, x + j) and
+// (-y, j - y, x + j - y), weights (0, 0.5 / N).
+RuleHost position_rule(int L, int kind, const int* sh, int n_shifts) {
+    const int N = 2 * L;
+    RuleHost r;
+    ED_REQUIRE(n_shifts >= 0 && (kind != ED_POS_SINGLET || n_shifts == 3), ED_ERR_ARG,
+               "singlet-singlet needs two fixed distances and the free shift");
+    ED_REQUIRE(kind == ED_POS_SINGLET || n_shifts + 1 <= RULE_W, ED_ERR_ARG,
+               "at most three shifts for hole or S^z correlations");
+    ED_REQUIRE(kind >= ED_POS_HOLES && kind <= ED_POS_SINGLET, ED_ERR_ARG, "unknown position-correlation kind");
+    auto push = [&](const int* c, int n, std::complex<double> w0, std::complex<double> w1) {
+        for (int k = 0; k < RULE_W; ++k) r.idx.push_back(int8_t(k < n ? c[k] : 0));
+        r.w.insert(r.w.end(), {w0.real(), w0.imag(), w1.real(), w1.imag()});
+    };
+    if (kind == ED_POS_SINGLET) {
+        const int y0 = sh[0], y1 = sh[1], x0 = sh[2], x1 = sh[3], j0 = sh[4], j1 = sh[5];
+        const int ends[2][3][2] = {{{y0, y1}, {j0, j1}, {x0 + j0, x1 + j1}},
+                                   {{-y0, -y1}, {j0 - y0, j1 - y1}, {x0 + j0 - y0, x1 + j1 - y1}}};
+        for (int i = 0; i < N; ++i)
+            for (const auto& e : ends) {
+                int c[4] = {i, 0, 0, 0};
+                for (int k = 0; k < 3; ++k) c[k + 1] = shifted_index(N, i, e[k][0], e[k][1]);
+                push(c, 4, 0.0, 0.5 / N);
+            }
+        r.arity = 4;
+        r.valid = RV_UNEQUAL_SPINS_2HOLES;
+        r.make = RM_REMOVE2_ADD_UP_DOWN;
+        r.fermion = 1;
+        r.singlet = 1;
+        r.max_per_col = r.n_combo() + 1;
+    } else {
+        for (int i = 0; i < N; ++i) {
+            int c[RULE_W] = {i, 0, 0, 0};
+            for (int k = 0; k < n_shifts; ++k) c[k + 1] = shifted_index(N, i, sh[2 * k], sh[2 * k + 1]);
+            push(c, n_shifts + 1, 1.0, 0.0);
+        }
+        r.arity = n_shifts + 1;
+        r.valid = kind == ED_POS_HOLES ? RV_ALL_HOLES : RV_NO_HOLES;
+        r.make = RM_SAME;
+        r.spin_proj = kind == ED_POS_SZ;
+        r.max_per_col = 1;
     }
     return r;
 }

@@ -182,6 +182,48 @@ class Sector:
                                    ct.byref(ms), _stream()))
         return t.value, g.value, ms.value
 
+    # -- position correlations (tJ_spin_half_ladder/position_correlations.pyx) --------
+    @staticmethod
+    def _shifts(shifts):
+        sh = np.ascontiguousarray(np.asarray(shifts, np.int32).reshape(-1, 2))
+        return sh, len(sh)
+
+    def position_columns(self, kind: int, shifts, max_per_col: int | None = None):
+        """Columns of a position-correlation operator (kind 0 holes, 1 S^z, 2 singlet-singlet); shifts =
+        the fixed distances then the free shift, as (leg, rung) pairs."""
+        sh, n = self._shifts(shifts)
+        mpc = max_per_col or (2 * self.N + 2)
+        return self._columns(self._lib.ed_position_op_columns, (self._h, int(kind), sh.ctypes.data, n), mpc,
+                             self.dim)
+
+    def position_apply(self, kind: int, shifts, x):
+        torch = _torch()
+        sh, n = self._shifts(shifts)
+        x = x.to(torch.complex128).contiguous()
+        y = torch.empty(max(self.dim, 1), dtype=torch.complex128, device=self.device)[: self.dim]
+        check(self._lib.ed_position_op_apply(self._h, int(kind), sh.ctypes.data, n, _ptr(x), _ptr(y), _stream()))
+        return y
+
+    def position_expectation(self, kind: int, shifts, psi) -> complex:
+        """<psi|O|psi> of a position-correlation operator (fused: no matrix, no O psi vector)."""
+        torch = _torch()
+        sh, n = self._shifts(shifts)
+        psi = psi.to(torch.complex128).contiguous()
+        out = np.zeros(2, np.float64)
+        check(self._lib.ed_position_expectation(self._h, int(kind), sh.ctypes.data, n, _ptr(psi),
+                                                out.ctypes.data_as(ct.POINTER(ct.c_double)), _stream()))
+        return complex(out[0], out[1])
+
+    def position_correlations(self, kind: int, fixed, psi) -> np.ndarray:
+        """Hole (kind 0) or S^z (kind 1) correlations of psi at every node index (2L values)."""
+        torch = _torch()
+        fx, n = self._shifts(fixed)
+        psi = psi.to(torch.complex128).contiguous()
+        out = np.zeros(self.N, np.float64)
+        check(self._lib.ed_position_correlations(self._h, int(kind), fx.ctypes.data, n, _ptr(psi),
+                                                 out.ctypes.data_as(ct.POINTER(ct.c_double)), _stream()))
+        return out
+
     def op_apply(self, final: "Sector", op: int, qx: int, qy: int, x):
         torch = _torch()
         x = x.to(torch.complex128).contiguous()

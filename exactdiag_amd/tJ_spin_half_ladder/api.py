@@ -1,8 +1,7 @@
 """Top-level entry points (reference: tJ_spin_half_ladder/api.py).
 
 Same functions and return shapes as the reference.  Plotting
-(api.py:202-209) and position correlations (api.py:104-113) are outside the
-GPU hot path and not provided.
+(api.py:202-209) is outside the GPU path and not provided.
 """
 
 from __future__ import annotations
@@ -69,6 +68,21 @@ def get_excitation_spectrum(config, limited_qs: bool = True) -> Spectrum:
         ws, spectrum, info = get_spectral_function_spectra(config=config, limited_qs=limited_qs)
     else:
         raise ValueError(f"{base_name} is not supported")
+    return Spectrum(ws, spectrum, config, info)
+
+
+def get_position_correlations(config) -> Spectrum:
+    """Hole, S^z or singlet-singlet position correlations (api.py:104-113): ws = varied shifts."""
+    from exactdiag_amd.tJ_spin_half_ladder import position_correlations
+
+    base_name = str(config.correlations.name)
+    if base_name in {"hole_correlations", "Sz_correlations"}:
+        ws, spectrum = position_correlations.get_hole_spin_projection_correlations(config)
+    elif base_name == "singlet-singlet":
+        ws, spectrum = position_correlations.get_singlet_singlet_correlations(config)
+    else:
+        raise ValueError(f"{base_name} is not supported")
+    info = {"fixed_distances": config.correlations.fixed_distances}
     return Spectrum(ws, spectrum, config, info)
 
 

@@ -287,3 +287,124 @@ def _get_spectrum_figure_suffix(hamiltonian_config, spectrum_config, operator_na
     broad_string = f"_broad{spectrum_config.broadening}"
     vecs_string = f"_vecs{spectrum_config.num_lanczos_vecs}"
     return f"{operator_name_suffix}{q_string}{ws_string}{broad_string}{vecs_string}_{weight_string}"
+
+
+class Position_Correlation_Name(str, enum.Enum):
+    """Names of the position correlations (configs.py:236-241)."""
+
+    HOLE_CORRELATIONS = "hole_correlations"
+    SZ_CORRELATIONS = "Sz_correlations"
+    SINGLET_SINGLET = "singlet-singlet"
+
+    def __str__(self):
+        return self.value
+
+
+@dataclasses.dataclass
+class Position_Shift(gc._Validated):
+    """A relative node position (leg shift, rung shift), configs.py:244-247 with general/configs.py:47-52."""
+
+    leg: int
+    rung: int
+
+    @staticmethod
+    def _convert_leg(v):
+        return int(v)
+
+    @staticmethod
+    def _convert_rung(v):
+        v = int(v)
+        if v not in (0, 1):
+            raise ValueError("rung shift must be 0 or 1")
+        return v
+
+    def to_npint32(self):
+        return np.array([self.leg, self.rung], dtype=np.int32)
+
+
+def _as_shift(v):
+    if isinstance(v, Position_Shift):
+        return v
+    gc._check_fields(Position_Shift, v)
+    return Position_Shift(**v)
+
+
+@dataclasses.dataclass
+class Position_Correlation_Part(gc._Validated):
+    """Operator part of a position correlation (configs.py:250-274): name, fixed relative shifts between the
+    operator terms, threads (None: copied from the Hamiltonian config)."""
+
+    name: typing.Any
+    fixed_distances: typing.Any
+    num_threads: typing.Optional[int] = None
+
+    @staticmethod
+    def _convert_name(v):
+        return v if isinstance(v, Position_Correlation_Name) else Position_Correlation_Name(v)
+
+    @staticmethod
+    def _convert_fixed_distances(v):
+        return [_as_shift(e) for e in v]
+
+
+def _as_correlations(v):
+    if isinstance(v, Position_Correlation_Part):
+        return v
+    gc._check_fields(Position_Correlation_Part, v)
+    return Position_Correlation_Part(**v)
+
+
+@dataclasses.dataclass
+class Limited_Position_Correlation_Config(gc._Validated):
+    """Hamiltonian and position-correlation settings (configs.py:277-309)."""
+
+    hamiltonian: typing.Any
+    correlations: typing.Any
+
+    _convert_hamiltonian = staticmethod(_as_hamiltonian)
+    _convert_correlations = staticmethod(_as_correlations)
+
+    def __post_init__(self):
+        if self.correlations.num_threads is None:
+            self.correlations.num_threads = self.hamiltonian.num_threads
+
+    def setup_excitation_operator(self, free_shift):
+        """(get_operator, Hamiltonian config, get_excited_H) of the operator at relative position free_shift."""
+        from exactdiag_amd.tJ_spin_half_ladder import matrix_setup
+
+        return matrix_setup.get_position_correlation_operator(self, _as_shift(free_shift))
+
+
+@dataclasses.dataclass
+class Full_Position_Correlation_Config(gc._Validated):
+    """Hamiltonian, eigenpair and position-correlation settings (configs.py:312-331)."""
+
+    hamiltonian: typing.Any
+    eigenpair: typing.Any
+    correlations: typing.Any
+
+    _convert_hamiltonian = staticmethod(_as_hamiltonian)
+    _convert_eigenpair = staticmethod(gc.as_eigenpair_part)
+    _convert_correlations = staticmethod(_as_correlations)
+
+    def __post_init__(self):
+        if self.correlations.num_threads is None:
+            self.correlations.num_threads = self.hamiltonian.num_threads
+        if self.eigenpair.num_threads is None:
+            self.eigenpair.num_threads = self.hamiltonian.num_threads
+
+    get_eigenpair_paths = Eigenpair_Config.get_eigenpair_paths
+    setup_excitation_operator = Limited_Position_Correlation_Config.setup_excitation_operator
+
+    def get_spectrum_path(self, operator_name_suffix=""):
+        folder = self.hamiltonian.get_calc_folder() / f"{self.correlations.name}_spectra"
+        suffix = _get_position_correlation_figure_suffix(self.hamiltonian, self.correlations, operator_name_suffix)
+        return folder / f"{self.correlations.name}{suffix}.npz"
+
+
+def _get_position_correlation_figure_suffix(hamiltonian_config, correlation_config, operator_name_suffix):
+    """configs.py:382-389."""
+    weight_string = "_".join(f"{n}{w}" for n, w in dataclasses.asdict(hamiltonian_config.weights).items())
+    operator_name_suffix = f"_{operator_name_suffix}" if operator_name_suffix else ""
+    fixed_string = f"{[shift.to_npint32().tolist() for shift in correlation_config.fixed_distances]}"
+    return f"{operator_name_suffix}_fixed{fixed_string}_{weight_string}"

@@ -11,6 +11,7 @@ from __future__ import annotations
 import copy
 import functools
 import logging
+import pathlib
 
 import numpy as np
 
@@ -63,6 +64,15 @@ class Symmetries:
         s1 = int(relative_shift[1])
         rung = (i + int(np.fmod(s1, 2))) % 2
         return (2 * leg + rung) % N
+
+    def get_unit_shifts_from_index(self, index):
+        """(leg, rung) shifts of a node index (symmetry_ladder.h:103-108)."""
+        index = int(index) % (2 * self.num_rungs)
+        return np.array([index // 2, index % 2], dtype=np.int32)
+
+    def get_index_by_relative_index(self, initial_index, relative_index):
+        """initial_index moved by the shifts that take node 0 to relative_index (symmetry_generator.h:36-45)."""
+        return self.get_index_by_relative_shift(initial_index, self.get_unit_shifts_from_index(relative_index))
 
 
 class Translator:
@@ -363,3 +373,59 @@ def get_excitation_operator_matrix(initial_config, overwrite=False):
         return Sparse_Matrix.from_device(m, sig)
     finally:
         m.close()
+
+
+# ---------------------------------------------------------------------------
+# Position-correlation operators (matrix_setup.pyx:409-520).
+
+POSITION_KINDS = {
+    configs.Position_Correlation_Name.HOLE_CORRELATIONS: _lib.POS_HOLES,
+    configs.Position_Correlation_Name.SZ_CORRELATIONS: _lib.POS_SZ,
+    configs.Position_Correlation_Name.SINGLET_SINGLET: _lib.POS_SINGLET,
+}
+
+
+def position_operator_signature(config, free_shift):
+    """The Signature get_position_correlation_operator gives the operator (matrix_setup.pyx:492-508)."""
+    from exactdiag_amd.general.sparse_matrices import FILE_NAMES, Signature
+
+    shifts = np.array([s.to_npint32() for s in config.correlations.fixed_distances] + [free_shift.to_npint32()],
+                      dtype=np.int32)
+    full_name = f"{config.correlations.name}_shifts{shifts.tolist()}"
+    name = configs.Position_Correlation_Name(config.correlations.name)
+    n_spins = 2 * config.hamiltonian.num_rungs - config.hamiltonian.num_holes
+    D = get_sector(config.hamiltonian).dim
+    return Signature(matrix_name=full_name, shape=(D, D), symmetry_strings=("sorted",),
+                     max_values_per_column=4 * n_spins if name == configs.Position_Correlation_Name.SINGLET_SINGLET
+                     else 1,
+                     major="column", initial_system_info=config.hamiltonian.get_symmetry_info(),
+                     final_system_info=None, folder_name=str(config.hamiltonian.get_calc_folder() / full_name),
+                     **FILE_NAMES), shifts
+
+
+def get_position_correlation_operator(config, free_shift):
+    """(get_operator, Hamiltonian config, get_excited_H) as the reference returns them.
+
+    get_operator() opens the operator's folder <calc folder>/<name>_shifts[...] when it is saved there,
+    otherwise builds the operator on the GPU and saves it, as the reference does; the returned matrix is
+    applied on the GPU.  (The correlations themselves do not go through the folder:
+    position_correlations.py contracts the operator with the state directly.)
+    """
+    from exactdiag_amd.csr import StoredCSR
+    from exactdiag_amd.general.sparse_matrices import FILE_NAMES, Sparse_Matrix
+
+    name = configs.Position_Correlation_Name(config.correlations.name)
+    kind = POSITION_KINDS[name]
+
+    def get_operator():
+        sig, shifts = position_operator_signature(config, free_shift)
+        if (pathlib.Path(sig.folder_name) / FILE_NAMES["info_name"]).exists():
+            return Sparse_Matrix(sig)
+        m = StoredCSR.from_position_operator(get_sector(config.hamiltonian), kind, shifts)
+        try:
+            return Sparse_Matrix.from_device(m, sig)
+        finally:
+            m.close()
+
+    get_excited_H = config.hamiltonian.setup()
+    return get_operator, config.hamiltonian, get_excited_H

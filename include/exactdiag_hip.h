@@ -42,6 +42,13 @@ enum ed_operator {
     ED_OP_CURRENT_LEG = 7   /* matrix_setup.pyx:265-279 */
 };
 
+/* Position-correlation kinds (tJ_spin_half_ladder/configs.py:236-241). */
+enum ed_position_kind {
+    ED_POS_HOLES = 0,   /* "hole_correlations" */
+    ED_POS_SZ = 1,      /* "Sz_correlations" */
+    ED_POS_SINGLET = 2  /* "singlet-singlet" */
+};
+
 const char* ed_last_error(void);
 int ed_version(void);
 
@@ -86,6 +93,35 @@ int ed_term_columns(const ed_sector* s, int term, int triangle, int max_per_col,
  * (setup_excitation_operator, matrix_setup.pyx:195-374). */
 int ed_op_columns(const ed_sector* init, const ed_sector* fin, int op, int qx, int qy, int max_per_col,
                   int32_t* d_count, int64_t* d_rows, double* d_vals, void* stream);
+
+/* Position-correlation operators of one sector
+ * (get_position_correlation_operator, tJ_spin_half_ladder/matrix_setup.pyx:409-480):
+ * kind ED_POS_HOLES / ED_POS_SZ project on holes / multiply S^z signs at
+ * (r, r + each shift) summed over r (diagonal); ED_POS_SINGLET is
+ * (0.5 / N) sum_i Delta_i^dagger(y) Delta_{i+j}(x).  shifts: n_shifts
+ * (leg, rung) int pairs in host memory, the config's fixed distances then
+ * the free shift (singlet: y, x, j).  ed_position_op_columns as
+ * ed_op_columns; ed_position_op_apply: d_y = O d_x (c128); 
+ * ed_position_expectation: out[0..1] = <psi|O|psi> (host), as
+ * calculate_singlet_singlet_correlations takes np.vdot(eigvec,
+ * operator.dot(eigvec)) (tJ_spin_half_ladder/position_correlations.pyx:129-142). */
+int ed_position_op_columns(const ed_sector* s, int kind, const int* shifts, int n_shifts, int max_per_col,
+                           int32_t* d_count, int64_t* d_rows, double* d_vals, void* stream);
+int ed_position_op_apply(const ed_sector* s, int kind, const int* shifts, int n_shifts, const double* d_x,
+                         double* d_y, void* stream);
+int ed_position_expectation(const ed_sector* s, int kind, const int* shifts, int n_shifts, const double* d_psi,
+                            double* out, void* stream);
+
+/* Hole or S^z position correlations of a state (c128 device vector), one
+ * value per node index d in out[0..2L) (host): sum over basis states of
+ * |psi|^2 times the number of reference sites r whose positions (r, r + the
+ * unit shifts of d, r + each fixed distance) all hold holes (kind
+ * ED_POS_HOLES), or the product of the S^z signs there (ED_POS_SZ, 0 with a
+ * hole).  Replaces calculate_hole_spin_projection_correlations and
+ * calculate_position_correlations
+ * (tJ_spin_half_ladder/position_correlations.pyx:84-127, :186-230). */
+int ed_position_correlations(const ed_sector* s, int kind, const int* fixed, int n_fixed, const double* d_psi,
+                             double* out, void* stream);
 
 /* Matrix-free y = sum_T w_T H_T x over the four terms (w: 4 complex
  * weights as 8 doubles, host memory).  Optional d_dot (>= 1 c128 on device):
@@ -181,6 +217,10 @@ typedef struct ed_csr ed_csr;
 int ed_csr_from_sector(const ed_sector* s, int term, int triangle, int row_major, ed_csr** out, void* stream);
 int ed_csr_from_operator(const ed_sector* init, const ed_sector* fin, int op, int qx, int qy, ed_csr** out,
                          void* stream);
+/* A position-correlation operator (see ed_position_op_columns) stored as
+ * get_position_correlation_operator writes it: column-major, sorted. */
+int ed_csr_from_position_operator(const ed_sector* s, int kind, const int* shifts, int n_shifts, ed_csr** out,
+                                  void* stream);
 int ed_csr_create(int64_t rows, int64_t cols, int row_major, int hermitian_triangle, int64_t nnz, int n_values,
                   const double* values, const int16_t* value_inds, const int64_t* minor, const int64_t* major,
                   ed_csr** out, void* stream);

@@ -315,8 +315,8 @@ struct Sector {
 // Operator rules (general/column_functions.h, tJ_spin_half_ladder/
 // matrix_element_functions.pyx, tJ_spin_half_ladder/matrix_setup.pyx).
 
-enum Valid { V_HOP, V_SWAP, V_NO_HOLES, V_ALL_UP, V_ALL_DOWN, V_ALL_HOLES };
-enum NewState { N_SWAP, N_SAME, N_ADD_DOWN, N_ADD_UP, N_ADD_HOLE };
+enum Valid { V_HOP, V_SWAP, V_NO_HOLES, V_ALL_UP, V_ALL_DOWN, V_ALL_HOLES, V_UNEQUAL_SPINS_2HOLES };
+enum NewState { N_SWAP, N_SAME, N_ADD_DOWN, N_ADD_UP, N_ADD_HOLE, N_REMOVE2_ADD_UP_DOWN };
 
 struct Rule {
     std::vector<std::vector<int>> combos;  // reversed index lists
@@ -326,6 +326,7 @@ struct Rule {
     bool fermion_sign;  // count_commutations over non-hole sites
     bool spin_projection_weight;  // get_weights_spin_projection
     bool density_weight = false;  // N(q): no reference counterpart
+    bool singlet_weight = false;  // get_weights_annihilation_part_singlet_singlet
 };
 
 // tJ_spin_half_ladder/symmetry_ladder.h:96-101
@@ -460,6 +461,49 @@ Rule operator_rule(int L, int op, int qx, int qy) {
     return r;
 }
 
+// Position-correlation operators (tJ_spin_half_ladder/matrix_setup.pyx:409-480):
+// kind 0 hole projection, 1 S^z product, 2 singlet-singlet.  `sh` holds the
+// config's fixed distances followed by the free shift, as (leg, rung) pairs;
+// combinations start at every site (combinations_from_start_end_points,
+// general/matrix_rules_utils.pyx:15-87, reverse=False).
+Rule position_rule(int L, int kind, const std::vector<std::pair<int, int>>& sh) {
+    const int N = 2 * L;
+    Rule r;
+    r.fermion_sign = false;
+    r.spin_projection_weight = false;
+    if (kind == 2) {
+        if (sh.size() != 3) throw std::runtime_error("singlet-singlet needs two fixed distances and a free shift");
+        const auto f0 = sh[0], f1 = sh[1], f2 = sh[2];
+        const std::pair<int, int> ends[2][3] = {
+            {f0, f2, {f1.first + f2.first, f1.second + f2.second}},
+            {{-f0.first, -f0.second},
+             {f2.first - f0.first, f2.second - f0.second},
+             {f1.first + f2.first - f0.first, f1.second + f2.second - f0.second}}};
+        for (int i = 0; i < N; ++i)
+            for (const auto& e : ends) {
+                std::vector<int> c = {i};
+                for (const auto& s : e) c.push_back(shifted_index(N, i, s.first, s.second));
+                r.combos.push_back(c);
+                r.weights.push_back({0.0, 0.5 / N});
+            }
+        r.valid = V_UNEQUAL_SPINS_2HOLES;
+        r.make = N_REMOVE2_ADD_UP_DOWN;
+        r.fermion_sign = true;
+        r.singlet_weight = true;
+    } else {
+        for (int i = 0; i < N; ++i) {
+            std::vector<int> c = {i};
+            for (const auto& s : sh) c.push_back(shifted_index(N, i, s.first, s.second));
+            r.combos.push_back(c);
+            r.weights.push_back({1.0, 0.0});
+        }
+        r.valid = kind == 0 ? V_ALL_HOLES : V_NO_HOLES;
+        r.make = N_SAME;
+        r.spin_projection_weight = kind == 1;
+    }
+    return r;
+}
+
 bool rule_valid(const Rule& r, const state_t* s, const std::vector<int>& c) {
     // tJ_spin_half_ladder/matrix_element_functions.pyx:16-51
     switch (r.valid) {
@@ -477,6 +521,12 @@ bool rule_valid(const Rule& r, const state_t* s, const std::vector<int>& c) {
         case V_ALL_HOLES:
             for (int i : c) if (s[i] != HOLE) return false;
             return true;
+        case V_UNEQUAL_SPINS_2HOLES: {  // matrix_element_functions.pyx:53-62
+            if (c[0] == c[1]) return false;
+            const bool first = c[0] == c[2] || c[0] == c[3] || s[c[0]] == HOLE;
+            const bool second = c[1] == c[2] || c[1] == c[3] || s[c[1]] == HOLE;
+            return first && second && s[c[2]] != s[c[3]] && s[c[2]] != HOLE && s[c[3]] != HOLE;
+        }
     }
     return false;
 }
@@ -530,6 +580,10 @@ void column(const Sector& init, const Sector& fin, const Rule& r, bool triangle,
             dw *= f;
             ow *= f;
         }
+        if (r.singlet_weight && st[cb[2]] == DOWN) {  // matrix_element_functions.pyx:84-95
+            dw *= -1;
+            ow *= -1;
+        }
         if (std::abs(dw) > 1e-8) add_result(dw, i, vals, inds);
         if (std::abs(ow) < 1e-8) continue;
         ns = st;
@@ -542,6 +596,12 @@ void column(const Sector& init, const Sector& fin, const Rule& r, bool triangle,
             case N_ADD_DOWN: for (int p : cb) ns[p] = DOWN; break;
             case N_ADD_UP: for (int p : cb) ns[p] = UP; break;
             case N_ADD_HOLE: for (int p : cb) ns[p] = HOLE; break;
+            case N_REMOVE2_ADD_UP_DOWN: {  // matrix_element_functions.pyx:125-137, last index first
+                const state_t changes[4] = {HOLE, HOLE, UP, DOWN};
+                const int len = int(cb.size());
+                for (int k = 0; k < len; ++k) ns[cb[len - 1 - k]] = changes[k % 4];
+                break;
+            }
         }
         int64_t sp;
         cplx amp;
@@ -778,6 +838,99 @@ void orc_op_weights(int L, int op, int qx, int qy, double* out) {
         out[4 * c + 1] = r.weights[c].first.imag();
         out[4 * c + 2] = r.weights[c].second.real();
         out[4 * c + 3] = r.weights[c].second.imag();
+    }
+}
+
+// Position-correlation operator columns on one sector (same layout as orc_op_columns).
+int64_t orc_position_columns(void* p, int kind, const int* shifts, int n_shifts, int threads, int64_t* ptr,
+                             int64_t* rows, double* vals) {
+    try {
+        auto& s = *static_cast<Handle*>(p)->sec;
+        std::vector<std::pair<int, int>> sh;
+        for (int k = 0; k < n_shifts; ++k) sh.push_back({shifts[2 * k], shifts[2 * k + 1]});
+        Rule r = position_rule(s.L, kind, sh);
+        Columns c = all_columns(s, s, r, false, threads);
+        if (rows) {
+            std::copy(c.ptr.begin(), c.ptr.end(), ptr);
+            std::copy(c.rows.begin(), c.rows.end(), rows);
+            std::memcpy(vals, c.vals.data(), c.vals.size() * sizeof(cplx));
+        }
+        return int64_t(c.rows.size());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// calculate_hole_spin_projection_correlations / calculate_position_correlations
+// (tJ_spin_half_ladder/position_correlations.pyx:84-127, :186-230): out[d] for
+// every node index d (N values).  kind 0: holes (aggregated per hole pattern,
+// Hole_Index_Aggregator), 1: S^z products (each dense state).
+int orc_position_correlations(void* p, int kind, const int* fixed, int n_fixed, const double* psi, double* out) {
+    try {
+        auto& s = *static_cast<Handle*>(p)->sec;
+        const int N = s.N;
+        const int64_t D = s.dim();
+        const cplx* v = reinterpret_cast<const cplx*>(psi);
+        std::vector<state_t> st(N);
+        for (int d = 0; d < N; ++d) {
+            // positions[r] = (r, r shifted by the unit shifts of d, r shifted by each fixed distance)
+            std::vector<std::vector<int>> pos(N);
+            for (int r = 0; r < N; ++r) {
+                pos[r] = {r, shifted_index(N, r, d / 2, d % 2)};
+                for (int k = 0; k < n_fixed; ++k) pos[r].push_back(shifted_index(N, r, fixed[2 * k], fixed[2 * k + 1]));
+            }
+            double corr = 0;
+            int64_t i = 0;
+            while (i < D) {
+                const int64_t sparse = s.dense_to_sparse[i];
+                cplx amp;
+                int64_t next;
+                if (kind == 0) {
+                    const int64_t hole = sparse / s.n_spin;
+                    const int64_t top = std::min<int64_t>(D, i + s.n_spin);
+                    double value = 0;
+                    next = top;
+                    for (int64_t k = i; k < top; ++k) {
+                        if (s.dense_to_sparse[k] / s.n_spin != hole) {
+                            next = k;
+                            break;
+                        }
+                        value += std::norm(v[k]);
+                    }
+                    amp = std::sqrt(cplx(value));
+                } else {
+                    amp = v[i];
+                    next = i + 1;
+                }
+                s.unrank(sparse, st.data());
+                const double an = std::norm(amp);
+                for (int r = 0; r < N; ++r) {
+                    int val = 1;
+                    for (int q : pos[r]) {
+                        if (kind == 0) {
+                            if (st[q] != HOLE) {
+                                val = 0;
+                                break;
+                            }
+                        } else {
+                            if (st[q] == HOLE) {
+                                val = 0;
+                                break;
+                            }
+                            if (st[q] == DOWN) val *= -1;
+                        }
+                    }
+                    corr += an * val;
+                }
+                i = next;
+            }
+            out[d] = corr;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
     }
 }
 

@@ -75,6 +75,9 @@ def lib():
         L.orc_term_csr_sizes.argtypes = [ct.c_void_p, ct.c_int, _i64p, _i64p]
         L.orc_term_csr_copy.argtypes = [ct.c_void_p, ct.c_int, _dp, _i16p, _i64p, _i64p]
         L.orc_build_layout_csr.argtypes = [ct.c_void_p] + [ct.c_int] * 4
+        L.orc_position_columns.restype = ct.c_int64
+        L.orc_position_columns.argtypes = [ct.c_void_p, ct.c_int, _i32p, ct.c_int, ct.c_int, _i64p, _i64p, _dp]
+        L.orc_position_correlations.argtypes = [ct.c_void_p, ct.c_int, _i32p, ct.c_int, _dp, _dp]
         L.orc_build_op_csr.argtypes = [ct.c_void_p, ct.c_void_p] + [ct.c_int] * 4
         L.orc_hv.argtypes = [ct.c_void_p, _dp, _dp, _dp, ct.c_int]
         L.orc_cf.argtypes = [_dp, _dp, ct.c_int, _dp, ct.c_int, _dp]
@@ -190,6 +193,31 @@ class OracleSector:
         self._lib.orc_op_columns(self._h, final._h, op, qx, qy, t, _ptr(ptr, _i64p), _ptr(rows, _i64p),
                                  _ptr(vals.view(np.float64), _dp))
         return ptr, rows[:nnz], vals[:nnz]
+
+    def position_columns(self, kind, shifts, threads=None):
+        """Columns of a position-correlation operator (kind 0 holes, 1 S^z, 2 singlet-singlet); shifts =
+        fixed distances then the free shift, as (leg, rung) pairs."""
+        sh = np.ascontiguousarray(np.asarray(shifts, np.int32).reshape(-1, 2))
+        t = threads or default_threads()
+        nnz = self._lib.orc_position_columns(self._h, kind, _ptr(sh, _i32p), len(sh), t, None, None, None)
+        if nnz < 0:
+            raise ValueError(self._lib.orc_last_error().decode())
+        ptr = np.empty(self.dim + 1, np.int64)
+        rows = np.empty(max(nnz, 1), np.int64)
+        vals = np.empty(max(nnz, 1), np.complex128)
+        self._lib.orc_position_columns(self._h, kind, _ptr(sh, _i32p), len(sh), t, _ptr(ptr, _i64p),
+                                       _ptr(rows, _i64p), _ptr(vals.view(np.float64), _dp))
+        return ptr, rows[:nnz], vals[:nnz]
+
+    def position_correlations(self, kind, fixed, psi):
+        """calculate_hole_spin_projection_correlations: one value per node index (kind 0 holes, 1 S^z)."""
+        fx = np.ascontiguousarray(np.asarray(fixed, np.int32).reshape(-1, 2))
+        psi = np.ascontiguousarray(psi, np.complex128)
+        out = np.empty(2 * self.L, np.float64)
+        if self._lib.orc_position_correlations(self._h, kind, _ptr(fx, _i32p), len(fx),
+                                               _ptr(psi.view(np.float64), _dp), _ptr(out, _dp)) != 0:
+            raise ValueError(self._lib.orc_last_error().decode())
+        return out
 
     def build_csr(self, terms=(0, 1, 2, 3), threads=None):
         for t in terms:

@@ -185,3 +185,117 @@ def test_operators(case):
     # the reference's q.r uses single-precision division (~1e-7 phase error)
     assert np.max(np.abs(M - Os)) < 2e-6, case
     assert np.max(np.abs(Os)) > 0.05
+
+
+def _site(L, i, leg, rung):
+    """get_index_by_relative_shift on the ladder (symmetry_ladder.h:103-111 with symmetry_generator.h:47-59)."""
+    return 2 * ((i // 2 + leg) % L) + (i % 2 + rung) % 2
+
+
+def _apply(st, ops):
+    """Apply a product of fermion operators (rightmost first) to a Fock state: (sign, new state) or None.
+    ops: list of (site, spin, create) with spin 1 down, 2 up."""
+    st = list(st)
+    sign = 1
+    for site, spin, create in reversed(ops):
+        if create:
+            if st[site] != 0:
+                return None
+            sign *= ann_sign(st, site)
+            st[site] = spin
+        else:
+            if st[site] != spin:
+                return None
+            sign *= ann_sign(st, site)
+            st[site] = 0
+    return sign, tuple(st)
+
+
+def _fock_singlet(fk, y, x, j):
+    """(0.5 / N) sum_i Delta_i^dagger(y) Delta_{i+j}(x), Delta_i(x) = c_{i up} c_{i+x down} - c_{i down} c_{i+x up},
+    from explicit fermion operators."""
+    L, N = fk.L, fk.N
+    dim = len(fk.states)
+    O = np.zeros((dim, dim), np.complex128)
+    for c, st in enumerate(fk.states):
+        for i in range(N):
+            iy = _site(L, i, *y)
+            a = _site(L, i, *j)
+            ax = _site(L, a, *x)
+            creations = [(+1, [(iy, 1, True), (i, 2, True)]), (-1, [(iy, 2, True), (i, 1, True)])]
+            annihilations = [(+1, [(a, 2, False), (ax, 1, False)]), (-1, [(a, 1, False), (ax, 2, False)])]
+            for s1, cr in creations:
+                for s2, an in annihilations:
+                    res = _apply(st, cr + an)
+                    if res is None:
+                        continue
+                    sg, new = res
+                    O[fk.index[new], c] += 0.5 / N * s1 * s2 * sg
+    return O
+
+
+def _fock_diagonal(fk, kind, shifts):
+    """Hole projection (kind 0) or S^z product (kind 1) over positions (r, r + shifts...) summed over r."""
+    L, N = fk.L, fk.N
+    d = np.zeros(len(fk.states))
+    for c, st in enumerate(fk.states):
+        for r in range(N):
+            pos = [r] + [_site(L, r, *s) for s in shifts]
+            if kind == 0:
+                d[c] += all(st[p] == 0 for p in pos)
+            elif all(st[p] != 0 for p in pos):
+                d[c] += np.prod([-1 if st[p] == 1 else 1 for p in pos])
+    return np.diag(d).astype(np.complex128)
+
+
+POSITION_CASES = [
+    ((3, 2, 2, 0, 0), 2, [(1, 1), (1, 1), (1, 0)]),
+    ((4, 2, 3, 0, 0), 2, [(1, 0), (1, 1), (2, 1)]),
+    ((4, 2, 3, 1, 1), 2, [(0, 1), (1, 0), (1, 1)]),
+    ((4, 2, 3, 2, 0), 2, [(1, 1), (0, 1), (0, 0)]),
+    ((3, 3, 1, 1, 1), 2, [(0, 1), (0, 1), (2, 1)]),
+    ((4, 2, 3, 0, 1), 0, [(1, 1)]),
+    ((4, 3, 2, 1, 0), 0, [(1, 0), (2, 1)]),
+    ((4, 2, 3, 1, 0), 1, [(0, 1)]),
+    ((4, 1, 3, 2, 1), 1, [(1, 1), (0, 0)]),
+]
+
+
+@pytest.mark.parametrize("case", POSITION_CASES)
+def test_position_operators(case):
+    """get_position_correlation_operator (matrix_setup.pyx:409-480) against its second-quantised definition."""
+    sec, kind, shifts = case
+    fk = Fock(sec[0], sec[1], sec[2])
+    _, B = fk.symmetric_basis(sec[3], sec[4])
+    if kind == 2:
+        Of = _fock_singlet(fk, shifts[0], shifts[1], shifts[2])
+    else:
+        Of = _fock_diagonal(fk, kind, shifts)
+    Os = B.conj().T @ Of @ B
+    o = OracleSector(*sec)
+    ptr, rows, vals = o.position_columns(kind, shifts)
+    M = np.zeros((o.dim, o.dim), np.complex128)
+    for i in range(o.dim):
+        for k in range(ptr[i], ptr[i + 1]):
+            M[rows[k], i] += vals[k]
+    assert np.max(np.abs(M - Os)) < 1e-12, case
+    assert np.max(np.abs(Os)) > 1e-3, case
+
+
+@pytest.mark.parametrize("sec,kind,fixed", [((4, 2, 3, 0, 0), 0, []), ((4, 3, 2, 1, 1), 0, [(1, 1)]),
+                                            ((4, 1, 3, 2, 0), 1, []), ((3, 2, 2, 1, 1), 1, [(0, 1)])])
+def test_position_correlations(sec, kind, fixed):
+    """calculate_hole_spin_projection_correlations (position_correlations.pyx:84-127, :186-230) on a random
+    symmetric state = <Psi| sum_r P_r(d) |Psi> from the Fock-space diagonal operator, for every node index d."""
+    fk = Fock(sec[0], sec[1], sec[2])
+    _, B = fk.symmetric_basis(sec[3], sec[4])
+    o = OracleSector(*sec)
+    rng = np.random.default_rng(5)
+    psi = rng.normal(size=o.dim) + 1j * rng.normal(size=o.dim)
+    psi /= np.linalg.norm(psi)
+    got = o.position_correlations(kind, fixed, psi)
+    Psi = B @ psi
+    for d in range(2 * sec[0]):
+        Od = _fock_diagonal(fk, kind, [(d // 2, d % 2)] + list(fixed))
+        ref = np.real(np.vdot(Psi, Od @ Psi))
+        assert abs(got[d] - ref) < 1e-12, (sec, kind, d)
