@@ -207,6 +207,37 @@ def cpu_csr_build(mats, threads):
                        f"[0, {rows}) of each of the four terms (D = {o.dim}), {sum(t_terms):.1f}s{scaled}")}, same
 
 
+def cpu_positions(states, res, threads):
+    """The oracle's restatement of calculate_hole_spin_projection_correlations on the same 2x12 ground state
+    (single-threaded, as the reference's loop), and of the singlet operator's columns for one shift."""
+    import numpy as np
+
+    from oracle import oracle
+
+    s, x = states["holes"]
+    o = oracle.OracleSector(12, 2, 11, 0, 0, threads=threads)
+    t0 = time.perf_counter()
+    corr = o.position_correlations(0, [], x)
+    t_holes = time.perf_counter() - t0
+    got = np.asarray(res["holes_2x12"]["correlations"])
+    same = bool(np.max(np.abs(corr / corr[0] - got)) < 1e-10)
+    log(f"cpu positions: holes 2x12 in {t_holes:.1f}s; equal to the GPU to 1e-10: {same}")
+    s4, x4 = states["singlet"]
+    o4 = oracle.OracleSector(10, 4, 8, 0, 0, threads=threads)
+    t0 = time.perf_counter()
+    ptr, rows, vals = o4.position_columns(2, [(1, 1), (1, 1), (0, 0)], threads=threads)
+    cols = np.repeat(np.arange(o4.dim), np.diff(ptr))
+    e = np.sum(np.conj(x4[rows]) * vals * x4[cols]).real
+    t_singlet = time.perf_counter() - t0
+    same_s = bool(abs(e - res["singlet_2x10_h4"]["correlations"][0]) < 1e-10)
+    log(f"cpu positions: singlet 2x10 h4 shift 0 in {t_singlet:.1f}s; equal: {same_s}")
+    return ({"value": t_holes * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+             "sample": (f"oracle calculate_hole_spin_projection_correlations on the same 2x12 ground state, all 24 "
+                        f"distances, single-threaded ({t_holes:.1f}s); singlet-singlet of the 2x10 h4 state, one "
+                        f"shift: {t_singlet:.1f}s on {threads} threads (GPU: "
+                        f"{res['singlet_2x10_h4']['per_shift_ms']:.2f} ms)")}, same and same_s)
+
+
 def run_form(form, args=None):
     import torch
 
@@ -215,7 +246,21 @@ def run_form(form, args=None):
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    if form == "csr":
+    if form == "positions":
+        res, states = forms.positions_form(log=log)
+        h = res["holes_2x12"]
+        line = {"metric": "hole correlations of the 2x12 ground state, all 24 distances (ms per call)",
+                "value": h["kernel_ms"], "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+                "dtype": "complex128", "data": "synthetic (model built from its definition)",
+                "config": {"workload": "2x12 ladder, 2 holes, Sz=0, k=(0,0), t=1, J=0.4 ground state (D=8112468); "
+                                       "singlet-singlet of the 2x10, 4-hole ground state"},
+                "roofline": {"bound": "hbm", "achieved": h["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": h["gbs"] / HBM_PEAK_GBS, "traffic": None},
+                "form": res}
+        if args is None or not args.no_cpu_baseline:
+            threads = min(16, os.cpu_count() or 1)
+            line["cpu_baseline"], line["matches_oracle"] = cpu_positions(states, res, threads)
+    elif form == "csr":
         res, mats = forms.csr_form(L, HOLES, NDOWN, (KX, KY), WEIGHTS, log=log)
         line = {"metric": "GPU build of the reference's stored CSR matrices (four Hamiltonian terms), seconds",
                 "value": res["build_s"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
@@ -279,8 +324,8 @@ def main():
     ap.add_argument("--no-allq", action="store_true", help="skip the all-q time-to-accuracy section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy one vector (known bytes) for PMC calibration")
-    ap.add_argument("--form", choices=["hv", "batching", "precision", "csr"], default="hv",
-                    help="hv: the headline H·v + all-q line; batching / precision / csr: their own line")
+    ap.add_argument("--form", choices=["hv", "batching", "precision", "csr", "positions"], default="hv",
+                    help="hv: the headline H·v + all-q line; batching / precision / csr / positions: their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "

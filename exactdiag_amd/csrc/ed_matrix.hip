@@ -670,41 +670,67 @@ __global__ void __launch_bounds__(128) expect_kernel(SectorView s, RuleView r, c
     }
 }
 
+// Shifts of a position list for the correlation kernel: (legs, rungs) with
+// legs in [0, L), rungs in {0, 1}.
+struct PosShifts {
+    int n;
+    int leg[8], rung[8];
+};
+
+// Bit r of the result = bit (r moved by `leg` legs and `rung` rungs) of X
+// (get_index_by_relative_shift on the ladder, symmetry_ladder.h:103-111).
+__device__ inline uint32_t moved_mask(uint32_t X, int leg, int rung, int N, uint32_t full) {
+    if (rung) X = leg_swap(X);
+    const int s = 2 * leg;
+    return s ? rotl_n(X, N - s, N, full) : X;
+}
+
 // calculate_hole_spin_projection_correlations (position_correlations.pyx:84-127,
-// :186-230): for every node index d, sum over dense states of |psi|^2 times
-// sum over reference sites r of the value at positions(d, r).  masks[d * N + r]
-// = (all positions, positions listed an odd number of times).  Holes: 1 when
-// every position holds a hole.  S^z: 0 when any position holds a hole, else
-// (-1)^(down spins at the odd-listed positions).  NP >= N, a multiple of 8.
+// :186-230) in bit-parallel form: for every node index d, the sum over dense
+// states of |psi|^2 times the sum over reference sites r of the value at the
+// positions (r, r + unit shifts of d, r + each fixed distance).  With X_k the
+// state's masks moved by the k-th shift, a whole sum over r is a popcount:
+// holes: popc(AND_k H_k); S^z: the product over the positions is +1 or -1 where
+// every position holds an electron (A = AND_k E_k) and the parity of the down
+// spins is P = XOR_k Dn_k (a position listed twice cancels), so the sum is
+// popc(A) - 2 popc(A & P).  NP >= N, a multiple of 8.
 template <int NP>
-__global__ void __launch_bounds__(256) position_corr_kernel(SectorView s, int kind, const uint2* __restrict__ masks,
-                                                            int N, const double* __restrict__ psi, double* partial) {
-    __shared__ uint2 mk[NP * NP];
+__global__ void __launch_bounds__(256) position_corr_kernel(SectorView s, int kind, PosShifts fixed,
+                                                            const double* __restrict__ psi, double* partial) {
     __shared__ double red[256];
-    for (int k = threadIdx.x; k < N * N; k += 256) mk[k] = masks[k];
-    __syncthreads();
+    const int N = s.N;
     double acc[NP];
 #pragma unroll
     for (int d = 0; d < NP; ++d) acc[d] = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < s.D; i += int64_t(gridDim.x) * 256) {
         const uint64_t key = s.key[i];
         const uint32_t H = uint32_t(key >> 32), Dn = uint32_t(key);
+        const uint32_t E = ~H & s.full;
         const double p = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        uint32_t FA = (kind == ED_POS_HOLES ? H : E), FP = (kind == ED_POS_HOLES ? 0u : Dn);
+        for (int k = 0; k < fixed.n; ++k) {
+            FA &= moved_mask(kind == ED_POS_HOLES ? H : E, fixed.leg[k], fixed.rung[k], N, s.full);
+            if (kind != ED_POS_HOLES) FP ^= moved_mask(Dn, fixed.leg[k], fixed.rung[k], N, s.full);
+        }
+        // fully unrolled with d a constant, so that acc stays in registers
 #pragma unroll
         for (int d = 0; d < NP; ++d) {
-            if (d >= N) break;
-            int v = 0;
-            for (int r = 0; r < N; ++r) {
-                const uint2 m = mk[d * N + r];
-                if (kind == ED_POS_HOLES) v += (H & m.x) == m.x;
-                else if (!(H & m.x)) v += (popc32(Dn & m.y) & 1) ? -1 : 1;
+            if (d < N) {
+                int v;
+                if (kind == ED_POS_HOLES) {
+                    v = popc32(FA & moved_mask(H, d >> 1, d & 1, N, s.full));
+                } else {
+                    const uint32_t A = FA & moved_mask(E, d >> 1, d & 1, N, s.full);
+                    const uint32_t P = FP ^ moved_mask(Dn, d >> 1, d & 1, N, s.full);
+                    v = popc32(A) - 2 * popc32(A & P);
+                }
+                acc[d] += p * double(v);
             }
-            acc[d] += p * double(v);
         }
     }
 #pragma unroll
     for (int d = 0; d < NP; ++d) {
-        if (d >= N) break;
+        if (d >= N) continue;  // uniform: N is the same for every lane
         red[threadIdx.x] = acc[d];
         __syncthreads();
         for (int st = 128; st > 0; st >>= 1) {
@@ -1002,41 +1028,30 @@ int ed_position_correlations(const ed_sector* s, int kind, const int* fixed, int
         ED_REQUIRE(kind == ED_POS_HOLES || kind == ED_POS_SZ, ED_ERR_ARG, "kind must be holes or S^z");
         ED_REQUIRE(n_fixed >= 0 && n_fixed <= 8, ED_ERR_ARG, "at most 8 fixed distances");
         const int N = s->N;
-        // position lists (r, r + unit shifts of d, r + each fixed distance) as bit masks
-        std::vector<uint32_t> masks(size_t(2 * N * N));
-        for (int d = 0; d < N; ++d)
-            for (int r = 0; r < N; ++r) {
-                uint32_t all = 0, odd = 0;
-                auto put = [&](int q) {
-                    all |= 1u << q;
-                    odd ^= 1u << q;
-                };
-                put(r);
-                put(shifted_site(N, r, d / 2, d % 2));
-                for (int k = 0; k < n_fixed; ++k) put(shifted_site(N, r, fixed[2 * k], fixed[2 * k + 1]));
-                masks[2 * (size_t(d) * N + r)] = all;
-                masks[2 * (size_t(d) * N + r) + 1] = odd;
-            }
+        PosShifts fx{};
+        fx.n = n_fixed;
+        for (int k = 0; k < n_fixed; ++k) {
+            fx.leg[k] = ((fixed[2 * k] % s->L) + s->L) % s->L;
+            fx.rung[k] = ((fixed[2 * k + 1] % 2) + 2) % 2;
+        }
         for (int d = 0; d < N; ++d) out[d] = 0.0;
         if (s->D == 0) return;
         hipStream_t st = static_cast<hipStream_t>(stream);
         const int64_t nwg = std::min<int64_t>((s->D + 255) / 256, 2048);
-        uint32_t* d_masks = dev_upload(masks, st);
         double* partial = dev_alloc<double>(size_t(nwg * N));
         try {
-            const auto* mk = reinterpret_cast<const uint2*>(d_masks);
             if (N <= 8)
-                hipLaunchKernelGGL(position_corr_kernel<8>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind, mk,
-                                   N, d_psi, partial);
+                hipLaunchKernelGGL(position_corr_kernel<8>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind, fx,
+                                   d_psi, partial);
             else if (N <= 16)
                 hipLaunchKernelGGL(position_corr_kernel<16>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind,
-                                   mk, N, d_psi, partial);
+                                   fx, d_psi, partial);
             else if (N <= 24)
                 hipLaunchKernelGGL(position_corr_kernel<24>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind,
-                                   mk, N, d_psi, partial);
+                                   fx, d_psi, partial);
             else
                 hipLaunchKernelGGL(position_corr_kernel<32>, dim3(unsigned(nwg)), dim3(256), 0, st, s->view(), kind,
-                                   mk, N, d_psi, partial);
+                                   fx, d_psi, partial);
             ED_HIP(hipGetLastError());
             std::vector<double> h(size_t(nwg * N));
             ED_HIP(hipMemcpyAsync(h.data(), partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1044,11 +1059,9 @@ int ed_position_correlations(const ed_sector* s, int kind, const int* fixed, int
             for (int64_t b = 0; b < nwg; ++b)
                 for (int d = 0; d < N; ++d) out[d] += h[size_t(b * N + d)];
         } catch (...) {
-            dev_free(d_masks);
             dev_free(partial);
             throw;
         }
-        dev_free(d_masks);
         dev_free(partial);
     });
 }

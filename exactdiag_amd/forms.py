@@ -284,3 +284,68 @@ def csr_form(L=12, h=2, nd=11, k=(0, 0), weights=(1.0, 1.0, 0.4, 0.4), reps=20, 
         f"rel. diff vs stored H {out['rel_diff_vs_stored_h']:.1e}")
     op.close()
     return out, mats
+
+
+def positions_form(reps=20, log=print):
+    """Position correlations of ground states (tJ_spin_half_ladder/position_correlations.pyx).
+
+    Hole correlations of the 2x12, 2-hole, k = (0,0) ground state (D = 8,112,468; J = 0.4, t = 1): one
+    reduction over the basis for all 24 node distances.  Singlet-singlet correlations of the 2x10, 4-hole,
+    k = (0,0) ground state (weights 0.45, 0.36, 0.15, 0.12, y = x = (1, 1)): one fused expectation value per
+    relative position of the singlets (20).  Ground states by two-pass Lanczos on the stored H.  Returns
+    (results, states) with the states on the host for a CPU comparison.
+    """
+    import torch
+
+    from exactdiag_amd import _lib
+
+    out, states = {}, {}
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        _sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        _sync()
+        return r, e0.elapsed_time(e1) / reps, (time.perf_counter() - t0) / reps
+
+    # hole correlations, 2x12
+    s = Sector(12, 2, 11, 0, 0)
+    hm = s.store(weights_array([1.0, 1.0, 0.4, 0.4]))
+    t0 = time.perf_counter()
+    e0, x, _, res = two_pass_ground_state(hm, tol=1e-9)
+    _sync()
+    t_gs = time.perf_counter() - t0
+    corr, ms, wall = timed(lambda: s.position_correlations(_lib.POS_HOLES, np.zeros((0, 2), np.int32), x))
+    out["holes_2x12"] = {"dim": s.dim, "e0": e0, "ground_state_s": t_gs, "residual": res, "kernel_ms": ms,
+                         "call_ms": wall * 1e3, "correlations": (corr / corr[0]).tolist(),
+                         "sum_rule": float(corr.sum()), "bytes_per_call": 24 * s.dim,
+                         "gbs": 24 * s.dim / ms / 1e6}
+    log(f"  holes 2x12: D={s.dim}, E0={e0:.10f}, kernel {ms:.3f} ms ({24 * s.dim / ms / 1e6:.0f} GB/s)")
+    states["holes"] = (s, x.cpu().numpy())
+    hm.close()
+    # singlet-singlet, 2x10 with 4 holes
+    s4 = Sector(10, 4, 8, 0, 0)
+    hm = s4.store(weights_array([0.45, 0.36, 0.15, 0.12]))
+    t0 = time.perf_counter()
+    e0, x, _, res = two_pass_ground_state(hm, tol=1e-9)
+    _sync()
+    t_gs = time.perf_counter() - t0
+    fixed = np.array([[1, 1], [1, 1]], np.int32)
+
+    def all_shifts():
+        return [s4.position_expectation(_lib.POS_SINGLET, np.vstack([fixed, [[d // 2, d % 2]]]), x).real
+                for d in range(s4.N)]
+
+    corr, ms, wall = timed(all_shifts)
+    out["singlet_2x10_h4"] = {"dim": s4.dim, "e0": e0, "ground_state_s": t_gs, "residual": res,
+                              "all_shifts_ms": ms, "per_shift_ms": ms / s4.N, "correlations": corr}
+    log(f"  singlet 2x10 h4: D={s4.dim}, E0={e0:.10f}, {s4.N} expectation values in {ms:.2f} ms")
+    states["singlet"] = (s4, x.cpu().numpy())
+    hm.close()
+    return out, states
