@@ -10,7 +10,7 @@ synthetic vectors already resident in HBM.
 The timed H·v is the stored form (sliced-ELL H built once on the GPU before
 the timed region, as the reference builds its CSR matrices once before its
 Lanczos loop); the matrix-free kernel is timed too and reported under
-``matrix_free``.  Prints one JSON line with the driver's fields plus
+``matrix_free``.  Prints one JSON line (metric, value, unit, steps, timing and config fields) plus
 ``roofline`` (the H·v kernel against the 8 TB/s HBM peak, timed with HIP
 events on the launch stream) and ``cpu_baseline`` (the reference's own
 sparse-dot kernels, from oracle/_ref, timed on the host's cores over a
@@ -328,8 +328,8 @@ def main():
                     help="hv: the headline H·v + all-q line; batching / precision / csr / positions: their own line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
-              f"this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); several GPUs are not supported: "
+              f"independent q values or sectors would be distributed over GPUs, one sector per GPU", file=sys.stderr)
         sys.exit(2)
     if args.form != "hv":
         run_form(args.form, args)

@@ -4,7 +4,8 @@
 // host C++, the reference package's CPU algorithms so that the HIP product
 // path can be checked against it.  Only tests/, __graft_entry__.smoke() and
 // bench.py's cpu_baseline leg may load the library built from it, and only as
-// the checker (or as the timed CPU baseline), never as the thing that ships.
+// the reference results to compare with (or as the timed CPU baseline), never as
+// the thing that ships.
 //
 // Every function names the reference file:line it follows (paths relative to
 // the reference package root src/exactdiag/).  Arithmetic is kept in the
