@@ -866,9 +866,10 @@ int ed_csr_sum_info(const ed_csr_sum* m, int64_t* rows, int64_t* cols, int64_t* 
 
 int ed_csr_sum_hv(const ed_csr_sum* m, const double* d_x, double* d_y, void* stream) {
     return guarded([&] {
-        ED_REQUIRE(m && d_x && d_y, ED_ERR_ARG, "null argument");
+        ED_REQUIRE(m, ED_ERR_ARG, "null operator");
+        if (m->rows == 0) return;  // empty vectors may come with null pointers
+        ED_REQUIRE(d_y && (d_x || m->cols == 0), ED_ERR_ARG, "null vector");
         hipStream_t st = static_cast<hipStream_t>(stream);
-        if (m->rows == 0) return;
         const unsigned grid = unsigned((m->rows + 255) / 256);
         const int entries = 2 * m->n_table;
         const auto* x = reinterpret_cast<const double2*>(d_x);

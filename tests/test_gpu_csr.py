@@ -295,3 +295,27 @@ def test_large_sector_prefix_and_table_invariants(oracle_mod, torch):
         assert np.all(pg[1:] >= pg[:-1]) and pg[-1] == len(mg)
         rows_of = np.repeat(np.arange(s.dim), np.diff(pg))
         assert np.all(mg >= rows_of)  # upper triangle
+
+
+@pytest.mark.parametrize("p", [(2, 2, 0, 0, 0), (3, 0, 0, 1, 0), (2, 4, 0, 0, 0), (3, 6, 0, 0, 0)])
+def test_empty_and_tiny_sectors(oracle_mod, torch, p):
+    """Sectors without states or with one state: the stored arrays are the oracle's (the empty-matrix marker
+    included), the summed product has the sector's size, and position correlations of any state are finite."""
+    from exactdiag_amd.csr import CSRSum, StoredCSR
+    from exactdiag_amd.sector import Sector
+
+    s, o = Sector(*p), oracle_mod.OracleSector(*p)
+    assert s.dim == o.dim
+    mats = []
+    for t in range(4):
+        m = StoredCSR.from_sector(s, t)
+        _assert_csr(m.arrays(), o.term_csr(t), (p, t))
+        mats.append(m)
+    op = CSRSum(mats, [1, 1, 0.4, 0.4])
+    x = torch.ones(s.dim, dtype=torch.complex128, device="cuda")
+    y = op.hv(x)
+    assert y.shape == (s.dim,)
+    if s.dim:
+        np.testing.assert_allclose(y.cpu().numpy(), o.hv([1, 1, 0.4, 0.4], x.cpu().numpy()), atol=1e-13)
+        corr = s.position_correlations(0, np.zeros((0, 2), np.int32), x / x.norm())
+        assert np.all(np.isfinite(corr))
