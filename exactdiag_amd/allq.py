@@ -261,3 +261,41 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
 
 def max_l1(spectra, ref, ws):
     return max(spectral_l1(spectra[q], ref[q], ws) for q in ref)
+
+
+def allq_batched(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL,
+                 terminator=True):
+    """All q of the spectrum as one batch: every final sector's Lanczos run is an instance of one batched,
+    graph-captured run over stored H (batch.spectra), FP64 vectors, closed by the square-root terminator as in
+    the fast form.  Returns (seconds, spectra, stage timings)."""
+    from exactdiag_amd.batch import Excitation, spectra
+    from exactdiag_amd.closure import sqrt_terminator
+    from exactdiag_amd.forms import two_pass_ground_state
+
+    tm = {}
+    _sync()
+    t0 = time.perf_counter()
+    s0 = Sector(L, h, nd, k0[0], k0[1])
+    hm0 = s0.store(weights)
+    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, tol=tol)
+    hm0.close()
+    _sync()
+    tm["ground_state"] = time.perf_counter() - t0
+    qs = distinct_qs(L, limited)
+    exc = [Excitation("Szq", _lib.OP_SZQ, qx, qy, Sector(L, h, nd, *final_momentum(L, k0[0], k0[1], qx, qy)))
+           for qx, qy in qs]
+    t1 = time.perf_counter()
+    _, coeffs = spectra(exc, s0, psi, e0, weights, ws, eta, nsteps, stored=True)
+    _sync()
+    tm["batched_lanczos"] = time.perf_counter() - t1
+    out = {}
+    for (qx, qy), ex, (a, b) in zip(qs, exc, coeffs):
+        if len(a) == 0:
+            out[(qx, qy)] = np.zeros(len(ws))
+            continue
+        at, bt, tail = sqrt_terminator(a, b, len(a)) if (terminator and len(a) == nsteps) else (a, b, None)
+        out[(qx, qy)] = cf_eval(at, bt, ws, e0, eta, -ex.norm2 / np.pi, tail=tail)
+        ex.sector.close()
+    s0.close()
+    _sync()
+    return time.perf_counter() - t0, out, tm
