@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -121,6 +122,65 @@ __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, 
     if (f >= s.D) return;
     FillSink sink{col, code, pack, shift, map, sptr[f >> 6] + (f & 63), 0};
     walk_row<4>(s, T, w, f, sink);
+}
+
+// Order each row's stored elements by column (ties by value index and sign),
+// so that the lanes of a wavefront reading slot k gather from nearby columns.
+// One wavefront per 64-row slice, one lane per row: a row of up to SORT_CAP
+// elements is copied into the lane's own LDS column (K[k][lane], no bank
+// conflicts), insertion-sorted there and written back; a longer row is
+// insertion-sorted in place in HBM.  The key is the packed word, or
+// (column << 16) | code for the two-array form.
+constexpr int SORT_CAP = 64;
+
+template <bool PACKED>
+__global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* __restrict__ sptr,
+                                                     const uint8_t* __restrict__ cnt, int32_t* col, uint16_t* code,
+                                                     uint32_t* pack) {
+    using Key = typename std::conditional<PACKED, uint32_t, uint64_t>::type;
+    __shared__ Key K[SORT_CAP][64];
+    const int lane = threadIdx.x;
+    const int64_t f = int64_t(blockIdx.x) * 64 + lane;
+    if (f >= D) return;
+    const int64_t base = sptr[f >> 6] + lane;
+    const int n = cnt[f];
+    auto key_at = [&](int k) -> Key {
+        const int64_t at = base + int64_t(k) * 64;
+        if constexpr (PACKED) return pack[at];
+        else return (uint64_t(uint32_t(col[at])) << 16) | code[at];
+    };
+    auto put = [&](int k, Key v) {
+        const int64_t at = base + int64_t(k) * 64;
+        if constexpr (PACKED) {
+            pack[at] = v;
+        } else {
+            col[at] = int32_t(uint32_t(v >> 16));
+            code[at] = uint16_t(v & 0xffffu);
+        }
+    };
+    if (n <= SORT_CAP) {
+        for (int k = 0; k < n; ++k) K[k][lane] = key_at(k);
+        for (int k = 1; k < n; ++k) {
+            const Key v = K[k][lane];
+            int q = k - 1;
+            while (q >= 0 && K[q][lane] > v) {
+                K[q + 1][lane] = K[q][lane];
+                --q;
+            }
+            K[q + 1][lane] = v;
+        }
+        for (int k = 0; k < n; ++k) put(k, K[k][lane]);
+    } else {
+        for (int k = 1; k < n; ++k) {
+            const Key v = key_at(k);
+            int q = k - 1;
+            while (q >= 0 && key_at(q) > v) {
+                put(q + 1, key_at(q));
+                --q;
+            }
+            put(q + 1, v);
+        }
+    }
 }
 
 // The product over all rows (stored_row, ed_hmatrow.h).  Persistent
@@ -232,6 +292,16 @@ StoredView hmat_view(const ed_hmat* m) {
 
 // One 256-row tile per workgroup: the fastest setting measured for the 2x12
 // sector (ed_hmat_tune, product plus the reduction of its partials).
+// Rows sorted by column after the fill (the default; ED_HMAT_SORTED_ROWS=0,
+// read at ed_hmat_create, keeps the walk order).  On the 2x12, 2-hole sector
+// the product takes 0.471 ms instead of 0.529 ms: with each row's elements in
+// column order, the 64 lanes reading slot k gather from nearby columns more
+// often.
+bool hmat_sorted_rows() {
+    const char* e = std::getenv("ED_HMAT_SORTED_ROWS");
+    return !(e && std::atoi(e) == 0);
+}
+
 int hmat_default_tiles(int64_t) { return 1; }
 
 template <class VT, int U>
@@ -389,6 +459,11 @@ int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* str
                 }
                 hipLaunchKernelGGL(hm_fill_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
                                    d_map, m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift);
+                ED_HIP(hipGetLastError());
+                if (hmat_sorted_rows())
+                    hipLaunchKernelGGL(m->shift ? hm_sort_kernel<true> : hm_sort_kernel<false>,
+                                       dim3(unsigned(m->nslices)), dim3(64), 0, st, D, m->d_sptr, m->d_cnt, m->d_col,
+                                       m->d_code, m->d_pack);
                 ED_HIP(hipGetLastError());
                 ED_HIP(hipStreamSynchronize(st));
             } else {
