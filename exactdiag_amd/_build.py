@@ -10,7 +10,7 @@ PKG = pathlib.Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libexactdiag_hip.so"
-SOURCES = ["ed_sector.hip", "ed_matrix.hip", "ed_hv.hip", "ed_hmat.hip", "ed_batch.hip", "ed_lanczos.hip", "ed_mlp.hip", "ed_csr.hip"]
+SOURCES = ["ed_sector.hip", "ed_matrix.hip", "ed_hv.hip", "ed_hmat.hip", "ed_batch.hip", "ed_lanczos.hip", "ed_mlp.hip", "ed_csr.hip", "ed_multik.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

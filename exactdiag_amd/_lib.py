@@ -32,6 +32,8 @@ EXPORTED = [
     "ed_csr_sum_create", "ed_csr_sum_info", "ed_csr_sum_hv", "ed_csr_sum_destroy",
     "ed_position_op_columns", "ed_position_op_apply", "ed_position_expectation", "ed_position_correlations",
     "ed_csr_from_position_operator",
+    "ed_sector_create_union", "ed_mk_create", "ed_mk_destroy", "ed_mk_info", "ed_mk_attach", "ed_mk_extract",
+    "ed_mk_hv", "ed_mk_lanczos_run", "ed_mk_set_config",
 ]
 
 _vp = ct.c_void_p
@@ -106,6 +108,15 @@ _SIGNATURES = {
     "ed_position_expectation": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, _vp, _dp, _vp]),
     "ed_position_correlations": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, _vp, _dp, _vp]),
     "ed_csr_from_position_operator": (ct.c_int, [_vp, ct.c_int, _vp, ct.c_int, ct.POINTER(_vp), _vp]),
+    "ed_sector_create_union": (ct.c_int, [ct.c_int] * 3 + [ct.POINTER(_vp)]),
+    "ed_mk_create": (ct.c_int, [_vp, _dp, ct.c_int, _ip, _ip, ct.POINTER(_vp), _vp]),
+    "ed_mk_destroy": (ct.c_int, [_vp]),
+    "ed_mk_info": (ct.c_int, [_vp, ct.POINTER(_i64), ct.POINTER(_i64), ct.POINTER(_i64)]),
+    "ed_mk_attach": (ct.c_int, [_vp, ct.c_int, _vp, _vp, _vp, _vp]),
+    "ed_mk_extract": (ct.c_int, [_vp, ct.c_int, _vp, _vp, _vp, _vp]),
+    "ed_mk_hv": (ct.c_int, [_vp, _vp, _vp, _vp, ct.c_int, _vp]),
+    "ed_mk_lanczos_run": (ct.c_int, [_vp, _vp, ct.c_int, _dp, _dp, _ip, ct.c_int, _vp]),
+    "ed_mk_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
 }
 
 _lib = None

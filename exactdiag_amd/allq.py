@@ -299,3 +299,69 @@ def allq_batched(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, to
     s0.close()
     _sync()
     return time.perf_counter() - t0, out, tm
+
+
+def allq_multik(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL, precision=1,
+                terminator=True):
+    """The fast form with every q in one run: the final sectors share one stored H over the union basis
+    (multik.MultiK, up to 16 momenta per run), their Lanczos recurrences advance together (complex64 vectors
+    with FP64 reductions by default), each closed by the square-root terminator.  The ground state is the fast
+    form's.  Returns (seconds, spectra, stage timings)."""
+    from exactdiag_amd.closure import sqrt_terminator
+    from exactdiag_amd.forms import two_pass_ground_state
+    from exactdiag_amd.multik import SLOTS, MultiK
+
+    tm = dict.fromkeys(("ground_state", "bases", "stored_h", "operator", "lanczos", "continued_fraction"), 0.0)
+
+    def lap(key, t):
+        _sync()
+        now = time.perf_counter()
+        tm[key] += now - t
+        return now
+
+    _sync()
+    t0 = t = time.perf_counter()
+    s0 = Sector(L, h, nd, k0[0], k0[1])
+    t = lap("bases", t)
+    hm0 = s0.store(weights)
+    t = lap("stored_h", t)
+    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, tol=tol)
+    hm0.close()
+    t = lap("ground_state", t)
+    qs = distinct_qs(L, limited)
+    u = Sector.union(L, h, nd)
+    t = lap("bases", t)
+    out = {}
+    for lo in range(0, len(qs), SLOTS):
+        group = qs[lo: lo + SLOTS]
+        ks = [final_momentum(L, k0[0], k0[1], qx, qy) for qx, qy in group]
+        mk = MultiK(u, weights, ks)
+        t = lap("stored_h", t)
+        v0 = mk.block()
+        norm2 = []
+        for slot, ((qx, qy), k) in enumerate(zip(group, ks)):
+            sec = Sector(L, h, nd, *k)
+            t = lap("bases", t)
+            phi = s0.op_apply(sec, _lib.OP_SZQ, qx, qy, psi)
+            n2 = krylov.vec_norm(phi) ** 2 if sec.dim else 0.0
+            if n2 >= 1e-16:
+                mk.attach(slot, sec, phi, v0)
+            norm2.append(n2)
+            del phi
+            sec.close()
+            t = lap("operator", t)
+        coeffs = mk.lanczos(v0, nsteps, precision=precision)
+        del v0
+        mk.close()
+        t = lap("lanczos", t)
+        for (q, n2, (a, b)) in zip(group, norm2, coeffs):
+            if n2 < 1e-16 or len(a) == 0:
+                out[q] = np.zeros(len(ws))
+                continue
+            at, bt, tail = sqrt_terminator(a, b, len(a)) if (terminator and len(a) == nsteps) else (a, b, None)
+            out[q] = cf_eval(at, bt, ws, e0, eta, -n2 / np.pi, tail=tail)
+        t = lap("continued_fraction", t)
+    u.close()
+    s0.close()
+    _sync()
+    return time.perf_counter() - t0, out, tm

@@ -31,6 +31,7 @@ struct SectorView {
     int64_t n_spin;    // C(ne, nd)
     int n_hole_ranks;  // C(N, h)
     int n_blocks;
+    int any_k;         // 1: every lowest image is a basis state (no momentum compatibility test)
     const uint32_t* comb;         // C(n, k) for n, k <= 32
     const uint32_t* canon_gmask;  // per hole rank: bit p set if image p gives the minimal hole mask
     const int32_t* canon_block;   // per hole rank: block of the minimal image (-1 = no basis states)

@@ -351,132 +351,149 @@ int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, i
     return nwg;
 }
 
+// Stored structure of sector s.  coded = 0: each element's code indexes the
+// exact unique values (the product's table).  coded = 1: an element's code is
+// term * 32 + p (the term and the group element mapping its target to the
+// representative), the sign is kept and the value table holds 128 unused
+// entries followed by the diagonal values; the momentum phase and orbit-size
+// factors are left to the caller (the multi-momentum product).
+ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t st) {
+    ED_REQUIRE(s && s->D >= 0 && w, ED_ERR_ARG, "bad stored-matrix arguments");
+    auto* m = new ed_hmat();
+    std::vector<void*> tmp;
+    auto cleanup_tmp = [&] {
+        for (void* p : tmp) dev_free(p);
+        tmp.clear();
+    };
+    try {
+        m->s = s;
+        m->w = make_weights(w);
+        m->D = s->D;
+        const int N = s->N, L = s->L;
+        // exact unique values: off-diagonal combos, then diagonal (jl count, jr count) pairs
+        std::vector<double> vals;
+        std::map<std::pair<uint64_t, uint64_t>, int> uid;
+        auto intern = [&](C2 v) {
+            const auto key = std::make_pair(bits(v.re), bits(v.im));
+            auto it = uid.find(key);
+            if (it != uid.end()) return it->second;
+            const int id = int(vals.size() / 2);
+            vals.push_back(v.re);
+            vals.push_back(v.im);
+            uid.emplace(key, id);
+            return id;
+        };
+        const int ncombo = 4 * N * (N + 1);
+        std::vector<uint16_t> map(size_t(ncombo), 0);
+        if (coded) {
+            ED_REQUIRE(N <= 32, ED_ERR_ARG, "coded elements need at most 32 group elements");
+            vals.assign(2 * 128, 0.0);
+            for (int t = 0; t < 4; ++t)
+                for (int p = 0; p < N; ++p)
+                    for (int nj = 0; nj <= N; ++nj) map[size_t((t * N + p) * (N + 1) + nj)] = uint16_t(t * 32 + p);
+        }
+        for (int t = 0; t < 4 && !coded; ++t) {
+            if (!(m->w.active & (1 << t))) continue;
+            const double st_ = t < 2 ? 1.0 : 0.5;
+            for (int p = 0; p < N; ++p)
+                for (int nj = 0; nj <= N; ++nj) {
+                    const int idx = p * (N + 1) + nj;
+                    const double ar = s->amp[2 * idx], ai = s->amp[2 * idx + 1];
+                    map[size_t((t * N + p) * (N + 1) + nj)] =
+                        uint16_t(intern(cmul(m->w.w[t], C2{st_ * ar, -(st_ * ai)})));
+                }
+        }
+        const int n_off = int(vals.size() / 2);  // off-diagonal values come first
+        std::vector<uint16_t> dmap(size_t((N + 1) * (L + 1)), 0);
+        for (int a = 0; a <= N; ++a)
+            for (int b = 0; b <= L; ++b) {
+                C2 d{0.0, 0.0};
+                if (m->w.active & 4) d = cadd(d, cmul(m->w.w[2], C2{-0.5 * a, 0.0}));
+                if (m->w.active & 8) d = cadd(d, cmul(m->w.w[3], C2{-0.5 * b, 0.0}));
+                dmap[size_t(a * (L + 1) + b)] = uint16_t(intern(d));
+            }
+        m->n_vals = int(vals.size() / 2);
+        ED_REQUIRE(m->n_vals < 0x8000, ED_ERR_CAPACITY, "more than 32767 unique matrix values");
+        // 4-byte elements when column, value index and sign fit 32 bits (ED_HMAT_ELEMENT_BYTES=6 forces the
+        // 6-byte form)
+        {
+            int col_bits = 1;
+            while (col_bits < 31 && (int64_t(1) << col_bits) < s->D) ++col_bits;
+            const int code_bits = 32 - col_bits;
+            const char* force = std::getenv("ED_HMAT_ELEMENT_BYTES");
+            const bool six = force && std::atoi(force) == 6;
+            if (!six && code_bits >= 2 && int64_t(n_off) <= (int64_t(1) << (code_bits - 1))) m->shift = code_bits;
+        }
+        m->d_vals = dev_upload(vals, st);
+        const int64_t D = s->D;
+        m->nslices = (D + 63) / 64;
+        m->d_cnt = dev_alloc<uint8_t>(size_t(std::max<int64_t>(D, 1)));
+        m->d_diag = dev_alloc<uint16_t>(size_t(std::max<int64_t>(D, 1)));
+        // partials of the product (one per workgroup, at most one per tile) + one dot for the tuner
+        m->d_partial = dev_alloc<double>(size_t(2 * std::max<int64_t>((D + BS - 1) / BS, 1) + 2));
+        std::vector<int64_t> sptr(static_cast<size_t>(m->nslices + 1), 0);
+        if (D > 0) {
+            uint16_t* d_map = dev_upload(map, st);
+            tmp.push_back(d_map);
+            uint16_t* d_dmap = dev_upload(dmap, st);
+            tmp.push_back(d_dmap);
+            int32_t* d_width = dev_alloc<int32_t>(size_t(m->nslices));
+            tmp.push_back(d_width);
+            const LdsLayout lay = lds_layout(s);
+            const int lds = lay.bytes <= LDS_CAP;
+            const unsigned nwg = unsigned((D + BS - 1) / BS);
+            const SectorView v = s->view();
+            hipLaunchKernelGGL(hm_count_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
+                               d_dmap, m->d_cnt, m->d_diag);
+            hipLaunchKernelGGL(hm_width_kernel, dim3(unsigned((m->nslices + 3) / 4)), dim3(BS), 0, st, m->d_cnt,
+                               D, d_width, m->nslices);
+            ED_HIP(hipGetLastError());
+            std::vector<int32_t> width(static_cast<size_t>(m->nslices));
+            ED_HIP(hipMemcpyAsync(width.data(), d_width, width.size() * 4, hipMemcpyDeviceToHost, st));
+            std::vector<uint8_t> cnt(static_cast<size_t>(D));
+            ED_HIP(hipMemcpyAsync(cnt.data(), m->d_cnt, size_t(D), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+            for (int64_t i = 0; i < m->nslices; ++i) sptr[size_t(i + 1)] = sptr[size_t(i)] + int64_t(width[i]) * 64;
+            for (uint8_t c : cnt) m->nnz += c;
+            m->slots = sptr.back();
+            m->d_sptr = dev_upload(sptr, st);
+            if (m->shift) {
+                m->d_pack = dev_alloc<uint32_t>(size_t(std::max<int64_t>(m->slots, 1)));
+            } else {
+                m->d_col = dev_alloc<int32_t>(size_t(std::max<int64_t>(m->slots, 1)));
+                m->d_code = dev_alloc<uint16_t>(size_t(std::max<int64_t>(m->slots, 1)));
+            }
+            hipLaunchKernelGGL(hm_fill_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
+                               d_map, m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift);
+            ED_HIP(hipGetLastError());
+            if (hmat_sorted_rows())
+                hipLaunchKernelGGL(m->shift ? hm_sort_kernel<true> : hm_sort_kernel<false>,
+                                   dim3(unsigned(m->nslices)), dim3(64), 0, st, D, m->d_sptr, m->d_cnt, m->d_col,
+                                   m->d_code, m->d_pack);
+            ED_HIP(hipGetLastError());
+            ED_HIP(hipStreamSynchronize(st));
+        } else {
+            m->d_sptr = dev_upload(sptr, st);
+            ED_HIP(hipStreamSynchronize(st));
+        }
+        cleanup_tmp();
+    } catch (...) {
+        cleanup_tmp();
+        delete m;
+        throw;
+    }
+    return m;
+}
+
 }  // namespace ed
 
 extern "C" {
 
 int ed_hmat_create(const ed_sector* s, const double* w, ed_hmat** out, void* stream) {
     return guarded([&] {
-        ED_REQUIRE(s && s->D >= 0 && w && out, ED_ERR_ARG, "bad stored-matrix arguments");
+        ED_REQUIRE(out, ED_ERR_ARG, "bad stored-matrix arguments");
         *out = nullptr;
-        hipStream_t st = static_cast<hipStream_t>(stream);
-        auto* m = new ed_hmat();
-        std::vector<void*> tmp;
-        auto cleanup_tmp = [&] {
-            for (void* p : tmp) dev_free(p);
-            tmp.clear();
-        };
-        try {
-            m->s = s;
-            m->w = make_weights(w);
-            m->D = s->D;
-            const int N = s->N, L = s->L;
-            // exact unique values: off-diagonal combos, then diagonal (jl count, jr count) pairs
-            std::vector<double> vals;
-            std::map<std::pair<uint64_t, uint64_t>, int> uid;
-            auto intern = [&](C2 v) {
-                const auto key = std::make_pair(bits(v.re), bits(v.im));
-                auto it = uid.find(key);
-                if (it != uid.end()) return it->second;
-                const int id = int(vals.size() / 2);
-                vals.push_back(v.re);
-                vals.push_back(v.im);
-                uid.emplace(key, id);
-                return id;
-            };
-            const int ncombo = 4 * N * (N + 1);
-            std::vector<uint16_t> map(size_t(ncombo), 0);
-            for (int t = 0; t < 4; ++t) {
-                if (!(m->w.active & (1 << t))) continue;
-                const double st_ = t < 2 ? 1.0 : 0.5;
-                for (int p = 0; p < N; ++p)
-                    for (int nj = 0; nj <= N; ++nj) {
-                        const int idx = p * (N + 1) + nj;
-                        const double ar = s->amp[2 * idx], ai = s->amp[2 * idx + 1];
-                        map[size_t((t * N + p) * (N + 1) + nj)] =
-                            uint16_t(intern(cmul(m->w.w[t], C2{st_ * ar, -(st_ * ai)})));
-                    }
-            }
-            const int n_off = int(vals.size() / 2);  // off-diagonal values come first
-            std::vector<uint16_t> dmap(size_t((N + 1) * (L + 1)), 0);
-            for (int a = 0; a <= N; ++a)
-                for (int b = 0; b <= L; ++b) {
-                    C2 d{0.0, 0.0};
-                    if (m->w.active & 4) d = cadd(d, cmul(m->w.w[2], C2{-0.5 * a, 0.0}));
-                    if (m->w.active & 8) d = cadd(d, cmul(m->w.w[3], C2{-0.5 * b, 0.0}));
-                    dmap[size_t(a * (L + 1) + b)] = uint16_t(intern(d));
-                }
-            m->n_vals = int(vals.size() / 2);
-            ED_REQUIRE(m->n_vals < 0x8000, ED_ERR_CAPACITY, "more than 32767 unique matrix values");
-            // 4-byte elements when column, value index and sign fit 32 bits (ED_HMAT_ELEMENT_BYTES=6 forces the
-            // 6-byte form)
-            {
-                int col_bits = 1;
-                while (col_bits < 31 && (int64_t(1) << col_bits) < s->D) ++col_bits;
-                const int code_bits = 32 - col_bits;
-                const char* force = std::getenv("ED_HMAT_ELEMENT_BYTES");
-                const bool six = force && std::atoi(force) == 6;
-                if (!six && code_bits >= 2 && int64_t(n_off) <= (int64_t(1) << (code_bits - 1))) m->shift = code_bits;
-            }
-            m->d_vals = dev_upload(vals, st);
-            const int64_t D = s->D;
-            m->nslices = (D + 63) / 64;
-            m->d_cnt = dev_alloc<uint8_t>(size_t(std::max<int64_t>(D, 1)));
-            m->d_diag = dev_alloc<uint16_t>(size_t(std::max<int64_t>(D, 1)));
-            // partials of the product (one per workgroup, at most one per tile) + one dot for the tuner
-            m->d_partial = dev_alloc<double>(size_t(2 * std::max<int64_t>((D + BS - 1) / BS, 1) + 2));
-            std::vector<int64_t> sptr(static_cast<size_t>(m->nslices + 1), 0);
-            if (D > 0) {
-                uint16_t* d_map = dev_upload(map, st);
-                tmp.push_back(d_map);
-                uint16_t* d_dmap = dev_upload(dmap, st);
-                tmp.push_back(d_dmap);
-                int32_t* d_width = dev_alloc<int32_t>(size_t(m->nslices));
-                tmp.push_back(d_width);
-                const LdsLayout lay = lds_layout(s);
-                const int lds = lay.bytes <= LDS_CAP;
-                const unsigned nwg = unsigned((D + BS - 1) / BS);
-                const SectorView v = s->view();
-                hipLaunchKernelGGL(hm_count_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
-                                   d_dmap, m->d_cnt, m->d_diag);
-                hipLaunchKernelGGL(hm_width_kernel, dim3(unsigned((m->nslices + 3) / 4)), dim3(BS), 0, st, m->d_cnt,
-                                   D, d_width, m->nslices);
-                ED_HIP(hipGetLastError());
-                std::vector<int32_t> width(static_cast<size_t>(m->nslices));
-                ED_HIP(hipMemcpyAsync(width.data(), d_width, width.size() * 4, hipMemcpyDeviceToHost, st));
-                std::vector<uint8_t> cnt(static_cast<size_t>(D));
-                ED_HIP(hipMemcpyAsync(cnt.data(), m->d_cnt, size_t(D), hipMemcpyDeviceToHost, st));
-                ED_HIP(hipStreamSynchronize(st));
-                for (int64_t i = 0; i < m->nslices; ++i) sptr[size_t(i + 1)] = sptr[size_t(i)] + int64_t(width[i]) * 64;
-                for (uint8_t c : cnt) m->nnz += c;
-                m->slots = sptr.back();
-                m->d_sptr = dev_upload(sptr, st);
-                if (m->shift) {
-                    m->d_pack = dev_alloc<uint32_t>(size_t(std::max<int64_t>(m->slots, 1)));
-                } else {
-                    m->d_col = dev_alloc<int32_t>(size_t(std::max<int64_t>(m->slots, 1)));
-                    m->d_code = dev_alloc<uint16_t>(size_t(std::max<int64_t>(m->slots, 1)));
-                }
-                hipLaunchKernelGGL(hm_fill_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
-                                   d_map, m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift);
-                ED_HIP(hipGetLastError());
-                if (hmat_sorted_rows())
-                    hipLaunchKernelGGL(m->shift ? hm_sort_kernel<true> : hm_sort_kernel<false>,
-                                       dim3(unsigned(m->nslices)), dim3(64), 0, st, D, m->d_sptr, m->d_cnt, m->d_col,
-                                       m->d_code, m->d_pack);
-                ED_HIP(hipGetLastError());
-                ED_HIP(hipStreamSynchronize(st));
-            } else {
-                m->d_sptr = dev_upload(sptr, st);
-                ED_HIP(hipStreamSynchronize(st));
-            }
-            cleanup_tmp();
-        } catch (...) {
-            cleanup_tmp();
-            delete m;
-            throw;
-        }
-        *out = m;
+        *out = hmat_build(s, w, 0, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -97,6 +97,7 @@ int shifted_site(int N, int i, int leg, int rung);
 
 struct ed_sector {
     int L = 0, N = 0, h = 0, ne = 0, nd = 0, kx = 0, ky = 0;
+    int any_k = 0;  // union of the momentum sectors: see ed_sector_create_union
     int64_t n_spin = 0, n_hole = 0;
     std::vector<uint32_t> comb;        // COMB_W * COMB_W
     std::vector<double> amp;           // see SectorView::amp

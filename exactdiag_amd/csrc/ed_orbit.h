@@ -22,9 +22,11 @@ struct OrbitResult {
 // image is lower than the input (check_lowest), repeat detection only for
 // rung shifts m <= L/2, compatibility tested on the amplitude of the first
 // repeat (|a - a_ref| > 1e-8 rejects), and the mirrored half skipped once the
-// state is mirror symmetric.  Amplitudes are sign * F[m][b] exactly as the
-// reference's running product produces them (the running product differs
-// from the unsigned one only by exact sign flips).
+// state is mirror symmetric.  With any_k the compatibility test is skipped
+// (every lowest image is kept, with the same orbit size).  Amplitudes are
+// sign * F[m][b] exactly as the reference's running product produces them
+// (the running product differs from the unsigned one only by exact sign
+// flips).
 __host__ __device__ inline OrbitResult orbit_walk(const SectorView& s, uint32_t H, uint32_t Dn, bool check_lowest) {
     const uint64_t k0 = make_key(H, Dn);
     bool mirror_sym = false;
@@ -45,7 +47,7 @@ __host__ __device__ inline OrbitResult orbit_walk(const SectorView& s, uint32_t 
         if (m < half) {
             if (kt == k0) {
                 const double dr = ar - 1.0, di = ai;
-                if (dr * dr + di * di > 1e-16) return {0, 0, 0};
+                if (!s.any_k && dr * dr + di * di > 1e-16) return {0, 0, 0};
                 if (p == 1) {
                     mirror_sym = true;
                     continue;
@@ -54,7 +56,7 @@ __host__ __device__ inline OrbitResult orbit_walk(const SectorView& s, uint32_t 
             }
             if (count != 1 && kt == k1) {
                 const double dr = ar - a1r, di = ai - a1i;
-                if (dr * dr + di * di > 1e-16) return {0, 0, 0};
+                if (!s.any_k && dr * dr + di * di > 1e-16) return {0, 0, 0};
                 break;
             }
         }

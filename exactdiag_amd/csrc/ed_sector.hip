@@ -374,6 +374,7 @@ ed::SectorView ed_sector::host_view() const {
     v.n_spin = n_spin;
     v.n_hole_ranks = int(n_hole);
     v.n_blocks = int(block_rank.size());
+    v.any_k = any_k;
     v.comb = comb.data();
     v.amp = amp.data();
     v.sqrt_tab = sqrt_tab.data();
@@ -420,7 +421,12 @@ const char* ed_last_error(void) { return ed::g_last_error.c_str(); }
 
 int ed_version(void) { return 1; }
 
-int ed_sector_create(int L, int h, int nd, int kx, int ky, ed_sector** out) {
+}  // extern "C"
+
+namespace ed {
+namespace {
+
+int sector_create(int L, int h, int nd, int kx, int ky, int any_k, ed_sector** out) {
     return guarded([&] {
         ED_REQUIRE(out != nullptr, ED_ERR_ARG, "out is null");
         ED_REQUIRE(L >= 1 && 2 * L <= MAX_N, ED_ERR_ARG, "num_rungs must be in 1..16");
@@ -436,6 +442,7 @@ int ed_sector_create(int L, int h, int nd, int kx, int ky, ed_sector** out) {
         s->nd = nd;
         s->kx = kx;
         s->ky = ky;
+        s->any_k = any_k;
         // binomials C(n, k), exact integers (the reference's table,
         // general/matrix_rules_utils.pyx:122-134, holds the same values)
         s->comb.assign(COMB_W * COMB_W, 0);
@@ -523,6 +530,17 @@ int ed_sector_create(int L, int h, int nd, int kx, int ky, ed_sector** out) {
         *out = s;
     });
 }
+
+}  // namespace
+}  // namespace ed
+
+extern "C" {
+
+int ed_sector_create(int L, int h, int nd, int kx, int ky, ed_sector** out) {
+    return sector_create(L, h, nd, kx, ky, 0, out);
+}
+
+int ed_sector_create_union(int L, int h, int nd, ed_sector** out) { return sector_create(L, h, nd, 0, 0, 1, out); }
 
 int ed_sector_destroy(ed_sector* s) {
     return guarded([&] {

@@ -349,3 +349,76 @@ def positions_form(reps=20, log=print):
     states["singlet"] = (s4, x.cpu().numpy())
     hm.close()
     return out, states
+
+
+def multik_form(L=12, h=2, nd=11, weights=(1.0, 1.0, 0.4, 0.4), reps=10, log=print):
+    """The final sectors of all distinct q (14 momenta of the 2x12 ladder, 2 holes) in one stored product over
+    the union basis, against the same 14 products with one stored H per sector (StoredH), complex64 and
+    complex128.  Agreement: each slot against its sector's own product.  Returns a dict."""
+    import torch
+
+    from exactdiag_amd.allq import distinct_qs, final_momentum
+    from exactdiag_amd.multik import MultiK
+
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        _sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        _sync()
+        return e0.elapsed_time(e1) / reps
+
+    ks = [final_momentum(L, 0, 0, qx, qy) for qx, qy in distinct_qs(L)]
+    t0 = time.perf_counter()
+    u = Sector.union(L, h, nd)
+    _sync()
+    t_union = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mk = MultiK(u, weights, ks)
+    _sync()
+    t_build = time.perf_counter() - t0
+    secs = [Sector(L, h, nd, *k) for k in ks]
+    mats = [s.store(weights) for s in secs]
+    out = {"union_dim": u.dim, "sector_dims": [s.dim for s in secs], "slots": len(ks), "nnz": mk.nnz,
+           "union_basis_s": t_union, "build_s": t_build, "bytes": mk.bytes}
+    g = torch.Generator(device="cuda").manual_seed(7)
+    xs = [torch.complex(torch.randn(s.dim, dtype=torch.float64, device="cuda", generator=g),
+                        torch.randn(s.dim, dtype=torch.float64, device="cuda", generator=g)) for s in secs]
+    block = mk.block()
+    for k, (s, x) in enumerate(zip(secs, xs)):
+        mk.attach(k, s, x, block)
+    worst = 0.0
+    y = mk.hv(block)
+    for k, (s, m, x) in enumerate(zip(secs, mats, xs)):
+        yk = m.hv(x.clone())
+        worst = max(worst, float((mk.extract(k, s, y) - yk).abs().max() / yk.abs().max()))
+    out["max_rel_diff_c128"] = worst
+    del y
+    for name, dt in (("c64", torch.complex64), ("c128", torch.complex128)):
+        xb = block.to(dt)
+        yb = torch.empty_like(xb)
+        ms_mk = timed(lambda: mk.hv(xb, out=yb))
+        xv = [x.to(dt) for x in xs]
+        yv = [torch.empty_like(x) for x in xv]
+
+        def serial():
+            for m, a, b in zip(mats, xv, yv):
+                m.hv(a, out=b)
+
+        ms_serial = timed(serial)
+        states = sum(out["sector_dims"])
+        out[name] = {"multik_ms": ms_mk, "serial_ms": ms_serial, "speedup": ms_serial / ms_mk,
+                     "gstates_per_s": states / ms_mk / 1e6}
+        log(f"  {name}: one product over {len(ks)} momenta {ms_mk:.3f} ms, {len(ks)} stored products "
+            f"{ms_serial:.3f} ms ({ms_serial / ms_mk:.2f}x)")
+        del xb, yb, xv, yv
+    for m in mats:
+        m.close()
+    mk.close()
+    return out

@@ -55,17 +55,27 @@ def weights_array(weights) -> np.ndarray:
 class Sector:
     """Basis of one (L, h, nd, kx, ky) sector, built on the GPU."""
 
-    def __init__(self, num_rungs: int, num_holes: int, num_down_spins: int, kx: int, ky: int):
+    def __init__(self, num_rungs: int, num_holes: int, num_down_spins: int, kx: int, ky: int, *, _union=False):
         self.L, self.h, self.nd, self.kx, self.ky = num_rungs, num_holes, num_down_spins, int(kx), int(ky)
         self.N = 2 * num_rungs
+        self.any_k = bool(_union)
         self._lib = lib()
         torch = _torch()
         self.device = torch.device("cuda", torch.cuda.current_device())
         h = ct.c_void_p()
-        check(self._lib.ed_sector_create(num_rungs, num_holes, num_down_spins, self.kx, self.ky, ct.byref(h)))
+        if _union:
+            check(self._lib.ed_sector_create_union(num_rungs, num_holes, num_down_spins, ct.byref(h)))
+        else:
+            check(self._lib.ed_sector_create(num_rungs, num_holes, num_down_spins, self.kx, self.ky, ct.byref(h)))
         self._h = h
         check(self._lib.ed_basis_build(self._h, _stream()))
         self.dim = int(self._lib.ed_basis_size(self._h))
+
+    @classmethod
+    def union(cls, num_rungs: int, num_holes: int, num_down_spins: int) -> "Sector":
+        """Every lowest image of the ladder's states: the union of its momentum sectors
+        (ed_sector_create_union), the row space of :class:`exactdiag_amd.multik.MultiK`."""
+        return cls(num_rungs, num_holes, num_down_spins, 0, 0, _union=True)
 
     def close(self):
         h = getattr(self, "_h", None)

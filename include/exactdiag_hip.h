@@ -287,6 +287,44 @@ int ed_batch_get_config(const ed_batch* b, int* tiles, int* gather, int* n_jobs)
 int ed_hv_tune(ed_sector* s, const double* w, int precision, const double* d_x, double* d_y, int* tiles, int* gather,
                double* best_ms, void* stream);
 
+/* Several momentum sectors of one ladder in one product.  Replaces, for up
+ * to 16 sectors k of the same (L, h, nd) at once, the per-sector H.dot
+ * (general/sparse_matrices.py:83-105, :233-237) and the Lanczos recurrence
+ * (general/cython_lanczos_diagonalization.pyx:18-77) that get_spectrum runs
+ * once per q (tJ_spin_half_ladder/api.py:116-132).
+ *
+ * ed_sector_create_union: the union of the momentum sectors, every lowest
+ * image under the ladder's group (symmetry_ladder.h:113-235 without the
+ * compatibility test); build its basis with ed_basis_build.
+ * ed_mk_create: one stored structure over the union, slot k carrying the
+ * momentum (kx[k], ky[k]).  Vectors are blocks complex[D_union][16] in scaled
+ * coordinates x'_f = x_f / sqrt(n_f), n_f the orbit size; slot k is masked
+ * to sector k's states.
+ * ed_mk_attach: write sector vector d_phi (the sector's own dense order) into
+ * slot `slot` of d_block (zero the block first) and mark the slot's states;
+ * ed_mk_extract: the reverse.  The sector must have the slot's momentum.
+ * ed_mk_hv: d_y = H d_x for all slots (precision 0: complex128 blocks, 1:
+ * complex64 blocks); d_dot (device, 32 doubles, optional) gets <x_k, H x_k>.
+ * ed_mk_lanczos_run: plain Lanczos from the complex128 block d_v0 for all
+ * slots, a[k*nsteps + n], b[k*(nsteps+1) + n] (b_0 = 0), n_out[k] = steps
+ * done, each slot stopping at b < 1e-8; vectors complex128 (precision 0) or
+ * complex64 with FP64 reductions (precision 1). */
+typedef struct ed_mk ed_mk;
+int ed_sector_create_union(int L, int h, int nd, ed_sector** out);
+int ed_mk_create(const ed_sector* u, const double* w, int n_slots, const int* kx, const int* ky, ed_mk** out,
+                 void* stream);
+int ed_mk_destroy(ed_mk* m);
+/* Launch shape of the product (no counterpart in the reference): waves = 4
+ * or 8 wavefronts per workgroup (16 or 32 rows), unroll = 4 or 8 gathers in
+ * flight per lane. */
+int ed_mk_set_config(ed_mk* m, int waves, int unroll);
+int ed_mk_info(const ed_mk* m, int64_t* dim, int64_t* nnz, int64_t* bytes);
+int ed_mk_attach(ed_mk* m, int slot, const ed_sector* sk, const double* d_phi, double* d_block, void* stream);
+int ed_mk_extract(const ed_mk* m, int slot, const ed_sector* sk, const double* d_block, double* d_out, void* stream);
+int ed_mk_hv(const ed_mk* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream);
+int ed_mk_lanczos_run(const ed_mk* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
+                      int precision, void* stream);
+
 /* Continued fraction of general/cython_lanczos_diagonalization.pyx:42-48
  * evaluated on the GPU: out[k] = factor * Im(1 / det(z_k)) for z_k = ws[k] +
  * e0 + i*eta.  Host arrays in and out. */

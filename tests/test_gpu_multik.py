@@ -1,0 +1,240 @@
+"""Multi-momentum stored product (ed_mk_*): every slot against its own sector.
+
+One stored structure over the union basis carries up to 16 momentum sectors
+of a ladder.  Each slot must reproduce, for its sector, what the
+single-sector stored H gives (itself pinned against the oracle and the
+reference's eigenvalue fixtures): the product to rounding (the element values
+are formed in another order, so not bit for bit), the fused <x, Hx>, and the
+Lanczos coefficients of general/cython_lanczos_diagonalization.pyx:18-77,
+including where a slot stops at b < 1e-8.  Rows outside a slot's sector stay
+zero.  The union basis holds every sector's states with the same orbit sizes
+(the attach kernel checks both) and nothing but lowest images.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+W = [1.0, 1.0, 0.4, 0.4]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    assert t.cuda.is_available()
+    return t
+
+
+def _momenta(L):
+    from exactdiag_amd.allq import distinct_qs, final_momentum
+
+    return [final_momentum(L, 0, 0, qx, qy) for qx, qy in distinct_qs(L)]
+
+
+def _setup(L, h, nd):
+    from exactdiag_amd.multik import MultiK
+    from exactdiag_amd.sector import Sector
+
+    u = Sector.union(L, h, nd)
+    ks = _momenta(L)
+    secs = [Sector(L, h, nd, kx, ky) for kx, ky in ks]
+    return u, secs, MultiK(u, W, ks)
+
+
+def _rand(torch, n, seed):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    re = torch.randn(max(n, 1), dtype=torch.float64, device="cuda", generator=g)
+    im = torch.randn(max(n, 1), dtype=torch.float64, device="cuda", generator=g)
+    return torch.complex(re, im)[:n].contiguous()
+
+
+@pytest.fixture(scope="module")
+def ladder6(torch):
+    return _setup(6, 2, 5)
+
+
+@pytest.mark.parametrize("L,h,nd", [(4, 2, 3), (5, 1, 4), (6, 2, 5)])
+def test_union_holds_every_sector_and_only_lowest_images(torch, L, h, nd):
+    from exactdiag_amd.sector import Sector
+
+    print("union", L, h, nd, flush=True)
+    u = Sector.union(L, h, nd)
+    us, uo = u.basis()
+    assert np.all(np.diff(us) > 0)
+    all_k = [Sector(L, h, nd, kx, ky) for kx in range(L) for ky in (0, 1)]
+    covered = np.zeros(len(us), bool)
+    for s in all_k:
+        ss, so = s.basis()
+        at = np.searchsorted(us, ss)
+        assert np.all(us[at] == ss), (s.kx, s.ky)
+        assert np.array_equal(uo[at], so), (s.kx, s.ky)
+    # every union state is its own lowest image in some sector
+    for s in all_k:
+        rep, _ = s.lowest(us)
+        covered |= rep == us
+    assert covered.all()
+
+
+def test_hv_matches_each_sector(torch, ladder6):
+    from exactdiag_amd.sector import StoredH
+
+    u, secs, mk = ladder6
+    x = mk.block()
+    xs = []
+    for k, s in enumerate(secs):
+        xs.append(_rand(torch, s.dim, 10 + k))
+        mk.attach(k, s, xs[-1], x)
+    y, dots = mk.hv(x, with_dot=True)
+    for k, s in enumerate(secs):
+        print("slot", k, (s.kx, s.ky), "dim", s.dim, flush=True)
+        hm = StoredH(s, W)
+        yk, dk = hm.hv(xs[k].clone(), with_dot=True)
+        got = mk.extract(k, s, y)
+        err = float((got - yk).abs().max())
+        assert err <= 1e-13 * max(1.0, float(yk.abs().max())), (k, err)
+        assert abs(dots[k] - dk) <= 1e-11 * max(1.0, abs(dk)), (k, dots[k], dk)
+        back = mk.extract(k, s, x)
+        assert float((back - xs[k]).abs().max()) <= 1e-14 * float(xs[k].abs().max())
+        hm.close()
+
+
+def test_rows_outside_a_slot_stay_zero(torch, ladder6):
+    u, secs, mk = ladder6
+    x = mk.block()
+    for k, s in enumerate(secs):
+        mk.attach(k, s, _rand(torch, s.dim, 30 + k), x)
+    y = mk.hv(x).cpu().numpy()
+    us, _ = u.basis()
+    for k, s in enumerate(secs):
+        ss, _ = s.basis()
+        member = np.isin(us, ss)
+        assert not np.any(y[~member, k]), k
+        assert not np.any(y[:, len(secs):])
+
+
+def test_hv_complex64(torch, ladder6):
+    from exactdiag_amd.sector import StoredH
+
+    u, secs, mk = ladder6
+    x = mk.block()
+    xs = []
+    for k, s in enumerate(secs):
+        xs.append(_rand(torch, s.dim, 50 + k))
+        mk.attach(k, s, xs[-1], x)
+    y = mk.hv(x.to(torch.complex64)).to(torch.complex128)
+    for k, s in enumerate(secs):
+        hm = StoredH(s, W)
+        yk = hm.hv(xs[k].clone())
+        got = mk.extract(k, s, y)
+        err = float((got - yk).abs().max())
+        assert err <= 2e-5 * float(yk.abs().max()), (k, err)
+        hm.close()
+
+
+@pytest.mark.parametrize("precision,tol,n_cmp", [(0, 1e-10, 60), (1, 1e-5, 30)])
+def test_lanczos_matches_each_sector(torch, ladder6, precision, tol, n_cmp):
+    """complex128: all 60 steps to 1e-10.  complex64: two runs that differ in rounding drift apart once the
+    extreme Ritz values have converged (the sector has ~1400 states), so the first 30 steps are compared."""
+    from exactdiag_amd.sector import StoredH
+
+    u, secs, mk = ladder6
+    v0 = mk.block()
+    starts = []
+    for k, s in enumerate(secs):
+        starts.append(_rand(torch, s.dim, 70 + k))
+        mk.attach(k, s, starts[-1], v0)
+    got = mk.lanczos(v0, 60, precision=precision)
+    for k, s in enumerate(secs):
+        hm = StoredH(s, W)
+        a, b = hm.lanczos(starts[k], 60, precision=precision)
+        hm.close()
+        print("slot", k, "steps", len(a), len(got[k][0]), flush=True)
+        assert len(got[k][0]) == len(a), k
+        scale = max(1.0, float(np.abs(a).max()))
+        err_a = np.abs(got[k][0][:n_cmp] - a[:n_cmp]).max()
+        assert err_a <= tol * scale, (k, err_a)
+        assert np.abs(got[k][1][:n_cmp] - b[:n_cmp]).max() <= tol * scale, k
+
+
+def test_lanczos_matches_oracle_small(torch, oracle_mod):
+    """Slot coefficients against the oracle's Lanczos (general/cython_lanczos_diagonalization.pyx:18-77)."""
+    u, secs, mk = _setup(4, 2, 3)
+    v0 = mk.block()
+    starts = []
+    for k, s in enumerate(secs):
+        rng = np.random.default_rng(90 + k)
+        v = rng.standard_normal(s.dim) + 1j * rng.standard_normal(s.dim)
+        starts.append(v)
+        mk.attach(k, s, torch.as_tensor(v, device="cuda"), v0)
+    got = mk.lanczos(v0, 12)
+    for k, s in enumerate(secs):
+        o = oracle_mod.OracleSector(4, 2, 3, s.kx, s.ky)
+        a, b = o.lanczos(W, starts[k], 12)
+        n = min(len(a), len(got[k][0]))
+        assert len(got[k][0]) == len(a), (k, len(got[k][0]), len(a))
+        assert np.allclose(got[k][0][:n], a[:n], atol=1e-10, rtol=0), k
+        assert np.allclose(got[k][1][:n], b[:n], atol=1e-10, rtol=0), k
+
+
+def test_slots_stop_independently(torch):
+    """Small sectors exhaust their Krylov space at different steps; each slot stops as its own run does."""
+    from exactdiag_amd.sector import StoredH
+
+    u, secs, mk = _setup(3, 1, 1)
+    v0 = mk.block()
+    starts = []
+    for k, s in enumerate(secs):
+        starts.append(_rand(torch, s.dim, 110 + k))
+        mk.attach(k, s, starts[-1], v0)
+    got = mk.lanczos(v0, 40)
+    for k, s in enumerate(secs):
+        if s.dim == 0:
+            assert len(got[k][0]) == 0
+            continue
+        hm = StoredH(s, W)
+        a, b = hm.lanczos(starts[k], 40)
+        hm.close()
+        assert len(got[k][0]) == len(a) <= s.dim, (k, len(got[k][0]), len(a), s.dim)
+        assert np.allclose(got[k][0], a, atol=1e-10, rtol=0), k
+
+
+def test_attach_checks_the_slot_momentum(torch, ladder6):
+    from exactdiag_amd.sector import Sector
+
+    u, secs, mk = ladder6
+    other = secs[1]
+    x = mk.block()
+    with pytest.raises(ValueError):
+        mk.attach(0, other, _rand(torch, other.dim, 1), x)
+    with pytest.raises(ValueError):
+        mk.attach(len(secs), secs[0], _rand(torch, secs[0].dim, 1), x)
+    with pytest.raises(ValueError):
+        from exactdiag_amd.multik import MultiK
+
+        MultiK(Sector(6, 2, 5, 0, 0), W, [(0, 0)])
+
+
+def test_2x10_all_q(torch):
+    """All 12 distinct q of the 2x10 ladder with 2 holes (union D ~ 4.6e5): products and 30 Lanczos steps."""
+    from exactdiag_amd.sector import StoredH
+
+    u, secs, mk = _setup(10, 2, 9)
+    print("union dim", u.dim, "slots", len(secs), flush=True)
+    v0 = mk.block()
+    starts = []
+    for k, s in enumerate(secs):
+        starts.append(_rand(torch, s.dim, 200 + k))
+        mk.attach(k, s, starts[-1], v0)
+    y = mk.hv(v0)
+    got = mk.lanczos(v0, 30)
+    for k, s in enumerate(secs):
+        hm = StoredH(s, W)
+        yk = hm.hv(starts[k].clone())
+        err = float((mk.extract(k, s, y) - yk).abs().max())
+        assert err <= 1e-13 * float(yk.abs().max()), (k, err)
+        a, b = hm.lanczos(starts[k], 30)
+        hm.close()
+        assert np.abs(got[k][0] - a).max() <= 1e-10 * max(1.0, np.abs(a).max()), k
+        assert np.abs(got[k][1] - b).max() <= 1e-10 * max(1.0, np.abs(a).max()), k
