@@ -124,6 +124,15 @@ __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, 
     walk_row<4>(s, T, w, f, sink);
 }
 
+// present[n] = 1 for every orbit size n in the basis.
+__global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __restrict__ orbit, int64_t D, int N,
+                                                            int* present) {
+    for (int64_t f = int64_t(blockIdx.x) * BS + threadIdx.x; f < D; f += int64_t(gridDim.x) * BS) {
+        const int n = orbit[f];
+        if (n <= N && !present[n]) present[n] = 1;
+    }
+}
+
 // Order each row's stored elements by column (ties by value index and sign),
 // so that the lanes of a wavefront reading slot k gather from nearby columns.
 // One wavefront per 64-row slice, one lane per row: a row of up to SORT_CAP
@@ -392,11 +401,28 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
                 for (int p = 0; p < N; ++p)
                     for (int nj = 0; nj <= N; ++nj) map[size_t((t * N + p) * (N + 1) + nj)] = uint16_t(t * 32 + p);
         }
+        // orbit sizes a target can have: those of the basis states (N for generic blocks); values of other
+        // sizes are never addressed, and leaving them out keeps the table small enough for 4-byte elements
+        std::vector<int> nj_present(size_t(N) + 1, 0);
+        nj_present[size_t(N)] = 1;
+        if (!coded && s->D > 0) {
+            int* d_present = dev_alloc<int>(size_t(N) + 1);
+            tmp.push_back(d_present);
+            ED_HIP(hipMemsetAsync(d_present, 0, (size_t(N) + 1) * sizeof(int), st));
+            hipLaunchKernelGGL(hm_orbit_sizes_kernel, dim3(unsigned(std::min<int64_t>((s->D + BS - 1) / BS, 4096))),
+                               dim3(BS), 0, st, s->d_orbit, s->D, N, d_present);
+            ED_HIP(hipGetLastError());
+            std::vector<int> present(size_t(N) + 1, 0);
+            ED_HIP(hipMemcpyAsync(present.data(), d_present, present.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+            for (int n = 0; n <= N; ++n) nj_present[size_t(n)] |= present[size_t(n)];
+        }
         for (int t = 0; t < 4 && !coded; ++t) {
             if (!(m->w.active & (1 << t))) continue;
             const double st_ = t < 2 ? 1.0 : 0.5;
             for (int p = 0; p < N; ++p)
                 for (int nj = 0; nj <= N; ++nj) {
+                    if (!nj_present[size_t(nj)]) continue;
                     const int idx = p * (N + 1) + nj;
                     const double ar = s->amp[2 * idx], ai = s->amp[2 * idx + 1];
                     map[size_t((t * N + p) * (N + 1) + nj)] =

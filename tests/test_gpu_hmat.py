@@ -164,3 +164,19 @@ def test_stored_tuner_keeps_results(torch):
         assert torch.equal(y0, y2), cfg
     with pytest.raises(ValueError):
         m.set_config(1, 5, False)
+
+
+def test_momentum_sectors_of_the_2x12_ladder_pack(torch):
+    """Only the orbit sizes present in the basis get values (2 holes on 2x12: every orbit has 24 states), so a
+    sector with complex phases (k = (1,0)) keeps 4-byte elements; its product still matches the matrix-free one."""
+    from exactdiag_amd.sector import Sector
+
+    s = Sector(12, 2, 11, 1, 0)
+    m = s.store([1.0, 1.0, 0.4, 0.4])
+    print("2x12 k=(1,0): values", m.n_values, "element bytes", m.element_bytes, flush=True)
+    assert m.element_bytes == 4
+    x = _rand(torch, s.dim, 5)
+    y_mf = s.hv([1.0, 1.0, 0.4, 0.4], x)
+    y_st = m.hv(x)
+    assert float((y_st - y_mf).abs().max()) <= 1e-13 * float(y_mf.abs().max())
+    m.close()
