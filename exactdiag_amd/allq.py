@@ -135,6 +135,7 @@ def run_allq(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), precision=0, limited
 # -- time to all-q S(q, w) at a given accuracy ----------------------------------
 
 REFERENCE_EIG_TOL = 2e-7  # the reference's ARPACK tolerance (general/lanczos_diagonalization.py:89)
+GS_STEPS = 100  # first-pass length of the fast forms' two-pass ground state (restarts cover slower systems)
 LAST_FAST_TIMINGS = {}  # seconds per stage of the last allq_fast run
 
 
@@ -227,7 +228,7 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
     if two_pass_gs:
         from exactdiag_amd.forms import two_pass_ground_state
 
-        e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, tol=tol)
+        e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol)
     else:
         e0, psi = _gs(hm0.hv, s0.dim, tol)
     hm0.close()
@@ -277,7 +278,7 @@ def allq_batched(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, to
     t0 = time.perf_counter()
     s0 = Sector(L, h, nd, k0[0], k0[1])
     hm0 = s0.store(weights)
-    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, tol=tol)
+    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol)
     hm0.close()
     _sync()
     tm["ground_state"] = time.perf_counter() - t0
@@ -325,7 +326,7 @@ def allq_multik(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol
     t = lap("bases", t)
     hm0 = s0.store(weights)
     t = lap("stored_h", t)
-    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, tol=tol)
+    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol)
     hm0.close()
     t = lap("ground_state", t)
     qs = distinct_qs(L, limited)

@@ -32,12 +32,44 @@ def _sync():
     torch.cuda.synchronize()
 
 
+def _shortest_converged_prefix(a, b, tol, stride=8):
+    """Lowest eigenvector y of the shortest leading k-step tridiagonal whose Ritz residual estimate
+    b_k |y_{k-1}| is below tol / 4: prefixes every `stride` steps, then bisection inside the first stride that
+    converges (the estimate falls steadily until orthogonality is lost, so the scan runs forward); the whole
+    tridiagonal when no prefix converges."""
+    n = len(a)
+
+    def lowest(k):
+        if k == 1:
+            return np.array([a[0]]), np.ones((1, 1))
+        return eigh_tridiagonal(a[:k], b[1:k], select="i", select_range=(0, 0))
+
+    def converged(k):
+        th, Y = lowest(k)
+        return abs(b[k] * Y[-1, 0]) <= 0.25 * tol * max(1.0, abs(th[0]))
+
+    prev = 0
+    for k in range(stride, n, stride):  # prefixes k < n have b[k]
+        if converged(k):
+            lo, hi = prev + 1, k
+            while lo < hi:
+                mid = (lo + hi) // 2
+                if converged(mid):
+                    hi = mid
+                else:
+                    lo = mid + 1
+            return lowest(hi)[1][:, 0]
+        prev = k
+    return lowest(n)[1][:, 0]
+
+
 def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=30, seed=0):
     """Lowest eigenpair of a stored H by restarted two-pass Lanczos in the given precision.
 
-    Pass one gives the coefficients, the tridiagonal's lowest eigenvector y; pass two reruns the recurrence and
-    accumulates x = sum_n y_n v_n in FP64; restart from x until ||H x - E x|| <= tol * max(1, |E|), E the FP64
-    Rayleigh quotient.  Returns (E, x, restarts, residual).
+    Pass one gives the coefficients; the shortest leading tridiagonal whose Ritz residual estimate meets the
+    tolerance gives the lowest eigenvector y; pass two reruns that many steps of the recurrence and accumulates
+    x = sum_n y_n v_n in FP64; restart from x until ||H x - E x|| <= tol * max(1, |E|), E the FP64 Rayleigh
+    quotient.  Returns (E, x, restarts, residual).
     """
     import torch
 
@@ -46,11 +78,7 @@ def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=3
     res = np.inf
     for restart in range(max_restarts):
         a, b = hm.lanczos(v, min(nsteps, dim), precision=precision)
-        if len(a) == 1:
-            y = np.ones(1)
-        else:
-            _, Y = eigh_tridiagonal(a, b[1:], select="i", select_range=(0, 0))
-            y = Y[:, 0]
+        y = _shortest_converged_prefix(a, b, tol)
         x = hm.lanczos_ritz(v, y, precision=precision)
         krylov.vec_scale(x, 1.0 / krylov.vec_norm(x))
         hx, e = hm.hv(x, with_dot=True)
