@@ -238,3 +238,35 @@ def test_2x10_all_q(torch):
         hm.close()
         assert np.abs(got[k][0] - a).max() <= 1e-10 * max(1.0, np.abs(a).max()), k
         assert np.abs(got[k][1] - b).max() <= 1e-10 * max(1.0, np.abs(a).max()), k
+
+
+@pytest.mark.parametrize("L,h,nd", [(2, 0, 2), (3, 0, 3), (4, 0, 4), (2, 4, 0), (3, 5, 0), (4, 1, 3), (5, 3, 1),
+                                    (6, 4, 4)])
+def test_small_and_edge_ladders(torch, L, h, nd):
+    """No holes (exchange only), all holes (one state, no elements), one electron, odd electron counts: every
+    slot's product and 20 Lanczos steps against its own sector's stored H."""
+    from exactdiag_amd.sector import StoredH
+
+    u, secs, mk = _setup(L, h, nd)
+    print("ladder", L, h, nd, "union", u.dim, "sectors", [s.dim for s in secs], flush=True)
+    assert u.dim >= max(s.dim for s in secs)
+    x = mk.block()
+    xs = []
+    for k, s in enumerate(secs):
+        xs.append(_rand(torch, s.dim, 300 + k))
+        mk.attach(k, s, xs[-1], x)
+    y = mk.hv(x)
+    got = mk.lanczos(x, 20)
+    for k, s in enumerate(secs):
+        if s.dim == 0:
+            assert len(got[k][0]) == 0
+            continue
+        hm = StoredH(s, W)
+        yk = hm.hv(xs[k].clone())
+        err = float((mk.extract(k, s, y) - yk).abs().max())
+        assert err <= 1e-13 * max(1.0, float(yk.abs().max())), (k, err)
+        a, b = hm.lanczos(xs[k], 20)
+        hm.close()
+        assert len(got[k][0]) == len(a), (k, len(got[k][0]), len(a))
+        assert np.allclose(got[k][0], a, atol=1e-10, rtol=0), k
+        assert np.allclose(got[k][1], b, atol=1e-10, rtol=0), k
