@@ -33,7 +33,7 @@ EXPORTED = [
     "ed_position_op_columns", "ed_position_op_apply", "ed_position_expectation", "ed_position_correlations",
     "ed_csr_from_position_operator",
     "ed_sector_create_union", "ed_mk_create", "ed_mk_destroy", "ed_mk_info", "ed_mk_attach", "ed_mk_extract",
-    "ed_mk_hv", "ed_mk_lanczos_run", "ed_mk_set_config",
+    "ed_mk_hv", "ed_mk_lanczos_run", "ed_mk_set_config", "ed_mk_lanczos_ritz", "ed_mk_residual_norms", "ed_mk_scale",
 ]
 
 _vp = ct.c_void_p
@@ -117,6 +117,9 @@ _SIGNATURES = {
     "ed_mk_hv": (ct.c_int, [_vp, _vp, _vp, _vp, ct.c_int, _vp]),
     "ed_mk_lanczos_run": (ct.c_int, [_vp, _vp, ct.c_int, _dp, _dp, _ip, ct.c_int, _vp]),
     "ed_mk_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
+    "ed_mk_lanczos_ritz": (ct.c_int, [_vp, _vp, ct.c_int, _dp, _vp, ct.c_int, _vp]),
+    "ed_mk_residual_norms": (ct.c_int, [_vp, _vp, _vp, _dp, _dp, _vp]),
+    "ed_mk_scale": (ct.c_int, [_vp, _vp, _dp, _vp]),
 }
 
 _lib = None

@@ -363,9 +363,10 @@ int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, i
 // Stored structure of sector s.  coded = 0: each element's code indexes the
 // exact unique values (the product's table).  coded = 1: an element's code is
 // term * 32 + p (the term and the group element mapping its target to the
-// representative), the sign is kept and the value table holds 128 unused
-// entries followed by the diagonal values; the momentum phase and orbit-size
-// factors are left to the caller (the multi-momentum product).
+// representative), the sign is kept, a row's diagonal code is
+// jl_count * (L + 1) + jr_count (its antiparallel leg and rung bonds) and the
+// value table holds 128 unused entries; values, momentum phases and
+// orbit-size factors are left to the caller (the multi-momentum product).
 ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t st) {
     ED_REQUIRE(s && s->D >= 0 && w, ED_ERR_ARG, "bad stored-matrix arguments");
     auto* m = new ed_hmat();
@@ -436,7 +437,7 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
                 C2 d{0.0, 0.0};
                 if (m->w.active & 4) d = cadd(d, cmul(m->w.w[2], C2{-0.5 * a, 0.0}));
                 if (m->w.active & 8) d = cadd(d, cmul(m->w.w[3], C2{-0.5 * b, 0.0}));
-                dmap[size_t(a * (L + 1) + b)] = uint16_t(intern(d));
+                dmap[size_t(a * (L + 1) + b)] = coded ? uint16_t(a * (L + 1) + b) : uint16_t(intern(d));
             }
         m->n_vals = int(vals.size() / 2);
         ED_REQUIRE(m->n_vals < 0x8000, ED_ERR_CAPACITY, "more than 32767 unique matrix values");

@@ -48,14 +48,13 @@ namespace {
 constexpr int MK = 16;          // slots per block
 constexpr int MBS = 256;        // threads per workgroup: 16 rows x 16 slots
 constexpr int MK_CODES = 128;   // term * 32 + p
-constexpr int MK_MAX_DIAG = 1024;
 constexpr int MK_WGS = 2048;    // workgroups of the product and vector kernels (multiple of 8)
 constexpr int RBS = 1024;
 
 struct MkTabs {
-    const double* q;     // [MK_CODES][MK] complex
-    const double* vals;  // diagonal values, indexed by the row's diagonal code
-    int n_vals;
+    const double* q;   // [MK_CODES][MK] complex
+    const double* wd;  // [MK][4]: slot's jl and jr weights (re, im), zero when the term is off
+    int L1;            // L + 1: diagonal code = jl_count * (L + 1) + jr_count
 };
 
 template <class VT>
@@ -132,15 +131,13 @@ __global__ void __launch_bounds__(64 * NW) mk_spmv_kernel(StoredView m, int64_t 
                                                           const uint16_t* __restrict__ mask, const VT* __restrict__ x,
                                                           VT* __restrict__ y, double* __restrict__ partial) {
     constexpr int NT = 64 * NW, RW = 4 * NW;
-    extern __shared__ __align__(16) unsigned char smem[];
-    double2* Q = reinterpret_cast<double2*>(smem);  // [MK_CODES * MK], then the diagonal values
-    double2* V = Q + MK_CODES * MK;
+    __shared__ double2 Q[MK_CODES * MK];
     __shared__ double red[NW][MK][2];
     for (int i = threadIdx.x; i < MK_CODES * MK; i += NT) Q[i] = double2{tb.q[2 * i], tb.q[2 * i + 1]};
-    for (int i = threadIdx.x; i < tb.n_vals; i += NT) V[i] = double2{tb.vals[2 * i], tb.vals[2 * i + 1]};
     __syncthreads();
     const int k = threadIdx.x & (MK - 1);
     const int r = threadIdx.x >> 4;
+    const C2 wjl{tb.wd[4 * k], tb.wd[4 * k + 1]}, wjr{tb.wd[4 * k + 2], tb.wd[4 * k + 3]};
     double dre = 0.0, dim = 0.0;
     const GroupRange g = xcd_groups((D + RW - 1) / RW);
     for (int64_t grp = g.lo + g.first; grp < g.hi; grp += g.step) {
@@ -177,8 +174,12 @@ __global__ void __launch_bounds__(64 * NW) mk_spmv_kernel(StoredView m, int64_t 
             acc = cadd(acc, cmul(C2{sg * q.x, sg * q.y}, xv));
         }
         const C2 xf = ldv(x, f * MK + k);
-        const double2 d = V[__builtin_nontemporal_load(m.diag + f)];
-        C2 yf = cadd(acc, cmul(C2{d.x, d.y}, xf));
+        // diagonal: -1/2 per antiparallel bond times the slot's jl / jr weight, as the single-sector table has it
+        const int dc = __builtin_nontemporal_load(m.diag + f);
+        const int na = dc / tb.L1, nb = dc - na * tb.L1;
+        C2 d = cadd(C2{0.0, 0.0}, cmul(wjl, C2{-0.5 * na, 0.0}));
+        d = cadd(d, cmul(wjr, C2{-0.5 * nb, 0.0}));
+        C2 yf = cadd(acc, cmul(d, xf));
         if (!((mask[f] >> k) & 1u)) yf = C2{0.0, 0.0};
         stv(y, f * MK + k, yf);
         const double nf = double(m.orbit[f]);
@@ -323,12 +324,14 @@ __global__ void __launch_bounds__(RBS) mk_lz_a_kernel(MkLz lz, const double* par
 }
 
 // w = s_n y - a_n s_n u - b_n s_{n-1} u_prev into u_prev, with its weighted
-// squared norm per slot; a stopped slot is set to zero.
+// squared norm per slot; a stopped slot is set to zero.  With acc (second
+// pass of two-pass Lanczos), acc += ritz[k][n] v_n in FP64.
 template <class VT>
 __global__ void __launch_bounds__(MBS) mk_lz_update_kernel(MkLz lz, int step, int64_t D,
                                                            const uint8_t* __restrict__ orbit,
                                                            const VT* __restrict__ y, const VT* __restrict__ u,
-                                                           VT* __restrict__ up, double* __restrict__ partial) {
+                                                           VT* __restrict__ up, double* __restrict__ partial,
+                                                           double2* __restrict__ acc, const double* __restrict__ ritz) {
     __shared__ double red[4][MK][2];
     const int k = threadIdx.x & (MK - 1), r = threadIdx.x >> 4;
     const int S = lz.nsteps + 1;
@@ -337,6 +340,7 @@ __global__ void __launch_bounds__(MBS) mk_lz_update_kernel(MkLz lz, int step, in
     const double an = stopped ? 0.0 : lz.a[k * lz.nsteps + step];
     const double bn = stopped ? 0.0 : lz.b[k * S + step];
     const double snm = (stopped || step == 0) ? 0.0 : lz.s[k * S + step - 1];
+    const double c = (acc && !stopped) ? ritz[k * lz.nsteps + step] * sn : 0.0;
     double n2 = 0.0, z = 0.0;
     const GroupRange g = xcd_groups((D + 15) / 16);
     for (int64_t grp = g.lo + g.first; grp < g.hi; grp += g.step) {
@@ -344,6 +348,10 @@ __global__ void __launch_bounds__(MBS) mk_lz_update_kernel(MkLz lz, int step, in
         if (f >= D) continue;
         const int64_t at = f * MK + k;
         const C2 yv = ldv(y, at), uv = ldv(u, at), pv = ldv(up, at);
+        if (acc) {
+            const double2 av = acc[at];
+            acc[at] = double2{av.x + c * uv.re, av.y + c * uv.im};
+        }
         const double vr = sn * uv.re, vi = sn * uv.im;
         const double wr = sn * yv.re - an * vr - bn * (snm * pv.re);
         const double wi = sn * yv.im - an * vi - bn * (snm * pv.im);
@@ -373,6 +381,46 @@ __global__ void __launch_bounds__(RBS) mk_lz_b_kernel(MkLz lz, const double* par
     }
 }
 
+// partial[wg][k] = sum_f n_f |y_k(f) - e_k x_k(f)|^2 (complex128 blocks; y null: |x|^2).
+__global__ void __launch_bounds__(MBS) mk_resid_kernel(int64_t D, const uint8_t* __restrict__ orbit,
+                                                       const double2* __restrict__ x, const double2* __restrict__ y,
+                                                       const double* __restrict__ e, double* __restrict__ partial) {
+    __shared__ double red[4][MK][2];
+    const int k = threadIdx.x & (MK - 1), r = threadIdx.x >> 4;
+    const double ek = e ? e[k] : 0.0;
+    double n2 = 0.0, z = 0.0;
+    const GroupRange g = xcd_groups((D + 15) / 16);
+    for (int64_t grp = g.lo + g.first; grp < g.hi; grp += g.step) {
+        const int64_t f = grp * 16 + r;
+        if (f >= D) continue;
+        const double2 xv = x[f * MK + k];
+        double rr = xv.x, ri = xv.y;
+        if (y) {
+            const double2 yv = y[f * MK + k];
+            rr = yv.x - ek * xv.x;
+            ri = yv.y - ek * xv.y;
+        }
+        n2 += double(orbit[f]) * (rr * rr + ri * ri);
+    }
+    slot_sum(n2, z, red);
+    if (threadIdx.x < MK) {
+        partial[2 * (int64_t(blockIdx.x) * MK + threadIdx.x)] = n2;
+        partial[2 * (int64_t(blockIdx.x) * MK + threadIdx.x) + 1] = 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(MBS) mk_scale_kernel(int64_t D, double2* __restrict__ x, const double* __restrict__ s) {
+    const int k = threadIdx.x & (MK - 1), r = threadIdx.x >> 4;
+    const double sk = s[k];
+    const GroupRange g = xcd_groups((D + 15) / 16);
+    for (int64_t grp = g.lo + g.first; grp < g.hi; grp += g.step) {
+        const int64_t f = grp * 16 + r;
+        if (f >= D) continue;
+        const double2 v = x[f * MK + k];
+        x[f * MK + k] = double2{sk * v.x, sk * v.y};
+    }
+}
+
 }  // namespace
 }  // namespace ed
 
@@ -387,12 +435,13 @@ struct ed_mk {
     int unroll = 4;  // gathers in flight per lane (4 or 8)
     int64_t D = 0;
     double* d_q = nullptr;
+    double* d_wd = nullptr;  // [MK][4] per-slot diagonal weights
     uint16_t* d_mask = nullptr;
     double* d_partial = nullptr;  // [MK_WGS][MK] complex
     int* d_err = nullptr;
 
     ~ed_mk() {
-        for (void* p : {(void*)d_q, (void*)d_mask, (void*)d_partial, (void*)d_err}) dev_free(p);
+        for (void* p : {(void*)d_q, (void*)d_wd, (void*)d_mask, (void*)d_partial, (void*)d_err}) dev_free(p);
         if (hm) ed_hmat_destroy(hm);
     }
 };
@@ -400,16 +449,12 @@ struct ed_mk {
 namespace ed {
 namespace {
 
-MkTabs mk_tabs(const ed_mk* m) {
-    const StoredView v = hmat_view(m->hm);
-    return MkTabs{m->d_q, v.vals, v.n_vals};
-}
+MkTabs mk_tabs(const ed_mk* m) { return MkTabs{m->d_q, m->d_wd, m->u->L + 1}; }
 
 template <class VT, bool PACKED, int U, int NW>
 void launch_mk_t(const ed_mk* m, const StoredView& v, const MkTabs& tb, const VT* x, VT* y, double* partial,
                  hipStream_t st) {
-    const size_t lds = size_t(MK_CODES * MK + tb.n_vals) * sizeof(double2);
-    hipLaunchKernelGGL((mk_spmv_kernel<VT, PACKED, U, NW>), dim3(MK_WGS), dim3(64 * NW), lds, st, v, m->D, tb,
+    hipLaunchKernelGGL((mk_spmv_kernel<VT, PACKED, U, NW>), dim3(MK_WGS), dim3(64 * NW), 0, st, v, m->D, tb,
                        m->d_mask, x, y, partial);
 }
 
@@ -447,18 +492,21 @@ void check_slot_sector(const ed_mk* m, int slot, const ed_sector* sk) {
 }
 
 template <class VT>
-void mk_lanczos_t(const ed_mk* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out, hipStream_t st) {
+void mk_lanczos_t(const ed_mk* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out, hipStream_t st,
+                  const double* ritz = nullptr, double2* acc = nullptr) {
     const int64_t D = m->D;
     const size_t nv = size_t(D) * MK;
     VT* buf[3] = {dev_alloc<VT>(nv), dev_alloc<VT>(nv), dev_alloc<VT>(nv)};
     MkLz lz{dev_alloc<double>(size_t(MK) * nsteps), dev_alloc<double>(size_t(MK) * (nsteps + 1)),
             dev_alloc<double>(size_t(MK) * (nsteps + 1)), dev_alloc<int>(2 * MK), nsteps};
+    double* d_ritz = nullptr;
     auto cleanup = [&] {
         for (auto* p : buf) dev_free(p);
         dev_free(lz.a);
         dev_free(lz.b);
         dev_free(lz.s);
         dev_free(lz.flag);
+        dev_free(d_ritz);
     };
     try {
         VT* u = buf[0];
@@ -468,6 +516,12 @@ void mk_lanczos_t(const ed_mk* m, const double* d_v0, int nsteps, double* a, dou
         ED_HIP(hipMemsetAsync(up, 0, nv * sizeof(VT), st));
         ED_HIP(hipMemsetAsync(lz.a, 0, size_t(MK) * nsteps * sizeof(double), st));
         ED_HIP(hipMemsetAsync(lz.b, 0, size_t(MK) * (nsteps + 1) * sizeof(double), st));
+        if (acc) {
+            std::vector<double> r(size_t(MK) * nsteps, 0.0);
+            std::copy(ritz, ritz + size_t(m->n_slots) * nsteps, r.begin());
+            d_ritz = dev_upload(r, st);
+            ED_HIP(hipMemsetAsync(acc, 0, nv * sizeof(double2), st));
+        }
         hipLaunchKernelGGL((mk_copy_in_kernel<VT>), dim3(MK_WGS), dim3(MBS), 0, st, D, orbit, d_v0, u, m->d_partial);
         hipLaunchKernelGGL(mk_lz_init_kernel, dim3(MK), dim3(RBS), 0, st, lz, m->d_partial, int64_t(MK_WGS));
         ED_HIP(hipGetLastError());
@@ -476,7 +530,7 @@ void mk_lanczos_t(const ed_mk* m, const double* d_v0, int nsteps, double* a, dou
             launch_mk<VT>(m, u, y, m->d_partial, st);
             hipLaunchKernelGGL(mk_lz_a_kernel, dim3(MK), dim3(RBS), 0, st, lz, m->d_partial, int64_t(MK_WGS), n);
             hipLaunchKernelGGL((mk_lz_update_kernel<VT>), dim3(MK_WGS), dim3(MBS), 0, st, lz, n, D, orbit, y, u, up,
-                               m->d_partial);
+                               m->d_partial, acc, d_ritz);
             hipLaunchKernelGGL(mk_lz_b_kernel, dim3(MK), dim3(RBS), 0, st, lz, m->d_partial, int64_t(MK_WGS), n);
             ED_HIP(hipGetLastError());
             std::swap(u, up);  // w (u_{n+1}) now lives in u
@@ -489,14 +543,14 @@ void mk_lanczos_t(const ed_mk* m, const double* d_v0, int nsteps, double* a, dou
             }
         }
         ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
-        for (int k = 0; k < m->n_slots; ++k) {
+        for (int k = 0; k < m->n_slots && a && b; ++k) {
             ED_HIP(hipMemcpyAsync(a + size_t(k) * nsteps, lz.a + size_t(k) * nsteps, size_t(nsteps) * sizeof(double),
                                   hipMemcpyDeviceToHost, st));
             ED_HIP(hipMemcpyAsync(b + size_t(k) * (nsteps + 1), lz.b + size_t(k) * (nsteps + 1),
                                   size_t(nsteps + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
         }
         ED_HIP(hipStreamSynchronize(st));
-        for (int k = 0; k < m->n_slots; ++k) n_out[k] = flag[2 * k + 1];
+        for (int k = 0; k < m->n_slots && n_out; ++k) n_out[k] = flag[2 * k + 1];
     } catch (...) {
         cleanup();
         throw;
@@ -529,33 +583,49 @@ int ed_mk_create(const ed_sector* u, const double* w, int n_slots, const int* kx
                 m->kx[k] = kx[k];
                 m->ky[k] = ky[k];
             }
-            m->hm = hmat_build(u, w, 1, st);
-            const StoredView v = hmat_view(m->hm);
-            ED_REQUIRE(v.n_vals <= MK_MAX_DIAG, ED_ERR_CAPACITY, "too many diagonal values");
+            // the structure holds every term that some slot uses
+            std::vector<HvWeights> hw(static_cast<size_t>(n_slots));
+            double wb[8] = {0.0};
+            for (int k = 0; k < n_slots; ++k) {
+                hw[size_t(k)] = make_weights(w + 8 * k);
+                for (int t = 0; t < 4; ++t)
+                    if (hw[size_t(k)].active & (1 << t)) wb[2 * t] = 1.0;
+            }
+            m->hm = hmat_build(u, wb, 1, st);
             // Q[t * 32 + p][k] = w_t s_t conj(F_k[p]), F_k[p] built as for the sector's amplitude table
-            const HvWeights hw = make_weights(w);
             using cd = std::complex<double>;
             const cd i2pi(0.0, 2.0 * std::numbers::pi);
             std::vector<double> q(size_t(MK_CODES) * MK * 2, 0.0);
+            std::vector<double> wd(size_t(MK) * 4, 0.0);
             for (int k = 0; k < n_slots; ++k) {
+                const HvWeights& h = hw[size_t(k)];
                 const cd mirror_exp = std::exp(-i2pi * cd(double(ky[k]) / double(2)));
                 const cd rung_exp = std::exp(-i2pi * cd(double(kx[k]) / double(L)));
                 std::vector<cd> E(size_t(L) + 1);
                 E[0] = 1;
                 for (int mm = 1; mm <= L; ++mm) E[size_t(mm)] = E[size_t(mm - 1)] * rung_exp;
                 for (int t = 0; t < 4; ++t) {
-                    if (!(hw.active & (1 << t))) continue;
+                    if (!(h.active & (1 << t))) continue;
                     const double st_ = t < 2 ? 1.0 : 0.5;
                     for (int p = 0; p < N; ++p) {
                         const int mm = p >> 1, bb = p & 1;
                         const cd F = bb ? E[size_t(mm)] * mirror_exp : E[size_t(mm)];
-                        const C2 val = cmul(hw.w[t], C2{st_ * F.real(), -(st_ * F.imag())});
+                        const C2 val = cmul(h.w[t], C2{st_ * F.real(), -(st_ * F.imag())});
                         const size_t at = size_t((t * 32 + p) * MK + k);
                         q[2 * at] = val.re;
                         q[2 * at + 1] = val.im;
                     }
                 }
+                if (h.active & 4) {
+                    wd[size_t(4 * k)] = h.w[2].re;
+                    wd[size_t(4 * k + 1)] = h.w[2].im;
+                }
+                if (h.active & 8) {
+                    wd[size_t(4 * k + 2)] = h.w[3].re;
+                    wd[size_t(4 * k + 3)] = h.w[3].im;
+                }
             }
+            m->d_wd = dev_upload(wd, st);
             m->d_q = dev_upload(q, st);
             m->d_mask = dev_alloc<uint16_t>(size_t(std::max<int64_t>(m->D, 1)));
             ED_HIP(hipMemsetAsync(m->d_mask, 0, size_t(std::max<int64_t>(m->D, 1)) * 2, st));
@@ -668,6 +738,78 @@ int ed_mk_lanczos_run(const ed_mk* m, const double* d_v0, int nsteps, double* a,
             mk_lanczos_t<double2>(m, d_v0, nsteps, a, b, n_out, st);
         else
             mk_lanczos_t<float2>(m, d_v0, nsteps, a, b, n_out, st);
+    });
+}
+
+int ed_mk_lanczos_ritz(const ed_mk* m, const double* d_v0, int nsteps, const double* ritz, double* d_x, int precision,
+                       void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && ritz, ED_ERR_ARG, "null argument");
+        ED_REQUIRE(nsteps >= 1, ED_ERR_ARG, "nsteps must be positive");
+        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 (c128) or 1 (c64)");
+        if (m->D == 0) return;
+        ED_REQUIRE(d_v0 && d_x, ED_ERR_ARG, "null block");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        double2* acc = reinterpret_cast<double2*>(d_x);
+        if (precision == 0)
+            mk_lanczos_t<double2>(m, d_v0, nsteps, nullptr, nullptr, nullptr, st, ritz, acc);
+        else
+            mk_lanczos_t<float2>(m, d_v0, nsteps, nullptr, nullptr, nullptr, st, ritz, acc);
+    });
+}
+
+int ed_mk_residual_norms(const ed_mk* m, const double* d_x, const double* d_hx, const double* e, double* out,
+                         void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && out, ED_ERR_ARG, "null argument");
+        for (int k = 0; k < m->n_slots; ++k) out[k] = 0.0;
+        if (m->D == 0) return;
+        ED_REQUIRE(d_x, ED_ERR_ARG, "null block");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        double* d_e = nullptr;
+        double* d_out = nullptr;
+        try {
+            if (d_hx && e) {
+                std::vector<double> ev(size_t(MK), 0.0);
+                std::copy(e, e + m->n_slots, ev.begin());
+                d_e = dev_upload(ev, st);
+            }
+            d_out = dev_alloc<double>(2 * MK);
+            hipLaunchKernelGGL(mk_resid_kernel, dim3(MK_WGS), dim3(MBS), 0, st, m->D, m->u->d_orbit,
+                               reinterpret_cast<const double2*>(d_x), reinterpret_cast<const double2*>(d_hx), d_e,
+                               m->d_partial);
+            hipLaunchKernelGGL(mk_reduce_kernel, dim3(MK), dim3(RBS), 0, st, m->d_partial, int64_t(MK_WGS), d_out);
+            ED_HIP(hipGetLastError());
+            double h[2 * MK];
+            ED_HIP(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+            for (int k = 0; k < m->n_slots; ++k) out[k] = h[2 * k];
+        } catch (...) {
+            dev_free(d_e);
+            dev_free(d_out);
+            throw;
+        }
+        dev_free(d_e);
+        dev_free(d_out);
+    });
+}
+
+int ed_mk_scale(const ed_mk* m, double* d_x, const double* s, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m && s, ED_ERR_ARG, "null argument");
+        if (m->D == 0) return;
+        ED_REQUIRE(d_x, ED_ERR_ARG, "null block");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        std::vector<double> sv(size_t(MK), 0.0);
+        std::copy(s, s + m->n_slots, sv.begin());
+        double* d_s = dev_upload(sv, st);
+        hipLaunchKernelGGL(mk_scale_kernel, dim3(MK_WGS), dim3(MBS), 0, st, m->D, reinterpret_cast<double2*>(d_x),
+                           d_s);
+        const hipError_t e = hipGetLastError();
+        const hipError_t e2 = hipStreamSynchronize(st);
+        dev_free(d_s);
+        ED_HIP(e);
+        ED_HIP(e2);
     });
 }
 

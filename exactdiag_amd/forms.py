@@ -153,6 +153,107 @@ def _spectral_excitations(L, h, nd, k0):
     return out
 
 
+def _groups(items, size=15):
+    """Items in groups of at most `size`, as even as possible."""
+    n = max(1, -(-len(items) // size))
+    per = -(-len(items) // n)
+    return [items[i: i + per] for i in range(0, len(items), per)]
+
+
+def _batching_multik(L, h, nd, Js, inst, sectors, energies_batched, batched_spectra, nsteps, ws, eta, tol, log):
+    """The batching form's two steps with the instances of one ladder in multi-momentum products (per-slot
+    momentum and couplings, at most 16 slots per product): lowest eigenpairs of all (J, k) instances, then
+    A(k, w) for every k from each coupling's ground state.  Compared with the batched engine's results."""
+    import torch
+
+    from exactdiag_amd.multik import MultiK
+
+    out = {}
+    _sync()
+    t0 = time.perf_counter()
+    u = Sector.union(L, h, nd)
+    energies = np.empty(len(inst))
+    psis = {}
+    best = {J: min((i for i, (JJ, _) in enumerate(inst) if JJ == J), key=lambda i: energies_batched[i]) for J in Js}
+    for grp in _groups(list(range(len(inst)))):
+        mk = MultiK(u, [[1.0, 1.0, inst[i][0], inst[i][0]] for i in grp], [inst[i][1] for i in grp])
+        v0 = mk.block()
+        for slot, i in enumerate(grp):
+            s = sectors[inst[i][1]]
+            mk.attach(slot, s, krylov.device_start_vector(s.dim, i), v0)
+        e, x, _, _ = mk.lowest(v0, tol=tol)
+        for slot, i in enumerate(grp):
+            energies[i] = e[slot]
+            if i in best.values():
+                psis[i] = mk.extract(slot, sectors[inst[i][1]], x).clone()
+        del v0, x
+        mk.close()
+    _sync()
+    out["gs_s"] = time.perf_counter() - t0
+    out["gs_max_abs_diff_vs_batched"] = float(np.max(np.abs(energies - energies_batched)))
+    log(f"  multi-momentum lowest eigenpairs: {out['gs_s']:.2f}s, max |dE| vs batched "
+        f"{out['gs_max_abs_diff_vs_batched']:.1e}")
+    # spectra: the excitations of all couplings, grouped by final ladder
+    t0 = time.perf_counter()
+    jobs = {}
+    for J in Js:
+        i0 = best[J]
+        k0, e0, exc = batched_spectra[J][:3]
+        assert inst[i0][1] == k0
+        for n_ex, ex in enumerate(exc):
+            jobs.setdefault((ex.sector.h, ex.sector.nd), []).append((J, n_ex, ex, i0))
+    got = _multik_spectra(L, jobs, sectors, inst, psis, batched_spectra, nsteps, ws, eta)
+    _sync()
+    out["spectra_s"] = time.perf_counter() - t0
+    diffs = [spectral_l1(got[(J, n)], g, ws) for J in Js for n, g in enumerate(batched_spectra[J][3])
+             if np.abs(g).sum() > 0]
+    out["spectra_max_l1_vs_batched"] = float(max(diffs))
+    out["spectra_median_l1_vs_batched"] = float(np.median(diffs))
+    out["n_spectra"] = len(got)
+    # the same spectra from the batched engine's ground states: the difference of the two Lanczos paths alone
+    same = _multik_spectra(L, jobs, sectors, inst, {best[J]: batched_spectra[J][4] for J in Js}, batched_spectra,
+                           nsteps, ws, eta)
+    diffs2 = [spectral_l1(same[(J, n)], g, ws) for J in Js for n, g in enumerate(batched_spectra[J][3])
+              if np.abs(g).sum() > 0]
+    out["spectra_same_gs_max_l1_vs_batched"] = float(max(diffs2))
+    out["spectra_same_gs_median_l1_vs_batched"] = float(np.median(diffs2))
+    log(f"  multi-momentum A(k,w): {out['spectra_s']:.2f}s, L1 vs batched median {out['spectra_median_l1_vs_batched']:.1e}"
+        f" max {out['spectra_max_l1_vs_batched']:.1e}; from the batched ground states median "
+        f"{out['spectra_same_gs_median_l1_vs_batched']:.1e} max {out['spectra_same_gs_max_l1_vs_batched']:.1e}")
+    u.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def _multik_spectra(L, jobs, sectors, inst, psis, batched_spectra, nsteps, ws, eta):
+    """Spectra of the grouped excitations (jobs: final ladder -> [(J, index, excitation, ground-state
+    instance)]) with multi-momentum products; returns {(J, index): S(w)}."""
+    from exactdiag_amd.multik import MultiK
+
+    got = {}
+    for (hf, ndf), items in jobs.items():
+        uf = Sector.union(L, hf, ndf)
+        for grp in _groups(items):
+            mk = MultiK(uf, [[1.0, 1.0, J, J] for J, *_ in grp], [(ex.sector.kx, ex.sector.ky) for _, _, ex, _ in grp])
+            v0 = mk.block()
+            norm2 = []
+            for slot, (J, n_ex, ex, i0) in enumerate(grp):
+                phi = sectors[inst[i0][1]].op_apply(ex.sector, ex.op, ex.qx, ex.qy, psis[i0])
+                n2 = krylov.vec_norm(phi) ** 2 if ex.sector.dim else 0.0
+                if n2 >= 1e-16:
+                    mk.attach(slot, ex.sector, phi, v0)
+                norm2.append(n2)
+            coeffs = mk.lanczos(v0, nsteps)
+            for (J, n_ex, ex, i0), n2, (a, b) in zip(grp, norm2, coeffs):
+                e0 = batched_spectra[J][1]
+                got[(J, n_ex)] = (np.zeros(len(ws)) if n2 < 1e-16 or len(a) == 0
+                                  else cf_eval(a, b, ws, e0, eta, -n2 / np.pi))
+            del v0
+            mk.close()
+        uf.close()
+    return got
+
+
 def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspace(-6, 6, 2001), eta=0.05,
                   tol=1e-9, serial=True, stored=True, log=print):
     """All k sectors x couplings as one batch, then A(k, w) for every k as another; optionally the serial runs.
@@ -205,6 +306,7 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
         out["gs_max_abs_diff"] = float(np.max(np.abs(np.array(e_serial) - energies)))
         log(f"  serial lowest eigenpairs: {out['gs_serial_s']:.2f}s, max |dE| {out['gs_max_abs_diff']:.2e}")
     # ground state per coupling: the lowest of its sectors
+    batched_spectra = {}
     exc_params = {}
     spec_out = {}
     t_spec_b = t_spec_s = 0.0
@@ -226,6 +328,7 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
         _sync()
         t_spec_b += time.perf_counter() - t1
         spec_out[J] = {"k0": k0, "e0": e0, "n_spectra": len(exc)}
+        batched_spectra[J] = (k0, e0, exc, got, psi)
         if serial:
             t1 = time.perf_counter()
             wa = weights_array(weights[ji])
@@ -251,6 +354,8 @@ def batching_form(L=10, h=2, nd=9, Js=(0.3, 0.4, 0.5), nsteps=250, ws=np.linspac
         out["spectra_max_l1_batched_vs_serial"] = max_diff
     log(f"  A(k,w): batched {t_spec_b:.2f}s" + (f", serial {t_spec_s:.2f}s, max L1 diff {max_diff:.1e}"
                                                    if serial else ""))
+    out["multik"] = _batching_multik(L, h, nd, Js, inst, sectors, energies, batched_spectra, nsteps, ws, eta, tol,
+                                     log)
     batch.close()
     for s in list(exc_params.values()) + list(sectors.values()):
         s.close()

@@ -26,8 +26,18 @@ from exactdiag_amd.sector import Sector, _dbl, _ptr, _stream, _torch, weights_ar
 SLOTS = 16
 
 
+def _per_slot_weights(weights, n):
+    """One set of four term weights for every slot, or a list of n sets."""
+    if isinstance(weights, (list, tuple)) and len(weights) == n and all(
+            isinstance(w, (list, tuple, np.ndarray, dict)) for w in weights):
+        return np.concatenate([weights_array(w) for w in weights])
+    return np.tile(weights_array(weights), n)
+
+
 class MultiK:
-    """H of up to 16 momentum sectors (kx, ky) of one ladder, stored once over the union basis."""
+    """H of up to 16 momentum sectors (kx, ky) of one ladder, stored once over the union basis.
+
+    ``weights``: the four term weights (tl, tr, jl, jr) for every slot, or one set per slot."""
 
     def __init__(self, union: Sector, weights, momenta):
         if not getattr(union, "any_k", False):
@@ -42,7 +52,7 @@ class MultiK:
         n = len(momenta)
         kx = (ct.c_int * n)(*[m[0] for m in momenta])
         ky = (ct.c_int * n)(*[m[1] for m in momenta])
-        _, wp = _dbl(weights_array(weights))
+        _, wp = _dbl(_per_slot_weights(weights, n))
         h = ct.c_void_p()
         check(lib().ed_mk_create(union.handle, wp, n, kx, ky, ct.byref(h), _stream()))
         self._h = h
@@ -110,3 +120,70 @@ class MultiK:
         check(lib().ed_mk_lanczos_run(self._h, _ptr(v0), int(nsteps), a.ctypes.data_as(ct.POINTER(ct.c_double)),
                                       b.ctypes.data_as(ct.POINTER(ct.c_double)), cnt, int(precision), _stream()))
         return [(a[k, : cnt[k]].copy(), b[k, : cnt[k]].copy()) for k in range(n)]
+
+    def lanczos_ritz(self, v0, ritz, precision: int = 0):
+        """Second pass of two-pass Lanczos: the complex128 block sum_n ritz[k][n] v_n(k) for every slot (ritz: one
+        coefficient list per slot, zero-padded to a common length)."""
+        torch = _torch()
+        v0 = v0.to(torch.complex128).contiguous()
+        n = len(self.momenta)
+        nsteps = max(1, max(len(r) for r in ritz))
+        r = np.zeros((n, nsteps), np.float64)
+        for k, rk in enumerate(ritz):
+            r[k, : len(rk)] = rk
+        x = self.block()
+        check(lib().ed_mk_lanczos_ritz(self._h, _ptr(v0), nsteps, r.ctypes.data_as(ct.POINTER(ct.c_double)), _ptr(x),
+                                       int(precision), _stream()))
+        return x
+
+    def residual_norms2(self, x, hx=None, e=None):
+        """sum_f n_f |hx_k - e_k x_k|^2 per slot (complex128 blocks); without hx the squared norms of x."""
+        n = len(self.momenta)
+        out = np.zeros(n, np.float64)
+        ev = np.ascontiguousarray(e if e is not None else np.zeros(n), np.float64)
+        check(lib().ed_mk_residual_norms(self._h, _ptr(x), _ptr(hx) if hx is not None else None,
+                                         ev.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                         out.ctypes.data_as(ct.POINTER(ct.c_double)), _stream()))
+        return out
+
+    def scale(self, x, s):
+        """Slot k of the complex128 block x times s[k], in place."""
+        sv = np.ascontiguousarray(s, np.float64)
+        check(lib().ed_mk_scale(self._h, _ptr(x), sv.ctypes.data_as(ct.POINTER(ct.c_double)), _stream()))
+        return x
+
+    def lowest(self, v0, tol=1e-10, nsteps=120, max_restarts=30):
+        """Lowest eigenpair of every slot's sector by restarted two-pass Lanczos, all slots in one recurrence.
+
+        Pass one gives each slot's coefficients; the shortest leading tridiagonal whose Ritz residual estimate
+        meets the tolerance gives its lowest eigenvector; pass two accumulates the Ritz vectors in FP64; restart
+        from them until every slot has ||H x - E x|| <= tol * max(1, |E|).  A slot whose recurrence stops before
+        nsteps (b < 1e-8, the reference's threshold: its Krylov space is invariant to that precision) is final
+        after that pass.  Slots whose start vector is zero are skipped.  Returns (energies, block of unit
+        eigenvectors, restarts, residuals)."""
+        from exactdiag_amd.forms import _shortest_converged_prefix
+
+        n = len(self.momenta)
+        live = self.residual_norms2(v0) > 0.0
+        v = v0
+        e = np.full(n, np.nan)
+        res = np.full(n, np.inf)
+        x = v0
+        for restart in range(max_restarts):
+            coeffs = self.lanczos(v, nsteps)
+            exhausted = [len(a) < nsteps for a, _ in coeffs]
+            ritz = [(_shortest_converged_prefix(a, b, tol) if (live[k] and len(a)) else np.zeros(0))
+                    for k, (a, b) in enumerate(coeffs)]
+            x = self.lanczos_ritz(v, ritz)
+            nrm = np.sqrt(self.residual_norms2(x))
+            self.scale(x, np.where(nrm > 0, 1.0 / np.where(nrm > 0, nrm, 1.0), 0.0))
+            hx, dots = self.hv(x, with_dot=True)
+            e = np.where(live, dots.real, np.nan)
+            res = np.sqrt(self.residual_norms2(x, hx, np.where(live, dots.real, 0.0)))
+            done = [(not live[k]) or exhausted[k] or res[k] <= tol * max(1.0, abs(e[k])) for k in range(n)]
+            del hx
+            if all(done):
+                return e, x, restart, res
+            v = x
+        return e, x, max_restarts, res
+

@@ -297,7 +297,9 @@ int ed_hv_tune(ed_sector* s, const double* w, int precision, const double* d_x, 
  * image under the ladder's group (symmetry_ladder.h:113-235 without the
  * compatibility test); build its basis with ed_basis_build.
  * ed_mk_create: one stored structure over the union, slot k carrying the
- * momentum (kx[k], ky[k]).  Vectors are blocks complex[D_union][16] in scaled
+ * momentum (kx[k], ky[k]) and the term weights w[8k .. 8k+7] (tl, tr, jl, jr
+ * as re, im pairs); the union sector must outlive the matrix.  Vectors are
+ * blocks complex[D_union][16] in scaled
  * coordinates x'_f = x_f / sqrt(n_f), n_f the orbit size; slot k is masked
  * to sector k's states.
  * ed_mk_attach: write sector vector d_phi (the sector's own dense order) into
@@ -308,7 +310,14 @@ int ed_hv_tune(ed_sector* s, const double* w, int precision, const double* d_x, 
  * ed_mk_lanczos_run: plain Lanczos from the complex128 block d_v0 for all
  * slots, a[k*nsteps + n], b[k*(nsteps+1) + n] (b_0 = 0), n_out[k] = steps
  * done, each slot stopping at b < 1e-8; vectors complex128 (precision 0) or
- * complex64 with FP64 reductions (precision 1). */
+ * complex64 with FP64 reductions (precision 1).
+ * ed_mk_lanczos_ritz: the same recurrence again (second pass of two-pass
+ * Lanczos) accumulating d_x (complex128 block) = sum_n ritz[k*nsteps + n]
+ * v_n(k) for every slot in FP64.
+ * ed_mk_residual_norms: out[k] = sum_f n_f |hx_k(f) - e[k] x_k(f)|^2 for
+ * complex128 blocks (d_hx null: the squared norm of x_k); ed_mk_scale: slot k
+ * of a complex128 block times s[k].  Calls on one matrix share its scratch and
+ * must not overlap. */
 typedef struct ed_mk ed_mk;
 int ed_sector_create_union(int L, int h, int nd, ed_sector** out);
 int ed_mk_create(const ed_sector* u, const double* w, int n_slots, const int* kx, const int* ky, ed_mk** out,
@@ -324,6 +333,11 @@ int ed_mk_extract(const ed_mk* m, int slot, const ed_sector* sk, const double* d
 int ed_mk_hv(const ed_mk* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream);
 int ed_mk_lanczos_run(const ed_mk* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
                       int precision, void* stream);
+int ed_mk_lanczos_ritz(const ed_mk* m, const double* d_v0, int nsteps, const double* ritz, double* d_x, int precision,
+                       void* stream);
+int ed_mk_residual_norms(const ed_mk* m, const double* d_x, const double* d_hx, const double* e, double* out,
+                         void* stream);
+int ed_mk_scale(const ed_mk* m, double* d_x, const double* s, void* stream);
 
 /* Continued fraction of general/cython_lanczos_diagonalization.pyx:42-48
  * evaluated on the GPU: out[k] = factor * Im(1 / det(z_k)) for z_k = ws[k] +
