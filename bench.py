@@ -291,6 +291,12 @@ def run_form(form, args=None):
             line["speedup_vs_cpu"] = cpu["value"] / res["build_s"]
         for m in mats:
             m.close()
+    elif form == "allq":
+        res = allq_section()
+        line = {"metric": "time to all-q S(q,w) at <= 1e-3 spectral L1 (seconds, best form)",
+                "value": res["best"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                "dtype": "complex64", "data": "synthetic (model built from its definition)",
+                "config": {"workload": res["workload"]}, "allq": res}
     elif form == "multik":
         res = forms.multik_form(log=log)
         c64 = res["c64"]
@@ -356,9 +362,10 @@ def main():
     ap.add_argument("--no-allq", action="store_true", help="skip the all-q time-to-accuracy section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy one vector (known bytes) for PMC calibration")
-    ap.add_argument("--form", choices=["hv", "batching", "precision", "csr", "positions", "multik"], default="hv",
-                    help="hv: the headline H·v + all-q line; batching / precision / csr / positions / multik: their own "
-                         "line")
+    ap.add_argument("--form", choices=["hv", "batching", "precision", "csr", "positions", "multik", "allq"],
+                    default="hv",
+                    help="hv: the headline H·v + all-q line; allq: the all-q section alone; batching / precision / csr / "
+                         "positions / multik: their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); several GPUs are not supported: "

@@ -438,10 +438,12 @@ struct ed_mk {
     double* d_wd = nullptr;  // [MK][4] per-slot diagonal weights
     uint16_t* d_mask = nullptr;
     double* d_partial = nullptr;  // [MK_WGS][MK] complex
+    double* d_aux = nullptr;      // [4 * MK]: per-slot host inputs (e or s) and reduced outputs
     int* d_err = nullptr;
 
     ~ed_mk() {
-        for (void* p : {(void*)d_q, (void*)d_wd, (void*)d_mask, (void*)d_partial, (void*)d_err}) dev_free(p);
+        for (void* p : {(void*)d_q, (void*)d_wd, (void*)d_mask, (void*)d_partial, (void*)d_aux, (void*)d_err})
+            dev_free(p);
         if (hm) ed_hmat_destroy(hm);
     }
 };
@@ -630,6 +632,7 @@ int ed_mk_create(const ed_sector* u, const double* w, int n_slots, const int* kx
             m->d_mask = dev_alloc<uint16_t>(size_t(std::max<int64_t>(m->D, 1)));
             ED_HIP(hipMemsetAsync(m->d_mask, 0, size_t(std::max<int64_t>(m->D, 1)) * 2, st));
             m->d_partial = dev_alloc<double>(size_t(2) * MK_WGS * MK);
+            m->d_aux = dev_alloc<double>(size_t(4) * MK);
             m->d_err = dev_alloc<int>(1);
             ED_HIP(hipStreamSynchronize(st));
         } catch (...) {
@@ -766,31 +769,21 @@ int ed_mk_residual_norms(const ed_mk* m, const double* d_x, const double* d_hx, 
         if (m->D == 0) return;
         ED_REQUIRE(d_x, ED_ERR_ARG, "null block");
         hipStream_t st = static_cast<hipStream_t>(stream);
-        double* d_e = nullptr;
-        double* d_out = nullptr;
-        try {
-            if (d_hx && e) {
-                std::vector<double> ev(size_t(MK), 0.0);
-                std::copy(e, e + m->n_slots, ev.begin());
-                d_e = dev_upload(ev, st);
-            }
-            d_out = dev_alloc<double>(2 * MK);
-            hipLaunchKernelGGL(mk_resid_kernel, dim3(MK_WGS), dim3(MBS), 0, st, m->D, m->u->d_orbit,
-                               reinterpret_cast<const double2*>(d_x), reinterpret_cast<const double2*>(d_hx), d_e,
-                               m->d_partial);
-            hipLaunchKernelGGL(mk_reduce_kernel, dim3(MK), dim3(RBS), 0, st, m->d_partial, int64_t(MK_WGS), d_out);
-            ED_HIP(hipGetLastError());
-            double h[2 * MK];
-            ED_HIP(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, st));
-            ED_HIP(hipStreamSynchronize(st));
-            for (int k = 0; k < m->n_slots; ++k) out[k] = h[2 * k];
-        } catch (...) {
-            dev_free(d_e);
-            dev_free(d_out);
-            throw;
+        double h[2 * MK] = {0.0};
+        const bool with_e = d_hx && e;
+        if (with_e) {
+            std::copy(e, e + m->n_slots, h);
+            ED_HIP(hipMemcpyAsync(m->d_aux, h, MK * sizeof(double), hipMemcpyHostToDevice, st));
         }
-        dev_free(d_e);
-        dev_free(d_out);
+        hipLaunchKernelGGL(mk_resid_kernel, dim3(MK_WGS), dim3(MBS), 0, st, m->D, m->u->d_orbit,
+                           reinterpret_cast<const double2*>(d_x), reinterpret_cast<const double2*>(d_hx),
+                           with_e ? m->d_aux : nullptr, m->d_partial);
+        hipLaunchKernelGGL(mk_reduce_kernel, dim3(MK), dim3(RBS), 0, st, m->d_partial, int64_t(MK_WGS),
+                           m->d_aux + 2 * MK);
+        ED_HIP(hipGetLastError());
+        ED_HIP(hipMemcpyAsync(h, m->d_aux + 2 * MK, sizeof(h), hipMemcpyDeviceToHost, st));
+        ED_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < m->n_slots; ++k) out[k] = h[2 * k];
     });
 }
 
@@ -800,16 +793,13 @@ int ed_mk_scale(const ed_mk* m, double* d_x, const double* s, void* stream) {
         if (m->D == 0) return;
         ED_REQUIRE(d_x, ED_ERR_ARG, "null block");
         hipStream_t st = static_cast<hipStream_t>(stream);
-        std::vector<double> sv(size_t(MK), 0.0);
-        std::copy(s, s + m->n_slots, sv.begin());
-        double* d_s = dev_upload(sv, st);
+        double h[MK] = {0.0};
+        std::copy(s, s + m->n_slots, h);
+        ED_HIP(hipMemcpyAsync(m->d_aux, h, sizeof(h), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(mk_scale_kernel, dim3(MK_WGS), dim3(MBS), 0, st, m->D, reinterpret_cast<double2*>(d_x),
-                           d_s);
-        const hipError_t e = hipGetLastError();
-        const hipError_t e2 = hipStreamSynchronize(st);
-        dev_free(d_s);
-        ED_HIP(e);
-        ED_HIP(e2);
+                           m->d_aux);
+        ED_HIP(hipGetLastError());
+        ED_HIP(hipStreamSynchronize(st));  // h is read by the copy before it goes out of scope
     });
 }
 
