@@ -350,3 +350,42 @@ def test_lowest_eigenpairs_all_slots(torch, oracle_mod):
         r = hm.hv(psi.clone()) - e[k] * psi
         hm.close()
         assert float(r.abs().max()) <= 1e-8, slots[k]
+
+
+def test_lowest_eigenpairs_reference_fixtures(torch):
+    """All 51 eigenvalue fixtures of the reference (tests/tJ_spin_half_ladder/test_eigvals.py:9-22, tolerance
+    1e-8), grouped by ladder: every ladder's sectors and couplings as slots of multi-momentum products."""
+    import glob
+    import json
+    import pathlib
+
+    from exactdiag_amd.general import krylov
+    from exactdiag_amd.multik import SLOTS, MultiK
+    from exactdiag_amd.sector import Sector
+
+    gold = pathlib.Path(__file__).parent / "golden" / "precalculated_basis_size_eigenvalues"
+    fx = [d for d in (json.load(open(f, encoding="utf-8")) for f in sorted(glob.glob(str(gold / "*.txt"))))
+          if "eigvals" in d]
+    assert len(fx) == 51
+    ladders = {}
+    for d in fx:
+        ladders.setdefault((d["num_rungs"], d["num_holes"], d["num_down_spins"]), []).append(d)
+    checked = 0
+    for (L, h, nd), items in ladders.items():
+        u = Sector.union(L, h, nd)
+        for lo in range(0, len(items), SLOTS):
+            grp = items[lo: lo + SLOTS]
+            ws = [[d["weights"][t] for t in ("tl", "tr", "jl", "jr")] for d in grp]
+            ks = [tuple(d["symmetry_qs"]) for d in grp]
+            mk = MultiK(u, ws, ks)
+            secs = [Sector(L, h, nd, *k) for k in ks]
+            v0 = mk.block()
+            for slot, s in enumerate(secs):
+                mk.attach(slot, s, krylov.device_start_vector(s.dim, slot), v0)
+            e, x, _, res = mk.lowest(v0, tol=1e-10)
+            for slot, d in enumerate(grp):
+                assert abs(e[slot] - d["eigvals"][0]) < 1e-8, (L, h, nd, ks[slot], e[slot], d["eigvals"][0])
+                checked += 1
+            mk.close()
+        print("ladder", (L, h, nd), "union", u.dim, "fixtures", len(items), flush=True)
+    assert checked == 51
