@@ -27,6 +27,7 @@
 // workgroup (1..32, i.e. 64..2048 states per wavefront) and keeps the fastest.
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -97,9 +98,9 @@ __device__ inline bool inactive(const BLz& lz, int inst, int step) {
 template <int G>
 __global__ void __launch_bounds__(BS) bhv_kernel(const BInst* __restrict__ insts, const BJob* __restrict__ jobs,
                                                  int njobs, BLz lz, int step, const double2* __restrict__ x,
-                                                 double2* __restrict__ y, double* __restrict__ partial) {
+                                                 double2* __restrict__ y, double* __restrict__ partial, int chunk) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int jb = job_index(njobs);
+    const int jb = xcd_order(blockIdx.x, njobs, chunk);  // the eight XCDs on neighbouring chunks of jobs
     const BJob job = jobs[jb];
     const BInst& I = insts[job.inst];
     if (inactive(lz, job.inst, step)) {
@@ -342,15 +343,21 @@ struct ed_batch {
         const int nj = int(jobs.size());
         if (nj == 0) return;
         const dim3 grid{unsigned(nj), 1u, 1u};
+        // XCD chunks of about 131,072 rows (ED_BATCH_XCD_CHUNK_ROWS overrides, 0 = one eighth per XCD)
+        static const int64_t chunk_rows = [] {
+            const char* e = std::getenv("ED_BATCH_XCD_CHUNK_ROWS");
+            return e ? int64_t(std::atoll(e)) : int64_t(131072);
+        }();
+        const int chunk = chunk_rows > 0 ? int(std::max<int64_t>(1, chunk_rows / (int64_t(tiles) * BS))) : 0;
         if (gather == 2)
             hipLaunchKernelGGL(bhv_kernel<2>, grid, dim3(BS), lds_bytes, st, d_inst, d_jobs, nj, lz, step, x, y,
-                               d_partial);
+                               d_partial, chunk);
         else if (gather == 8)
             hipLaunchKernelGGL(bhv_kernel<8>, grid, dim3(BS), lds_bytes, st, d_inst, d_jobs, nj, lz, step, x, y,
-                               d_partial);
+                               d_partial, chunk);
         else
             hipLaunchKernelGGL(bhv_kernel<4>, grid, dim3(BS), lds_bytes, st, d_inst, d_jobs, nj, lz, step, x, y,
-                               d_partial);
+                               d_partial, chunk);
         ED_HIP(hipGetLastError());
     }
 

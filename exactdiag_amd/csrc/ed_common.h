@@ -62,6 +62,21 @@ __host__ __device__ inline uint32_t image_mask(uint32_t x, int p, int N, uint32_
 
 __host__ __device__ inline uint64_t make_key(uint32_t H, uint32_t Dn) { return (uint64_t(H) << 32) | Dn; }
 
+// Workgroup w of a grid of nwg runs on XCD w % 8.  xcd_order gives the piece
+// of work it takes: XCD x walks chunks x, x + 8, ... of `chunk` consecutive
+// pieces, so that the eight XCDs work on neighbouring chunks at the same time
+// (chunk <= 0 or >= nwg / 8: one contiguous eighth per XCD).  A bijection on
+// [0, nwg).
+__host__ __device__ inline int xcd_order(int w, int nwg, int chunk) {
+    const int per = nwg / 8;
+    if (per <= 0 || w >= per * 8) return w;
+    const int x = w & 7, l = w >> 3;
+    const int S = chunk > 0 && chunk < per ? chunk : per;
+    const int full = per / S;
+    if (l < full * S) return ((l / S) * 8 + x) * S + l % S;
+    return full * S * 8 + x * (per - full * S) + (l - full * S);
+}
+
 __host__ __device__ inline int popc32(uint32_t x) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __popc(x);

@@ -207,18 +207,7 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(StoredView m, int64_t D, co
     __syncthreads();
     const int nwg = gridDim.x;
     int wg = blockIdx.x;
-    const int per = nwg / 8;
-    if (per > 0 && wg < per * 8) {
-        // XCD x (hardware workgroups wg % 8 == x) takes chunks of S consecutive tiles, chunks x, x + 8, ...;
-        // S = per gives each XCD one contiguous eighth of the rows
-        const int x = wg & 7, l = wg >> 3;
-        const int S = chunk > 0 && chunk < per ? chunk : per;
-        const int full = per / S;
-        if (l < full * S)
-            wg = ((l / S) * 8 + x) * S + l % S;
-        else
-            wg = full * S * 8 + x * (per - full * S) + (l - full * S);
-    }
+    wg = xcd_order(wg, nwg, chunk);
     const int64_t f0 = int64_t(wg) * tiles * BS;
     const int64_t f1 = f0 + tiles * BS < D ? f0 + tiles * BS : D;
     C2 dot{0.0, 0.0};

@@ -13,6 +13,7 @@
 //    sector (ed_hv_tune times the candidates and keeps the fastest).
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "ed_hvrow.h"
 
@@ -22,15 +23,12 @@ namespace {
 template <class VT, int BS, int G>
 __global__ void __launch_bounds__(BS) hv_lds_kernel(SectorView s, HvWeights w, const VT* __restrict__ x,
                                                     VT* __restrict__ y, double* __restrict__ partial, LdsLayout lay,
-                                                    int64_t ntiles, int64_t tiles_per_wg) {
+                                                    int64_t ntiles, int64_t tiles_per_wg, int chunk) {
     extern __shared__ __align__(16) unsigned char lds[];
     const Tabs T = stage_tabs<BS>(s, lay, lds);
     __syncthreads();
-    // neighbouring row ranges for consecutive workgroups of one XCD
-    const int nwg = gridDim.x;
-    int wg = blockIdx.x;
-    const int per = nwg / 8;
-    if (per > 0 && wg < per * 8) wg = (wg & 7) * per + (wg >> 3);
+    // the eight XCDs on neighbouring chunks of row ranges
+    const int wg = xcd_order(blockIdx.x, gridDim.x, chunk);
     const int64_t t0 = int64_t(wg) * tiles_per_wg;
     const int64_t t1 = t0 + tiles_per_wg < ntiles ? t0 + tiles_per_wg : ntiles;
     C2 dot{0.0, 0.0};
@@ -68,18 +66,24 @@ template <class VT>
 void launch_g(int G, int64_t nwg, const LdsLayout& lay, hipStream_t st, const SectorView& v, const HvWeights& w,
               const void* x, void* y, double* partial, int64_t ntiles, int64_t tpw) {
     constexpr int BS = 256;
+    // XCD chunks of about 131,072 rows (ED_HV_XCD_CHUNK_ROWS overrides, 0 = one eighth per XCD)
+    static const int64_t chunk_rows = [] {
+        const char* e = std::getenv("ED_HV_XCD_CHUNK_ROWS");
+        return e ? int64_t(std::atoll(e)) : int64_t(131072);
+    }();
+    const int chunk = chunk_rows > 0 ? int(std::max<int64_t>(1, chunk_rows / (tpw * BS))) : 0;
     const VT* xx = static_cast<const VT*>(x);
     VT* yy = static_cast<VT*>(y);
     const dim3 grid{unsigned(nwg), 1u, 1u};
     if (G == 2)
         hipLaunchKernelGGL((hv_lds_kernel<VT, BS, 2>), grid, dim3(BS), lay.bytes, st, v, w, xx, yy, partial, lay,
-                           ntiles, tpw);
+                           ntiles, tpw, chunk);
     else if (G == 8)
         hipLaunchKernelGGL((hv_lds_kernel<VT, BS, 8>), grid, dim3(BS), lay.bytes, st, v, w, xx, yy, partial, lay,
-                           ntiles, tpw);
+                           ntiles, tpw, chunk);
     else
         hipLaunchKernelGGL((hv_lds_kernel<VT, BS, 4>), grid, dim3(BS), lay.bytes, st, v, w, xx, yy, partial, lay,
-                           ntiles, tpw);
+                           ntiles, tpw, chunk);
 }
 
 }  // namespace
