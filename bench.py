@@ -76,28 +76,36 @@ def cpu_baseline(threads):
     x = rng.normal(size=D) + 1j * rng.normal(size=D)
     y = np.zeros(D, np.complex128)
 
-    def once():
+    def once(nthreads=threads):
         for t in range(4):
             values, vinds, minor, major = csr[t]
             if kind == "reference":
-                oracle.ref_csr_dot_rows(D, 0, values, vinds, minor, major, x, y, WEIGHTS[t], threads=threads)
+                oracle.ref_csr_dot_rows(D, 0, values, vinds, minor, major, x, y, WEIGHTS[t], threads=nthreads)
             else:
                 raise RuntimeError("oracle/_ref missing: no CPU sparse-dot kernel to time")
 
-    once()
-    reps, t_total = 0, 0.0
-    while t_total < 10.0 and reps < 200:
-        t1 = time.perf_counter()
-        once()
-        t_total += time.perf_counter() - t1
-        reps += 1
+    def timed(nthreads, budget, max_reps):
+        once(nthreads)
+        reps, t_total = 0, 0.0
+        while t_total < budget and reps < max_reps:
+            t1 = time.perf_counter()
+            once(nthreads)
+            t_total += time.perf_counter() - t1
+            reps += 1
+        return reps, t_total
+
+    reps, t_total = timed(threads, 10.0, 200)
     rate = rows * reps / t_total / 1e9
+    reps1, t1_total = timed(1, 5.0, 50)
+    rate1 = rows * reps1 / t1_total / 1e9
+    log(f"cpu baseline: {rate:.4f} Gstates/s on {threads} threads, {rate1:.4f} on one")
     return {
         "value": rate,
         "unit": "Gstates/s",
         "cores": threads,
         "kind": kind,
         "csr_build_s_per_row": t_build / rows,
+        "single_core": {"value": rate1, "unit": "Gstates/s", "cores": 1, "reps": reps1, "seconds": t1_total},
         "sample": (f"reference sparse-dot kernels (general/cpp_sparse_dot.h) over rows [0, {rows}) of the "
                    f"2x12 h2 k=(0,0) sector's stored 4-term upper-triangle CSR (explicit + Hermitian passes), "
                    f"{reps} repetitions, {t_total:.1f}s"),
