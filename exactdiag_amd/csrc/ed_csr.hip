@@ -607,7 +607,8 @@ __global__ void __launch_bounds__(256) csr_sum_hv_kernel(int64_t rows, const int
         for (int i = threadIdx.x; i < n_entries; i += 256) tab[i] = table[i];
         __syncthreads();
     }
-    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    // row tiles dealt to the XCDs in neighbouring chunks of 512 (the stored product's order)
+    const int64_t r = int64_t(xcd_order(blockIdx.x, gridDim.x, 512)) * 256 + threadIdx.x;
     if (r >= rows) return;
     const int64_t base = sptr[r / SLICE];
     const int64_t width = (sptr[r / SLICE + 1] - base) / SLICE;
