@@ -266,7 +266,7 @@ struct ed_hmat {
     int n_vals = 0;
     int tiles = 0;   // row tiles per workgroup of the product (0 = default)
     int unroll = 4;  // gathers in flight per lane (4 or 8)
-    int nontemporal = -1;  // stream elements and output non-temporally: 1 yes, 0 no, -1 for complex128 only
+    int nontemporal = -1;  // stream elements and output non-temporally: 0 no, otherwise yes (-1: default, yes)
     int64_t* d_sptr = nullptr;
     int32_t* d_col = nullptr;    // 6-byte form
     uint16_t* d_code = nullptr;  // 6-byte form
@@ -316,9 +316,9 @@ template <class VT, int U>
 void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int64_t nwg, int64_t tiles,
                  hipStream_t st) {
     const size_t lds = size_t(m->n_vals) * 16;
-    // measured on the 2x12 sector: non-temporal streaming gains 4 % with complex128 vectors, nothing with
-    // complex64 (ed_hmat_tune)
-    const bool nt = m->nontemporal < 0 ? sizeof(VT) == 16 : m->nontemporal != 0;
+    // non-temporal streaming by default: with the chunked XCD order it gains for both vector types on the
+    // 2x12 sector (ed_hmat_tune: complex64 0.340 ms with it, 0.352 ms without)
+    const bool nt = m->nontemporal != 0;
     const StoredView v = hmat_view(m);
     auto go = [&](auto kernel) {
         // XCD chunk of 512 workgroups (131,072 rows with one tile each): measured on the 2x12 sector, the

@@ -320,7 +320,7 @@ class StoredH:
         return t.value, u.value, nt.value, ms.value
 
     def set_config(self, tiles=0, unroll=4, nontemporal=-1):
-        """tiles 0 = default; nontemporal -1 = default (complex128 only), 0 or 1 (bools accepted)."""
+        """tiles 0 = default; nontemporal -1 = default (non-temporal), 0 or 1 (bools accepted)."""
         check(lib().ed_hmat_set_config(self._h, int(tiles), int(unroll), int(nontemporal)))
 
     def lanczos(self, v0, nsteps: int, precision: int = 0):

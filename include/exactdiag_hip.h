@@ -181,7 +181,7 @@ int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, 
  * plain or non-temporal streaming of the stored elements and the output;
  * times the product with its fused dot and keeps the fastest in the matrix.
  * ed_hmat_set_config sets the same three choices (tiles 0 = default;
- * nontemporal -1 = the default, non-temporal for complex128 vectors only). */
+ * nontemporal -1 = the default: non-temporal). */
 int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll,
                  int* nontemporal, double* best_ms, void* stream);
 int ed_hmat_set_config(ed_hmat* m, int tiles, int unroll, int nontemporal);
