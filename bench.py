@@ -120,6 +120,37 @@ ALLQ_STEPS_FAST = 170  # 150 meets 1e-3 only by a few per cent (its FP64 max is 
 ALLQ_TARGET = 1e-3
 
 
+def allq_larger(rungs):
+    """The fast and multi-momentum all-q forms on a 2 x rungs ladder with 2 holes (the headline's settings),
+    against 1000-step FP64 spectra: the forms at a larger size than the bench line's."""
+    import numpy as np
+
+    from exactdiag_amd import allq
+
+    ws = np.linspace(*ALLQ_WS)
+    nd = rungs - 1
+    t0 = time.perf_counter()
+    log(f"2x{rungs}: all-q reference ({ALLQ_STEPS_REF} steps) ...")
+    e0, ref = allq.allq_reference(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_REF, log=log)
+    t_ref = time.perf_counter() - t0
+    log(f"2x{rungs}: all-q reference ({ALLQ_STEPS_REF} steps) in {t_ref:.1f}s, E0 = {e0:.12f}")
+    t_fast, s_fast = allq.allq_fast(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
+    fast_breakdown = dict(allq.LAST_FAST_TIMINGS)
+    log(f"2x{rungs}: all-q fast in {t_fast:.2f}s: {fast_breakdown}")
+    t_m, s_m, tm_m = allq.allq_multik(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
+    log(f"2x{rungs}: all-q over one multi-momentum H in {t_m:.2f}s: {tm_m}")
+    return {
+        "workload": f"S(q,w) for the {len(ref)} distinct q of the 2x{rungs} ladder, 2 holes, Sz=0, k=(0,0), t=1, "
+                    "J=0.4; eta=0.05, w in [0,4] on 2001 points; L1 vs 1000-step FP64",
+        "reference_seconds": t_ref,
+        "fast": {"seconds": t_fast, "steps": ALLQ_STEPS_FAST, "breakdown_s": fast_breakdown,
+                 "max_l1": max(allq.spectral_l1(s_fast[q], ref[q], ws) for q in ref)},
+        "multik": {"seconds": t_m, "steps": ALLQ_STEPS_FAST, "breakdown_s": tm_m,
+                   "max_l1": max(allq.spectral_l1(s_m[q], ref[q], ws) for q in ref)},
+        "best": {"form": "multik" if t_m < t_fast else "fast", "seconds": min(t_m, t_fast)},
+    }
+
+
 def allq_section():
     """All-q S(q, w) of the same ladder (14 distinct q) to <= 1e-3 spectral L1 error: plain FP64 matrix-free
     Lanczos with truncated fraction vs the fast form (stored H, complex64 vectors, square-root terminator);
@@ -299,6 +330,12 @@ def run_form(form, args=None):
             line["speedup_vs_cpu"] = cpu["value"] / res["build_s"]
         for m in mats:
             m.close()
+    elif form == "allq" and args is not None and args.allq_rungs and args.allq_rungs != L:
+        res = allq_larger(args.allq_rungs)
+        line = {"metric": "time to all-q S(q,w) at <= 1e-3 spectral L1 (seconds, best form)",
+                "value": res["best"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                "dtype": "complex64", "data": "synthetic (model built from its definition)",
+                "config": {"workload": res["workload"]}, "allq": res}
     elif form == "allq":
         res = allq_section()
         line = {"metric": "time to all-q S(q,w) at <= 1e-3 spectral L1 (seconds, best form)",
@@ -370,6 +407,9 @@ def main():
     ap.add_argument("--no-allq", action="store_true", help="skip the all-q time-to-accuracy section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy one vector (known bytes) for PMC calibration")
+    ap.add_argument("--allq-rungs", type=int, default=0,
+                    help="with --form allq: the ladder's rungs (default: the headline's 12); others run the fast "
+                         "and multi-momentum forms only")
     ap.add_argument("--form", choices=["hv", "batching", "precision", "csr", "positions", "multik", "allq"],
                     default="hv",
                     help="hv: the headline H·v + all-q line; allq: the all-q section alone; batching / precision / csr / "

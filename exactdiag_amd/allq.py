@@ -150,12 +150,15 @@ def _gs(apply, dim, tol, seed=0):
     return float(vals[0]), vecs[0]
 
 
-def allq_reference(L, h, nd, weights, ws, eta, nsteps=1000, k0=(0, 0), limited=True):
-    """S_ref(q, w) for every distinct q: ground state to 1e-12, then nsteps FP64 Lanczos steps (stored H)."""
+def allq_reference(L, h, nd, weights, ws, eta, nsteps=1000, k0=(0, 0), limited=True, log=None):
+    """S_ref(q, w) for every distinct q: ground state to 1e-12, then nsteps FP64 Lanczos steps (stored H).
+    ``log`` (optional) receives one progress line per q."""
     s0 = Sector(L, h, nd, k0[0], k0[1])
     hm0 = s0.store(weights)
     e0, psi = _gs(hm0.hv, s0.dim, 1e-12)
     hm0.close()
+    if log is not None:
+        log(f"  reference ground state E0 = {e0:.12f}")
     out = {}
     for qx, qy in distinct_qs(L, limited):
         kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
@@ -170,6 +173,8 @@ def allq_reference(L, h, nd, weights, ws, eta, nsteps=1000, k0=(0, 0), limited=T
             out[(qx, qy)] = cf_eval(a, b, ws, e0, eta, -n2 / np.pi)
             hm.close()
         sec.close()
+        if log is not None:
+            log(f"  reference spectrum q=({qx},{qy}) done")
     s0.close()
     _sync()
     return e0, out
