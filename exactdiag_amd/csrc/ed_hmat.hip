@@ -135,8 +135,9 @@ __global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __res
 
 // Order each row's stored elements by column (ties by value index and sign),
 // so that the lanes of a wavefront reading slot k gather from nearby columns.
-// One wavefront per 64-row slice, one lane per row: a row of up to SORT_CAP
-// elements is copied into the lane's own LDS column (K[k][lane], no bank
+// One wavefront per 64-row slice, one lane per row: a row of up to 32
+// elements is sorted by a bitonic network in registers; one of up to
+// SORT_CAP is copied into the lane's own LDS column (K[k][lane], no bank
 // conflicts), insertion-sorted there and written back; a longer row is
 // insertion-sorted in place in HBM.  The key is the packed word, or
 // (column << 16) | code for the two-array form.
@@ -167,7 +168,31 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
             code[at] = uint16_t(v & 0xffffu);
         }
     };
-    if (n <= SORT_CAP) {
+    if (n <= 32) {
+        // up to 32 keys: a bitonic network in registers (slots past n padded with the largest key)
+        Key r[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) r[k] = k < n ? key_at(k) : Key(~Key(0));
+#pragma unroll
+        for (int size = 2; size <= 32; size <<= 1) {
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const int j = i ^ stride;
+                    if (j > i) {
+                        const Key a = r[i], b = r[j];
+                        const bool up = (i & size) == 0;
+                        r[i] = up ? (a < b ? a : b) : (a < b ? b : a);
+                        r[j] = up ? (a < b ? b : a) : (a < b ? a : b);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            if (k < n) put(k, r[k]);
+    } else if (n <= SORT_CAP) {
         for (int k = 0; k < n; ++k) K[k][lane] = key_at(k);
         for (int k = 1; k < n; ++k) {
             const Key v = K[k][lane];
