@@ -102,13 +102,18 @@ __global__ void __launch_bounds__(BS) hm_count_kernel(SectorView s, HvWeights w,
     diag[f] = diag_map[c.djl * (s.L + 1) + c.djr];
 }
 
-// width[slice] = largest element count of its 64 rows
+// width[slice] = largest element count of its 64 rows (low 16 bits) and
+// their sum (high 16 bits: at most 64 x 255)
 __global__ void __launch_bounds__(BS) hm_width_kernel(const uint8_t* cnt, int64_t D, int32_t* width, int64_t nslices) {
     const int64_t sl = int64_t(blockIdx.x) * (BS / 64) + threadIdx.x / 64;
     const int64_t f = sl * 64 + (threadIdx.x & 63);
     int c = f < D ? int(cnt[f]) : 0;
-    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off, 64));
-    if ((threadIdx.x & 63) == 0 && sl < nslices) width[sl] = c;
+    int t = c;
+    for (int off = 32; off > 0; off >>= 1) {
+        c = max(c, __shfl_xor(c, off, 64));
+        t += __shfl_xor(t, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && sl < nslices) width[sl] = c | (t << 16);
 }
 
 __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, LdsLayout lay, int lds,
@@ -509,11 +514,12 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
             ED_HIP(hipGetLastError());
             std::vector<int32_t> width(static_cast<size_t>(m->nslices));
             ED_HIP(hipMemcpyAsync(width.data(), d_width, width.size() * 4, hipMemcpyDeviceToHost, st));
-            std::vector<uint8_t> cnt(static_cast<size_t>(D));
-            ED_HIP(hipMemcpyAsync(cnt.data(), m->d_cnt, size_t(D), hipMemcpyDeviceToHost, st));
             ED_HIP(hipStreamSynchronize(st));
-            for (int64_t i = 0; i < m->nslices; ++i) sptr[size_t(i + 1)] = sptr[size_t(i)] + int64_t(width[i]) * 64;
-            for (uint8_t c : cnt) m->nnz += c;
+            for (int64_t i = 0; i < m->nslices; ++i) {
+                const int32_t wv = width[size_t(i)];
+                sptr[size_t(i + 1)] = sptr[size_t(i)] + int64_t(wv & 0xffff) * 64;
+                m->nnz += int64_t(uint32_t(wv) >> 16);
+            }
             m->slots = sptr.back();
             m->d_sptr = dev_upload(sptr, st);
             if (m->shift) {
