@@ -17,6 +17,8 @@ ED_OK, ED_ERR_ARG, ED_ERR_HIP, ED_ERR_STATE, ED_ERR_CAPACITY = 0, -1, -2, -3, -4
 (OP_SZQ, OP_NQ, OP_APLUS_DOWN, OP_APLUS_UP, OP_AMINUS_DOWN, OP_AMINUS_UP, OP_CURRENT_RUNG,
  OP_CURRENT_LEG) = range(8)
 POS_HOLES, POS_SZ, POS_SINGLET = range(3)
+# vector types of products and Lanczos runs (enum ed_precision)
+PREC_C128, PREC_C64, PREC_R64, PREC_R32 = range(4)
 
 EXPORTED = [
     "ed_last_error", "ed_version", "ed_sector_create", "ed_sector_destroy", "ed_basis_build", "ed_basis_size",
@@ -27,7 +29,7 @@ EXPORTED = [
     "ed_batch_create", "ed_batch_destroy", "ed_batch_offsets", "ed_batch_hv", "ed_batch_lanczos", "ed_batch_tune",
     "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune", "ed_lanczos_ritz",
     "ed_hmat_create", "ed_hmat_destroy", "ed_hmat_info", "ed_hmat_hv", "ed_hmat_lanczos_run", "ed_hmat_lanczos_ritz",
-    "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_batch_create_stored",
+    "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_hmat_is_real", "ed_batch_create_stored",
     "ed_csr_from_sector", "ed_csr_from_operator", "ed_csr_create", "ed_csr_info", "ed_csr_copy_out", "ed_csr_destroy",
     "ed_csr_sum_create", "ed_csr_sum_info", "ed_csr_sum_hv", "ed_csr_sum_destroy",
     "ed_position_op_columns", "ed_position_op_apply", "ed_position_expectation", "ed_position_correlations",
@@ -91,6 +93,7 @@ _SIGNATURES = {
     "ed_hmat_tune": (ct.c_int, [_vp, _vp, _vp, ct.c_int, _ip, _ip, _ip, _dp, _vp]),
     "ed_hmat_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int]),
     "ed_hmat_element_bytes": (ct.c_int, [_vp]),
+    "ed_hmat_is_real": (ct.c_int, [_vp]),
     "ed_batch_create_stored": (ct.c_int, [ct.POINTER(_vp), ct.c_int, ct.POINTER(_vp)]),
     "ed_csr_from_sector": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(_vp), _vp]),
     "ed_csr_from_operator": (ct.c_int, [_vp, _vp, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(_vp), _vp]),

@@ -208,13 +208,16 @@ def allq_plain(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=
 
 
 def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL, precision=1,
-              terminator=True, two_pass_gs=True):
+              terminator=True, two_pass_gs=True, real=True):
     """The fast form: stored H (built once per sector on the GPU), complex64 Lanczos vectors with FP64
     coefficient reductions, fewer steps closed by the square-root terminator; the ground state by restarted
-    two-pass Lanczos in complex128 (no re-orthogonalisation, the same residual criterion as the plain run).  With
-    precision=0, terminator=False and two_pass_gs=False it is the plain method on the stored H.
-    Returns (seconds, spectra)."""
+    two-pass Lanczos in complex128 (no re-orthogonalisation, the same residual criterion as the plain run).
+    With real=True, sectors whose stored H is real (kx = 0, L/2) run in real vectors of the same precision
+    (float64 for the ground state, float32 for the spectra) when the start vector is real up to a global
+    phase.  With precision=0, terminator=False, two_pass_gs=False and real=False it is the plain method on the
+    stored H.  Returns (seconds, spectra)."""
     from exactdiag_amd.closure import sqrt_terminator
+    from exactdiag_amd.sector import real_phase
 
     tm = dict.fromkeys(("ground_state", "bases", "stored_h", "operator", "lanczos", "continued_fraction"), 0.0)
 
@@ -233,7 +236,7 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
     if two_pass_gs:
         from exactdiag_amd.forms import two_pass_ground_state
 
-        e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol)
+        e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol, real=real)
     else:
         e0, psi = _gs(hm0.hv, s0.dim, tol)
     hm0.close()
@@ -251,7 +254,11 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
         else:
             hm = sec.store(weights)
             t = lap("stored_h", t)
-            a, b = hm.lanczos(phi, nsteps, precision=precision)
+            phr = real_phase(phi) if (real and hm.is_real) else None
+            if phr is not None:
+                a, b = hm.lanczos(phr, nsteps, precision=_lib.PREC_R64 if precision == 0 else _lib.PREC_R32)
+            else:
+                a, b = hm.lanczos(phi, nsteps, precision=precision)
             hm.close()
             t = lap("lanczos", t)
             at, bt, tail = sqrt_terminator(a, b, len(a)) if (terminator and len(a) == nsteps) else (a, b, None)
@@ -283,7 +290,7 @@ def allq_batched(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, to
     t0 = time.perf_counter()
     s0 = Sector(L, h, nd, k0[0], k0[1])
     hm0 = s0.store(weights)
-    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol)
+    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol, real=True)
     hm0.close()
     _sync()
     tm["ground_state"] = time.perf_counter() - t0
@@ -331,7 +338,7 @@ def allq_multik(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol
     t = lap("bases", t)
     hm0 = s0.store(weights)
     t = lap("stored_h", t)
-    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol)
+    e0, psi, _, _ = two_pass_ground_state(hm0, precision=0, nsteps=GS_STEPS, tol=tol, real=True)
     hm0.close()
     t = lap("ground_state", t)
     qs = distinct_qs(L, limited)

@@ -244,20 +244,28 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(StoredView m, int64_t D, co
     for (int64_t f = f0 + threadIdx.x; f < f1; f += BS) {
         C2 xf;
         const C2 yf = stored_row<VT, U, NT, PACKED>(m, V, x, f, xf);
-        VT o;
-        o.x = decltype(o.x)(yf.re);
-        o.y = decltype(o.y)(yf.im);
-        if constexpr (NT) {
-            using S = decltype(o.x);
-            typedef S S2 __attribute__((ext_vector_type(2)));
-            S2 v;
-            v.x = o.x;
-            v.y = o.y;
-            __builtin_nontemporal_store(v, reinterpret_cast<S2*>(y + f));
+        if constexpr (is_real_vec<VT>::value) {
+            if constexpr (NT)
+                __builtin_nontemporal_store(VT(yf.re), y + f);
+            else
+                y[f] = VT(yf.re);
+            dot.re += xf.re * yf.re;
         } else {
-            y[f] = o;
+            VT o;
+            o.x = decltype(o.x)(yf.re);
+            o.y = decltype(o.y)(yf.im);
+            if constexpr (NT) {
+                using S = decltype(o.x);
+                typedef S S2 __attribute__((ext_vector_type(2)));
+                S2 v;
+                v.x = o.x;
+                v.y = o.y;
+                __builtin_nontemporal_store(v, reinterpret_cast<S2*>(y + f));
+            } else {
+                y[f] = o;
+            }
+            dot = cadd(dot, cmul(C2{xf.re, -xf.im}, yf));
         }
-        dot = cadd(dot, cmul(C2{xf.re, -xf.im}, yf));
     }
     if (partial) {
         __shared__ double red[2 * BS];
@@ -302,6 +310,7 @@ struct ed_hmat {
     uint16_t* d_code = nullptr;  // 6-byte form
     uint32_t* d_pack = nullptr;  // 4-byte form
     int shift = 0;               // bits of value index + sign in the 4-byte form; 0 = 6-byte form
+    bool real = false;           // every stored value real: real vectors allowed
     uint8_t* d_cnt = nullptr;
     uint16_t* d_diag = nullptr;
     double* d_vals = nullptr;
@@ -382,19 +391,38 @@ int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, i
     if (m->D == 0) return 0;
     const int64_t tiles = m->tiles > 0 ? m->tiles : hmat_default_tiles(m->D);
     const int64_t nwg = ((m->D + BS - 1) / BS + tiles - 1) / tiles;
-    if (precision == 0) {
-        if (m->unroll == 8)
-            launch_spmv<double2, 8>(m, x, y, partial, nwg, tiles, st);
-        else
-            launch_spmv<double2, 4>(m, x, y, partial, nwg, tiles, st);
-    } else {
-        if (m->unroll == 8)
-            launch_spmv<float2, 8>(m, x, y, partial, nwg, tiles, st);
-        else
-            launch_spmv<float2, 4>(m, x, y, partial, nwg, tiles, st);
+    const bool u8 = m->unroll == 8;
+    switch (precision) {
+        case ED_PREC_C128:
+            u8 ? launch_spmv<double2, 8>(m, x, y, partial, nwg, tiles, st)
+               : launch_spmv<double2, 4>(m, x, y, partial, nwg, tiles, st);
+            break;
+        case ED_PREC_C64:
+            u8 ? launch_spmv<float2, 8>(m, x, y, partial, nwg, tiles, st)
+               : launch_spmv<float2, 4>(m, x, y, partial, nwg, tiles, st);
+            break;
+        case ED_PREC_R64:
+            u8 ? launch_spmv<double, 8>(m, x, y, partial, nwg, tiles, st)
+               : launch_spmv<double, 4>(m, x, y, partial, nwg, tiles, st);
+            break;
+        default:
+            u8 ? launch_spmv<float, 8>(m, x, y, partial, nwg, tiles, st)
+               : launch_spmv<float, 4>(m, x, y, partial, nwg, tiles, st);
+            break;
     }
     ED_HIP(hipGetLastError());
     return nwg;
+}
+
+// True when every value of the stored table is real (to 1e-13 of the largest
+// magnitude): H is then a real symmetric matrix in the sector's basis, which
+// holds for momenta kx = 0 and kx = L/2.
+bool hmat_is_real(const ed_hmat* m) { return m->real; }
+
+void hmat_require_precision(const ed_hmat* m, int precision) {
+    ED_REQUIRE(precision >= ED_PREC_C128 && precision <= ED_PREC_R32, ED_ERR_ARG, "precision must be 0, 1, 2 or 3");
+    ED_REQUIRE(precision <= ED_PREC_C64 || m->real, ED_ERR_ARG,
+               "real vectors need a real stored matrix (momentum kx = 0 or L/2)");
 }
 
 // Stored structure of sector s.  coded = 0: each element's code indexes the
@@ -478,6 +506,14 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
             }
         m->n_vals = int(vals.size() / 2);
         ED_REQUIRE(m->n_vals < 0x8000, ED_ERR_CAPACITY, "more than 32767 unique matrix values");
+        if (!coded) {
+            double big = 0.0, imag = 0.0;
+            for (int i = 0; i < m->n_vals; ++i) {
+                big = std::max({big, std::fabs(vals[size_t(2 * i)]), std::fabs(vals[size_t(2 * i + 1)])});
+                imag = std::max(imag, std::fabs(vals[size_t(2 * i + 1)]));
+            }
+            m->real = imag <= 1e-13 * big;
+        }
         // 4-byte elements when column, value index and sign fit 32 bits (ED_HMAT_ELEMENT_BYTES=6 forces the
         // 6-byte form)
         {
@@ -595,7 +631,7 @@ int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int*
                  int* nontemporal, double* best_ms, void* stream) {
     return guarded([&] {
         ED_REQUIRE(m && d_x && d_y, ED_ERR_ARG, "null argument");
-        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        hmat_require_precision(m, precision);
         hipStream_t st = static_cast<hipStream_t>(stream);
         double best = 1e300;
         int bt = 0, bu = 4, bn = 0;
@@ -642,10 +678,12 @@ int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int*
     });
 }
 
+int ed_hmat_is_real(const ed_hmat* m) { return m && m->real ? 1 : 0; }
+
 int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream) {
     return guarded([&] {
         ED_REQUIRE(m && d_x && d_y, ED_ERR_ARG, "null argument");
-        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        hmat_require_precision(m, precision);
         if (m->D == 0) return;
         hipStream_t st = static_cast<hipStream_t>(stream);
         const int64_t np = launch_hmat(m, d_x, d_y, d_dot ? m->d_partial : nullptr, precision, st);

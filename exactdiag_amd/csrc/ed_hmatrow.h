@@ -55,14 +55,45 @@ __device__ inline void ld_elem(const StoredView& m, int64_t at, int32_t& j, int&
 template <class VT>
 __device__ inline C2 ld_vec(const VT* __restrict__ x, int64_t j) {
     const VT v = x[j];
-    return {double(v.x), double(v.y)};
+    if constexpr (is_real_vec<VT>::value)
+        return {double(v), 0.0};
+    else
+        return {double(v.x), double(v.y)};
 }
 
 // y_f of the stored product; xf receives x_f for the fused <x, Hx>.
+// Real vectors (VT double or float) use only the real parts of the values.
 template <class VT, int U, bool NT, bool PACKED>
 __device__ inline C2 stored_row(const StoredView& m, const double* V, const VT* __restrict__ x, int64_t f, C2& xf) {
     const int64_t base = m.sptr[f >> 6] + (f & 63);
     const int n = ld_stream<NT>(m.cnt + f);
+    if constexpr (is_real_vec<VT>::value) {
+        double acc = 0.0;
+        int k = 0;
+        for (; k + U <= n; k += U) {
+            int32_t j[U];
+            int vi[U];
+            double sg[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) ld_elem<PACKED, NT>(m, base + int64_t(k + u) * 64, j[u], vi[u], sg[u]);
+            double xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = double(x[j[u]]);
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += sg[u] * V[2 * vi[u]] * xv[u];
+        }
+        for (; k < n; ++k) {
+            int32_t j;
+            int vi;
+            double sg;
+            ld_elem<PACKED, NT>(m, base + int64_t(k) * 64, j, vi, sg);
+            acc += sg * V[2 * vi] * double(x[j]);
+        }
+        const double sqf = m.sq[m.orbit[f]];
+        xf = C2{double(x[f]), 0.0};
+        const int dv = ld_stream<NT>(m.diag + f);
+        return C2{sqf * acc + V[2 * dv] * xf.re, 0.0};
+    }
     C2 acc{0.0, 0.0};
     int k = 0;
     for (; k + U <= n; k += U) {

@@ -14,6 +14,22 @@
 
 namespace ed {
 
+// Vector element types: complex (double2, float2) or real (double, float).
+// Real vectors serve sectors whose H is real (momenta kx = 0 or L/2): their
+// values have no imaginary part, so a real start vector stays real.
+template <class VT>
+struct is_real_vec {
+    static constexpr bool value = false;
+};
+template <>
+struct is_real_vec<double> {
+    static constexpr bool value = true;
+};
+template <>
+struct is_real_vec<float> {
+    static constexpr bool value = true;
+};
+
 void set_error(const std::string& msg);
 
 struct Fail {

@@ -27,19 +27,30 @@ struct LzDev {
     int* flag;  // [0] stopped, [1] count
 };
 
+// Vector elements: complex (double2, float2) or real (double, float; the
+// imaginary part reads as zero and is not stored).
 template <class VT>
 __device__ inline void ldc(const VT* p, int64_t i, double& re, double& im) {
     const VT v = p[i];
-    re = double(v.x);
-    im = double(v.y);
+    if constexpr (is_real_vec<VT>::value) {
+        re = double(v);
+        im = 0.0;
+    } else {
+        re = double(v.x);
+        im = double(v.y);
+    }
 }
 
 template <class VT>
 __device__ inline void stc(VT* p, int64_t i, double re, double im) {
-    VT o;
-    o.x = decltype(o.x)(re);
-    o.y = decltype(o.y)(im);
-    p[i] = o;
+    if constexpr (is_real_vec<VT>::value) {
+        p[i] = VT(re);
+    } else {
+        VT o;
+        o.x = decltype(o.x)(re);
+        o.y = decltype(o.y)(im);
+        p[i] = o;
+    }
 }
 
 constexpr int RBS = 1024;     // threads of the one-workgroup reductions
@@ -62,30 +73,38 @@ __device__ inline void block_sum2(double& re, double& im) {
     im = red[1];
 }
 
+// partial[2 wg] = squared norm of the stored start vector; for real vectors
+// partial[2 wg + 1] = squared norm of the imaginary part that was dropped.
 template <class VT>
 __global__ void __launch_bounds__(BS) copy_in_kernel(const double* src, VT* dst, int64_t D, double* partial) {
-    double n2 = 0.0;
+    double n2 = 0.0, lost = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x; i < D; i += int64_t(gridDim.x) * BS) {
         const double re = src[2 * i], im = src[2 * i + 1];
         stc(dst, i, re, im);
         double r2, i2;
         ldc(dst, i, r2, i2);
         n2 += r2 * r2 + i2 * i2;
+        if constexpr (is_real_vec<VT>::value) lost += im * im;
     }
-    double z = 0.0;
-    block_sum2(n2, z);
-    if (threadIdx.x == 0) partial[2 * blockIdx.x] = n2, partial[2 * blockIdx.x + 1] = 0.0;
+    block_sum2(n2, lost);
+    if (threadIdx.x == 0) partial[2 * blockIdx.x] = n2, partial[2 * blockIdx.x + 1] = lost;
 }
 
+// flag[1] = -1 marks a start vector whose dropped imaginary part exceeds
+// 1e-10 of its norm (real vectors only); the run then stops at once.
 __global__ void __launch_bounds__(RBS) lz_init_kernel(LzDev lz, const double* partial, int64_t n) {
     double re = 0.0, im = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += RBS) re += partial[2 * k];
+    for (int64_t k = threadIdx.x; k < n; k += RBS) {
+        re += partial[2 * k];
+        im += partial[2 * k + 1];
+    }
     block_sum2<RBS>(re, im);
     if (threadIdx.x == 0) {
+        const bool lost = im > 1e-20 * (re + im);
         lz.b[0] = 0.0;
         lz.s[0] = 1.0 / std::sqrt(re);
-        lz.flag[0] = re > 0.0 ? 0 : 1;
-        lz.flag[1] = 0;
+        lz.flag[0] = (re > 0.0 && !lost) ? 0 : 1;
+        lz.flag[1] = lost ? -1 : 0;
     }
 }
 
@@ -261,6 +280,7 @@ template <class VT>
 void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
                    int* n_out, VT* d_basis, int precision, hipStream_t st, double2* acc = nullptr,
                    const double* ritz = nullptr, const ed_hmat* m = nullptr) {
+    if constexpr (is_real_vec<VT>::value) ED_REQUIRE(m, ED_ERR_ARG, "real vectors need a stored matrix");
     const int64_t D = s->D;
     const int64_t nwg = (D + BS - 1) / BS;
     const int64_t nupd = std::min<int64_t>(nwg, UPD_WGS);  // grid-stride update: at most UPD_WGS partials
@@ -293,6 +313,12 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(RBS), 0, st, lz, partial, nupd);
         ED_HIP(hipGetLastError());
         int flag[2] = {0, 0};
+        if constexpr (is_real_vec<VT>::value) {
+            ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+            ED_REQUIRE(flag[1] >= 0, ED_ERR_ARG,
+                       "start vector not real (imaginary part above 1e-10 of its norm): rotate its phase first");
+        }
         for (int n = 0; n < nsteps; ++n) {
             const int64_t np =
                 m ? launch_hmat(m, u, y, partial, precision, st) : launch_hv(s, hw, u, y, partial, precision, 1, st);
@@ -324,6 +350,17 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         throw;
     }
     cleanup();
+}
+
+// Calls f with a value of the vector type of `precision` (ed_precision).
+template <class F>
+void hmat_lanczos(int precision, F&& f) {
+    switch (precision) {
+        case ED_PREC_C128: f(double2{}); break;
+        case ED_PREC_C64: f(float2{}); break;
+        case ED_PREC_R64: f(double{}); break;
+        default: f(float{}); break;
+    }
 }
 
 }  // namespace
@@ -372,15 +409,15 @@ int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double
         ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
         const ed_sector* s = hmat_sector(m);
         ED_REQUIRE(nsteps >= 1 && a && b && n_out, ED_ERR_ARG, "bad Lanczos arguments");
-        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        hmat_require_precision(m, precision);
         *n_out = 0;
         if (s->D == 0) return;
         hipStream_t st = static_cast<hipStream_t>(stream);
         const double w0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (precision == 0)
-            lanczos_run_t<double2>(s, w0, d_v0, nsteps, a, b, n_out, nullptr, 0, st, nullptr, nullptr, m);
-        else
-            lanczos_run_t<float2>(s, w0, d_v0, nsteps, a, b, n_out, nullptr, 1, st, nullptr, nullptr, m);
+        hmat_lanczos(precision, [&](auto vt) {
+            using VT = decltype(vt);
+            lanczos_run_t<VT>(s, w0, d_v0, nsteps, a, b, n_out, nullptr, precision, st, nullptr, nullptr, m);
+        });
     });
 }
 
@@ -390,16 +427,16 @@ int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const
         ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
         const ed_sector* s = hmat_sector(m);
         ED_REQUIRE(nsteps >= 1 && ritz && d_x, ED_ERR_ARG, "bad Ritz-vector arguments");
-        ED_REQUIRE(precision == 0 || precision == 1, ED_ERR_ARG, "precision must be 0 or 1");
+        hmat_require_precision(m, precision);
         if (s->D == 0) return;
         hipStream_t st = static_cast<hipStream_t>(stream);
         const double w0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         int n = 0;
         auto* acc = reinterpret_cast<double2*>(d_x);
-        if (precision == 0)
-            lanczos_run_t<double2>(s, w0, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 0, st, acc, ritz, m);
-        else
-            lanczos_run_t<float2>(s, w0, d_v0, nsteps, nullptr, nullptr, &n, nullptr, 1, st, acc, ritz, m);
+        hmat_lanczos(precision, [&](auto vt) {
+            using VT = decltype(vt);
+            lanczos_run_t<VT>(s, w0, d_v0, nsteps, nullptr, nullptr, &n, nullptr, precision, st, acc, ritz, m);
+        });
     });
 }
 

@@ -21,6 +21,8 @@ int64_t hv_partials(const ed_sector* s);
 int hv_default_tiles(const ed_sector* s);
 int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st);
 const ed_sector* hmat_sector(const ed_hmat* m);
+bool hmat_is_real(const ed_hmat* m);
+void hmat_require_precision(const ed_hmat* m, int precision);  // throws ED_ERR_ARG
 HvWeights hmat_weights(const ed_hmat* m);
 struct StoredView;
 StoredView hmat_view(const ed_hmat* m);

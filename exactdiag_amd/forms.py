@@ -63,29 +63,43 @@ def _shortest_converged_prefix(a, b, tol, stride=8):
     return lowest(n)[1][:, 0]
 
 
-def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=30, seed=0):
+def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=30, seed=0, real=False):
     """Lowest eigenpair of a stored H by restarted two-pass Lanczos in the given precision.
 
     Pass one gives the coefficients; the shortest leading tridiagonal whose Ritz residual estimate meets the
     tolerance gives the lowest eigenvector y; pass two reruns that many steps of the recurrence and accumulates
     x = sum_n y_n v_n in FP64; restart from x until ||H x - E x|| <= tol * max(1, |E|), E the FP64 Rayleigh
-    quotient.  Returns (E, x, restarts, residual).
+    quotient.  With real=True and a real stored H (hm.is_real) the vectors are real (float64 for precision 0,
+    float32 for 1) from a real start vector; x is still returned as complex128.
+    Returns (E, x, restarts, residual).
     """
     import torch
 
+    from exactdiag_amd._lib import PREC_R32, PREC_R64
+
     dim = hm.dim
     v = krylov.device_start_vector(dim, seed)
+    real = bool(real and hm.is_real)
+    if real:
+        v = v.real.to(torch.complex128)
+        precision = PREC_R64 if precision == 0 else PREC_R32
     res = np.inf
     for restart in range(max_restarts):
         a, b = hm.lanczos(v, min(nsteps, dim), precision=precision)
         y = _shortest_converged_prefix(a, b, tol)
         x = hm.lanczos_ritz(v, y, precision=precision)
         krylov.vec_scale(x, 1.0 / krylov.vec_norm(x))
-        hx, e = hm.hv(x, with_dot=True)
-        e = e.real
-        r = hx.clone()
-        krylov.vec_axpys(x.view(1, -1), 1, [e], r)
-        res = krylov.vec_norm(r)
+        if real:
+            xr = x.real.contiguous()
+            hx, e = hm.hv(xr, with_dot=True)
+            e = e.real
+            res = float(torch.linalg.vector_norm(hx - e * xr))
+        else:
+            hx, e = hm.hv(x, with_dot=True)
+            e = e.real
+            r = hx.clone()
+            krylov.vec_axpys(x.view(1, -1), 1, [e], r)
+            res = krylov.vec_norm(r)
         if res <= tol * max(1.0, abs(e)) or len(a) < min(nsteps, dim):
             return e, x, restart, res
         v = x

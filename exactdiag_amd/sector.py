@@ -15,7 +15,7 @@ import ctypes as ct
 import numpy as np
 
 from exactdiag_amd import _lib
-from exactdiag_amd._lib import check, lib
+from exactdiag_amd._lib import PREC_C64, PREC_C128, PREC_R32, PREC_R64, check, lib
 
 TERMS = ("tl", "tr", "jl", "jr")
 
@@ -262,6 +262,23 @@ class Sector:
         return a[:k].copy(), b[:k].copy()
 
 
+def real_phase(v, tol=1e-12):
+    """v (complex128, on the device) times the unit phase that makes its largest element real and positive,
+    or None when that leaves an imaginary part above tol of the norm.  The Lanczos coefficients and spectra
+    do not depend on a global phase, so the rotated vector can run in real vectors."""
+    torch = _torch()
+    if v.numel() == 0:
+        return None
+    i = int(torch.argmax(v.abs()))
+    p = v[i]
+    if float(p.abs()) == 0.0:
+        return None
+    r = v * (p.conj() / p.abs())
+    if float(torch.linalg.vector_norm(r.imag)) > tol * float(torch.linalg.vector_norm(r)):
+        return None
+    return r
+
+
 class StoredH:
     """H of one sector and one set of weights stored on the device (sliced ELL, 6 bytes per element).
 
@@ -282,6 +299,8 @@ class StoredH:
         check(lib().ed_hmat_info(self._h, ct.byref(b), ct.byref(nnz), ct.byref(slots), ct.byref(nv)))
         self.bytes, self.nnz, self.slots, self.n_values = b.value, nnz.value, slots.value, nv.value
         self.element_bytes = int(lib().ed_hmat_element_bytes(self._h))
+        # every stored value real (momenta kx = 0, L/2): products and Lanczos runs may use real vectors
+        self.is_real = bool(lib().ed_hmat_is_real(self._h))
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -296,10 +315,13 @@ class StoredH:
             pass
 
     def hv(self, x, out=None, with_dot=False):
+        """y = H x for complex128 or complex64 x, or float64 / float32 x when :attr:`is_real`."""
         torch = _torch()
-        if x.dtype not in (torch.complex128, torch.complex64):
-            raise TypeError("x must be complex128 or complex64")
-        prec = 0 if x.dtype == torch.complex128 else 1
+        precs = {torch.complex128: PREC_C128, torch.complex64: PREC_C64, torch.float64: PREC_R64,
+                 torch.float32: PREC_R32}
+        if x.dtype not in precs:
+            raise TypeError("x must be complex128, complex64, float64 or float32")
+        prec = precs[x.dtype]
         x = x.contiguous()
         y = torch.empty_like(x) if out is None else out
         dot = torch.zeros(1, dtype=torch.complex128, device=self.device) if with_dot else None
@@ -311,7 +333,8 @@ class StoredH:
     def tune(self, precision: int = 0):
         """Autotune the stored product; returns (tiles per workgroup, gathers in flight, non-temporal, ms)."""
         torch = _torch()
-        dt = torch.complex128 if precision == 0 else torch.complex64
+        dt = {PREC_C128: torch.complex128, PREC_C64: torch.complex64, PREC_R64: torch.float64,
+              PREC_R32: torch.float32}[precision]
         x = torch.ones(max(self.dim, 1), dtype=dt, device=self.device)
         y = torch.empty_like(x)
         t, u, nt, ms = ct.c_int(0), ct.c_int(0), ct.c_int(0), ct.c_double(0.0)
@@ -324,6 +347,8 @@ class StoredH:
         check(lib().ed_hmat_set_config(self._h, int(tiles), int(unroll), int(nontemporal)))
 
     def lanczos(self, v0, nsteps: int, precision: int = 0):
+        """Plain Lanczos coefficients from v0 in vectors of `precision` (PREC_*).  The real precisions need
+        :attr:`is_real` and a real v0 (see :func:`real_phase`)."""
         torch = _torch()
         v0 = v0.to(torch.complex128).contiguous()
         a = np.zeros(nsteps, np.float64)

@@ -30,6 +30,17 @@ enum ed_status {
     ED_ERR_CAPACITY = -4  /* output capacity too small (the reference raises IndexError) */
 };
 
+/* Vector types of the products and Lanczos runs ("precision" arguments).
+ * The reference computes in complex128 throughout.  The real types serve
+ * stored matrices that ed_hmat_is_real reports real (momenta kx = 0, L/2):
+ * there a real start vector keeps every Lanczos vector real. */
+enum ed_precision {
+    ED_PREC_C128 = 0,  /* complex128 vectors */
+    ED_PREC_C64 = 1,   /* complex64 vectors, FP64 sums and reductions */
+    ED_PREC_R64 = 2,   /* float64 vectors (stored matrices only) */
+    ED_PREC_R32 = 3    /* float32 vectors, FP64 sums and reductions (stored matrices only) */
+};
+
 /* Operator ids for ed_op_columns / ed_op_apply. */
 enum ed_operator {
     ED_OP_SZQ = 0,          /* tJ_spin_half_ladder/matrix_setup.pyx:235-246 */
@@ -175,7 +186,16 @@ int ed_hmat_info(const ed_hmat* m, int64_t* bytes, int64_t* nnz, int64_t* slots,
  * 32-bit word, else 6 (the environment variable ED_HMAT_ELEMENT_BYTES=6,
  * read at ed_hmat_create, forces 6). */
 int ed_hmat_element_bytes(const ed_hmat* m);
+/* precision ED_PREC_R64 / ED_PREC_R32: d_x, d_y are float64 / float32
+ * arrays of D values and d_dot receives <x, y> with a zero imaginary part;
+ * ED_ERR_ARG unless ed_hmat_is_real. */
 int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, int precision, void* stream);
+/* 1 when every stored value is real (to 1e-13 of the largest magnitude), so
+ * that H is a real symmetric matrix in the sector's basis: the momenta kx = 0
+ * and kx = L/2 (the reference's characters exp(-i 2 pi kx r / L) are then
+ * +-1; it still computes these sectors in complex128,
+ * tJ_spin_half_ladder/symmetry_ladder.h:113-235).  0 otherwise. */
+int ed_hmat_is_real(const ed_hmat* m);
 /* Autotuner for the stored product (no counterpart in the reference): row
  * tiles of 256 per workgroup (1..32) x gathers in flight per lane (4, 8) x
  * plain or non-temporal streaming of the stored elements and the output;
@@ -185,6 +205,10 @@ int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, 
 int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll,
                  int* nontemporal, double* best_ms, void* stream);
 int ed_hmat_set_config(ed_hmat* m, int tiles, int unroll, int nontemporal);
+/* With a real precision (ED_PREC_R64, ED_PREC_R32) the start vector d_v0
+ * (complex128) must be real: ED_ERR_ARG when its imaginary part exceeds
+ * 1e-10 of its norm (rotate a global phase away first) or when the matrix is
+ * not real.  The Ritz vector d_x stays complex128 (zero imaginary part). */
 int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
                         int precision, void* stream);
 int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const double* ritz, double* d_x,
