@@ -30,6 +30,34 @@ struct is_real_vec<float> {
     static constexpr bool value = true;
 };
 
+// This thread's share of the sum of n (re, im) partials (16-byte aligned
+// pairs), NT threads striding over them with four independent loads in
+// flight; the order is fixed, so a tree over the threads gives reproducible
+// sums.
+template <int NT>
+__device__ inline void sum_partials(const double* partial, int64_t n, double& re, double& im) {
+    const double2* p = reinterpret_cast<const double2*>(partial);
+    double r[4] = {0.0, 0.0, 0.0, 0.0}, i[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t k = threadIdx.x;
+    for (; k + 3 * NT < n; k += 4 * NT) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = p[k + u * NT];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            r[u] += v[u].x;
+            i[u] += v[u].y;
+        }
+    }
+    for (; k < n; k += NT) {
+        const double2 v = p[k];
+        r[0] += v.x;
+        i[0] += v.y;
+    }
+    re = (r[0] + r[1]) + (r[2] + r[3]);
+    im = (i[0] + i[1]) + (i[2] + i[3]);
+}
+
 void set_error(const std::string& msg);
 
 struct Fail {

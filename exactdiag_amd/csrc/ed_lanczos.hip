@@ -110,11 +110,8 @@ __global__ void __launch_bounds__(RBS) lz_init_kernel(LzDev lz, const double* pa
 
 __global__ void __launch_bounds__(RBS) lz_reduce_a_kernel(LzDev lz, const double* partial, int64_t n, int step) {
     if (lz.flag[0]) return;
-    double re = 0.0, im = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += RBS) {
-        re += partial[2 * k];
-        im += partial[2 * k + 1];
-    }
+    double re, im;
+    sum_partials<RBS>(partial, n, re, im);
     block_sum2<RBS>(re, im);
     if (threadIdx.x == 0) {
         const double sn = lz.s[step];

@@ -761,14 +761,12 @@ __global__ void zero_kernel(double* p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
-// Sum of n (re, im) partials in a fixed order: 1024 threads, then a tree.
+// Sum of n (re, im) partials in a fixed order: 1024 threads, each with four
+// independent 16-byte loads in flight, then a tree.
 __global__ void __launch_bounds__(1024) reduce_partials_kernel(const double* partial, int64_t n, double* out) {
     __shared__ double red[2048];
-    double re = 0.0, im = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += 1024) {
-        re += partial[2 * k];
-        im += partial[2 * k + 1];
-    }
+    double re, im;
+    sum_partials<1024>(partial, n, re, im);
     red[2 * threadIdx.x] = re;
     red[2 * threadIdx.x + 1] = im;
     __syncthreads();
