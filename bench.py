@@ -129,6 +129,8 @@ def allq_larger(rungs):
 
     ws = np.linspace(*ALLQ_WS)
     nd = rungs - 1
+    for run in (allq.allq_fast, allq.allq_multik):  # kernel loading, untimed
+        run(6, HOLES, 5, WEIGHTS, ws, ALLQ_ETA, 40)
     t0 = time.perf_counter()
     log(f"2x{rungs}: all-q reference ({ALLQ_STEPS_REF} steps) ...")
     e0, ref = allq.allq_reference(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_REF, log=log)

@@ -263,10 +263,11 @@ class Sector:
 
 
 def real_phase(v, tol=1e-12):
-    """v (complex128, on the device) times the unit phase that makes its largest element real and positive,
+    """v (a complex128 tensor on any device) times the unit phase that makes its largest element real and positive,
     or None when that leaves an imaginary part above tol of the norm.  The Lanczos coefficients and spectra
     do not depend on a global phase, so the rotated vector can run in real vectors."""
-    torch = _torch()
+    import torch
+
     if v.numel() == 0:
         return None
     i = int(torch.argmax(v.abs()))
