@@ -70,6 +70,9 @@ class Hamiltonian_Operator:
         self.num_matrices = len(self.names)
         self._stored = None
         self.use_stored = True
+        # Lanczos runs on a real stored H (momenta kx = 0, L/2) use real vectors when the start vector is real
+        # up to a global phase: the same coefficients to rounding, about 25 % less time per step
+        self.use_real_vectors = True
 
     @property
     def stored(self):
@@ -90,9 +93,17 @@ class Hamiltonian_Operator:
         return m.hv(x) if m is not None else self.sector.hv(self._w, x)
 
     def lanczos(self, v0, nsteps, precision=0):
-        """Plain Lanczos coefficients (a, b) from the device vector v0."""
+        """Plain Lanczos coefficients (a, b) from the device vector v0 (precision 0: complex128 vectors, 1:
+        complex64; float64 / float32 instead when the stored H is real and v0 is real up to a phase)."""
         m = self.stored
         if m is not None:
+            if self.use_real_vectors and m.is_real and precision in (0, 1):
+                from exactdiag_amd._lib import PREC_R32, PREC_R64
+                from exactdiag_amd.sector import real_phase
+
+                r = real_phase(v0)
+                if r is not None:
+                    return m.lanczos(r, nsteps, precision=PREC_R64 if precision == 0 else PREC_R32)
             return m.lanczos(v0, nsteps, precision=precision)
         return self.sector.lanczos(self._w, v0, nsteps, precision=precision)
 

@@ -162,3 +162,25 @@ def test_real_ground_state_and_fast_allq(torch):
         errs = {k: allq.spectral_l1(runs[k][q], ref[q], ws) for k in runs}
         print("q", q, "L1 vs 1000 steps:", errs, flush=True)
         assert max(errs.values()) <= 1e-3, q
+
+
+def test_api_lanczos_takes_real_vectors_on_real_sectors(torch):
+    """The reference-API Hamiltonian (Hamiltonian_Operator over the stored H) runs its Lanczos in real vectors
+    on a real sector: the coefficients are those of the explicit float64 run, and equal the complex128 ones to
+    rounding."""
+    from exactdiag_amd._lib import PREC_R64
+    from exactdiag_amd.general.sparse_matrices import Hamiltonian_Operator
+    from exactdiag_amd.sector import Sector
+
+    s = Sector(10, 2, 9, 5, 1)
+    ham = Hamiltonian_Operator(s, {"tl": 1.0, "tr": 1.0, "jl": 0.4, "jr": 0.4})
+    assert ham.stored is not None and ham.stored.is_real
+    v = _real(torch, s.dim, 8).to(torch.complex128) * complex(np.exp(-1.3j))
+    a1, b1 = ham.lanczos(v, 60)
+    a2, b2 = ham.stored.lanczos(v * complex(np.exp(1.3j)), 60, precision=PREC_R64)
+    np.testing.assert_allclose(a1, a2, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(b1, b2, rtol=0, atol=1e-12)
+    ham.use_real_vectors = False
+    a3, b3 = ham.lanczos(v, 60)
+    np.testing.assert_allclose(a1, a3, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(b1, b3, rtol=0, atol=1e-10)
