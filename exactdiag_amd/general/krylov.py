@@ -119,21 +119,46 @@ def plain_lanczos(apply, v0, nsteps):
     return a[:n], b[:n]
 
 
-def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=500):
+def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=500, real=False):
     """Lowest ``nev`` eigenpairs of the Hermitian operator ``apply`` (device tensor in, out).
 
     Returns (eigenvalues ascending, device tensor (nev, D) of eigenvectors, restarts).
     Convergence: residual norm of every wanted Ritz pair below tol * max(1, |theta|).
+    real=True: ``apply`` is a real symmetric operator on float64 vectors; the basis is kept in float64 (half the
+    bytes), the real part of v0 starts it, and the eigenvectors come back as complex128 with zero imaginary part.
+    The vector helpers then see each real vector as complex pairs (length padded to even): the real part of a
+    pair-view dot is the real dot, and real coefficients give the real combinations.
     """
     torch = _torch()
     m = ncv or max(2 * nev + 24, 40)
     m = min(m, D)
-    V = torch.empty((m + 1, D), dtype=torch.complex128, device="cuda")
     if isinstance(v0, torch.Tensor):
         v = v0.to("cuda", torch.complex128)
     else:
         v = torch.as_tensor(start_vector(D) if v0 is None else np.asarray(v0, np.complex128), device="cuda")
-    V[0].copy_(v)
+    if real:
+        Dp = D + (D & 1)
+        Vr = torch.zeros((m + 1, Dp), dtype=torch.float64, device="cuda")
+        V = torch.view_as_complex(Vr.view(m + 1, Dp // 2, 2))
+        Vr[0, :D].copy_(v.real)
+        wr = torch.zeros(Dp, dtype=torch.float64, device="cuda")
+        wbuf = torch.view_as_complex(wr.view(Dp // 2, 2))
+
+        def step(j):
+            wr[:D].copy_(apply(Vr[j, :D]))
+            return wbuf
+
+        def dots(nv, w):
+            return vec_dots(V, nv, w).real.astype(np.complex128)
+    else:
+        V = torch.empty((m + 1, D), dtype=torch.complex128, device="cuda")
+        V[0].copy_(v)
+
+        def step(j):
+            return apply(V[j])
+
+        def dots(nv, w):
+            return vec_dots(V, nv, w)
     vec_scale(V[0], 1.0 / vec_norm(V[0]))
     T = np.zeros((m, m))
     k = 0
@@ -141,10 +166,10 @@ def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=
     for restart in range(max_restarts):
         m_eff = m
         for j in range(k, m):
-            w = apply(V[j])
-            c = vec_dots(V, j + 1, w)
+            w = step(j)
+            c = dots(j + 1, w)
             vec_axpys(V, j + 1, c, w)
-            c2 = vec_dots(V, j + 1, w)
+            c2 = dots(j + 1, w)
             vec_axpys(V, j + 1, c2, w)
             T[j, j] = (c[j] + c2[j]).real
             beta = vec_norm(w)
@@ -162,14 +187,16 @@ def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=
         want = min(nev, m_eff)
         conv = res[:want] < tol * np.maximum(1.0, np.abs(theta[:want]))
         if np.all(conv) or m_eff == D or beta == 0.0:
-            X = torch.empty((want, D), dtype=torch.complex128, device="cuda")
+            X = torch.empty((want, V.shape[1]), dtype=torch.complex128, device="cuda")
             for i in range(want):
                 vec_combine(V, m_eff, Y[:, i], X[i])
                 vec_scale(X[i], 1.0 / vec_norm(X[i]))
+            if real:
+                X = torch.view_as_real(X).reshape(want, -1)[:, :D].to(torch.complex128)
             _logger.info(f"thick-restart Lanczos converged after {restart} restarts, residuals {res[:want]}")
             return theta[:want], X, restart
         k = min(m_eff - 1, nev + max(nev, (m_eff - nev) // 2))
-        tmp = torch.empty((k, D), dtype=torch.complex128, device="cuda")
+        tmp = torch.empty((k, V.shape[1]), dtype=torch.complex128, device="cuda")
         for i in range(k):
             vec_combine(V, m_eff, Y[:, i], tmp[i])
         V[:k].copy_(tmp)

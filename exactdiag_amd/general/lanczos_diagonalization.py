@@ -92,7 +92,11 @@ def diagonalize_IRLM(num_eigenpairs, operator, is_symmetric=False, num_added_vec
         return operator.apply(v)
 
     ncv = max(2 * num_eigenpairs + 24, 40, num_eigenpairs + num_added_vecs)
-    vals, vecs, it = krylov.lowest_eigenpairs(apply, D, num_eigenpairs, tol=tol, ncv=ncv, v0=initial_guess)
+    # a real stored H (momenta kx = 0, L/2): the Krylov basis in float64, half the bytes per vector
+    stored = getattr(operator, "stored", None)
+    real = bool(stored is not None and stored.is_real and getattr(operator, "use_real_vectors", False))
+    vals, vecs, it = krylov.lowest_eigenpairs(apply, D, num_eigenpairs, tol=tol, ncv=ncv, v0=initial_guess,
+                                              real=real)
     if device:
         return (vals, vecs), it
     return (vals, vecs.cpu().numpy().T.copy()), it

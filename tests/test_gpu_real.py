@@ -184,3 +184,28 @@ def test_api_lanczos_takes_real_vectors_on_real_sectors(torch):
     a3, b3 = ham.lanczos(v, 60)
     np.testing.assert_allclose(a1, a3, rtol=0, atol=1e-10)
     np.testing.assert_allclose(b1, b3, rtol=0, atol=1e-10)
+
+
+def test_thick_restart_in_real_vectors(torch, oracle_mod):
+    """The eigenpair solver in float64 vectors on a real sector: the energies of the complex128 run and of the
+    oracle's dense spectrum, eigenvectors real with small residuals, also for an odd dimension (padded pair
+    view)."""
+    from exactdiag_amd.general import krylov
+    from exactdiag_amd.sector import Sector
+
+    for p in [(10, 2, 9, 0, 0), (6, 2, 5, 3, 1), (7, 1, 6, 0, 1)]:
+        s = Sector(*p)
+        m = s.store(WEIGHTS)
+        assert m.is_real
+        print("sector", p, "dim", s.dim, flush=True)
+        ec, Xc, _ = krylov.lowest_eigenpairs(m.hv, s.dim, 3, tol=1e-10)
+        er, Xr, _ = krylov.lowest_eigenpairs(m.hv, s.dim, 3, tol=1e-10, real=True)
+        np.testing.assert_allclose(er, ec, rtol=0, atol=1e-9 * max(1.0, abs(ec[0])))
+        assert Xr.dtype == torch.complex128 and Xr.shape == (3, s.dim)
+        assert float(Xr.imag.abs().max()) == 0.0
+        for i in range(3):
+            r = m.hv(Xr[i]) - er[i] * Xr[i]
+            assert float(torch.linalg.vector_norm(r)) <= 1e-8 * max(1.0, abs(er[i]))
+        if s.dim <= 3000:
+            ev = np.linalg.eigvalsh(oracle_mod.OracleSector(*p).toarray(WEIGHTS))
+            np.testing.assert_allclose(er, ev[:3], rtol=0, atol=1e-9 * max(1.0, abs(ev[0])))
