@@ -465,7 +465,7 @@ def positions_form(reps=20, log=print):
     s = Sector(12, 2, 11, 0, 0)
     hm = s.store(weights_array([1.0, 1.0, 0.4, 0.4]))
     t0 = time.perf_counter()
-    e0, x, _, res = two_pass_ground_state(hm, tol=1e-9)
+    e0, x, _, res = two_pass_ground_state(hm, tol=1e-9, real=True)
     _sync()
     t_gs = time.perf_counter() - t0
     corr, ms, wall = timed(lambda: s.position_correlations(_lib.POS_HOLES, np.zeros((0, 2), np.int32), x))
@@ -480,7 +480,7 @@ def positions_form(reps=20, log=print):
     s4 = Sector(10, 4, 8, 0, 0)
     hm = s4.store(weights_array([0.45, 0.36, 0.15, 0.12]))
     t0 = time.perf_counter()
-    e0, x, _, res = two_pass_ground_state(hm, tol=1e-9)
+    e0, x, _, res = two_pass_ground_state(hm, tol=1e-9, real=True)
     _sync()
     t_gs = time.perf_counter() - t0
     fixed = np.array([[1, 1], [1, 1]], np.int32)
