@@ -448,6 +448,10 @@ def positions_form(reps=20, log=print):
 
     out, states = {}, {}
     stream = torch.cuda.current_stream()
+    # untimed: one small ground state first, so that the timed ones do not pay for loading kernels
+    _warm = Sector(6, 2, 5, 0, 0).store(weights_array([1.0, 1.0, 0.4, 0.4]))
+    two_pass_ground_state(_warm, tol=1e-9, real=True)
+    _warm.close()
 
     def timed(fn):
         fn()
