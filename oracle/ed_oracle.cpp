@@ -832,6 +832,9 @@ int64_t orc_op_columns(void* pi, void* pf, int op, int qx, int qy, int threads, 
 }
 
 // Operator site weights as the reference builds them (complex pairs).
+// q.r of general/group/symmetry_generator.h:87-102 for one site (radians).
+double orc_q_dot_r(int L, int site, int qx, int qy) { return q_dot_r(site, qx, qy, L); }
+
 void orc_op_weights(int L, int op, int qx, int qy, double* out) {
     Rule r = operator_rule(L, op, qx, qy);
     for (size_t c = 0; c < r.weights.size(); ++c) {

@@ -69,6 +69,8 @@ def lib():
         L.orc_op_columns.restype = ct.c_int64
         L.orc_op_columns.argtypes = [ct.c_void_p, ct.c_void_p] + [ct.c_int] * 4 + [_i64p, _i64p, _dp]
         L.orc_op_weights.argtypes = [ct.c_int] * 4 + [_dp]
+        L.orc_q_dot_r.restype = ct.c_double
+        L.orc_q_dot_r.argtypes = [ct.c_int] * 4
         L.orc_build_term_csr.argtypes = [ct.c_void_p, ct.c_int, ct.c_int]
         L.orc_build_term_csr_rows.argtypes = [ct.c_void_p, ct.c_int, ct.c_int64, ct.c_int64, ct.c_int]
         L.orc_term_csr_sizes.restype = ct.c_int64
@@ -109,6 +111,7 @@ def ref():
         R.ref_rank.argtypes = [ct.c_void_p, _i16p]
         R.ref_orbit.argtypes = [ct.c_void_p, _i16p, ct.c_int, _i16p, _dp]
         R.ref_lowest.argtypes = [ct.c_void_p, _i16p, _i64p, _dp]
+        R.ref_q_dot_r.argtypes = [ct.c_int] * 4 + [_dp]
         R.ref_csr_dot_rows.argtypes = [ct.c_int64, ct.c_int64, ct.c_int64, _i64p, _i16p, _i64p, _dp, _dp, _dp,
                                        ct.c_double, ct.c_double, ct.c_int]
         R.ref_csr_dot.argtypes = [ct.c_int64, _i64p, _i16p, _i64p, _dp, _dp, _dp, ct.c_double, ct.c_double,
@@ -302,10 +305,24 @@ class OracleSector:
         return a[:n].copy(), b[:n].copy()
 
 
+def q_dot_r(L, site, qx, qy):
+    """The oracle's q.r (radians) of one site."""
+    return lib().orc_q_dot_r(L, site, qx, qy)
+
+
+def ref_q_dot_r(L, site, qx, qy):
+    """The reference's own get_q_dot_r (compiled in oracle/_ref): (real, imag)."""
+    out = np.zeros(2)
+    ref().ref_q_dot_r(L, site, qx, qy, _ptr(out, _dp))
+    return out[0], out[1]
+
+
 def op_weights(L, op, qx, qy):
-    out = np.empty(4 * 2 * L, np.float64)
+    """(diagonal, off-diagonal) weight of every combination of an operator: 2L rows, 4L for the leg current."""
+    n = 2 * L * (2 if op == OP_CURRENT_LEG else 1)
+    out = np.empty(4 * n, np.float64)
     lib().orc_op_weights(L, op, qx, qy, _ptr(out, _dp))
-    return out.view(np.complex128).reshape(2 * L, 2)
+    return out.view(np.complex128).reshape(n, 2)
 
 
 def cf_imag(a, b, zs):

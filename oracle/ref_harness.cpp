@@ -111,6 +111,16 @@ void ref_lowest(void* p, const short* st, int64_t* ind, double* amp) {
     amp[1] = ia.amplitude.imag();
 }
 
+// The reference's I_Symmetry_Generator::get_q_dot_r
+// (general/group/symmetry_generator.h:87-102) on the ladder's generator, for
+// one site and one integer q: out = {real, imag}.
+void ref_q_dot_r(int L, int site, int qx, int qy, double* out) {
+    Symmetries_Single_Spin_Half_2leg_Ladder sym(L, std::vector<int>{0, 0});
+    const std::complex<double> v = sym.get_q_dot_r(pos_int(site), std::vector<int>{qx, qy});
+    out[0] = v.real();
+    out[1] = v.imag();
+}
+
 // Row-major Hermitian-triangle sparse dot as driven by
 // general/cython_sparse_dot.pyx:40-80 (driver restated), calling the
 // reference kernels of general/cpp_sparse_dot.h:59-71 and :104-138.

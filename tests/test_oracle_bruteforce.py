@@ -138,7 +138,21 @@ def test_basis_and_hamiltonian(sector):
         assert np.max(np.abs(y - Hs @ (np.arange(o.dim) + 1j))) < 1e-11
 
 
-def _fock_operator(fk_i, fk_f, op, qx, qy):
+def _hop(st, t, p):
+    """c^dagger_t c_p (same spin) on a Fock state: (sign, new state) or None."""
+    if st[t] != 0 or st[p] == 0:
+        return None
+    sg = ann_sign(st, p)
+    mid = list(st)
+    mid[t], mid[p] = st[p], 0
+    return sg * ann_sign(mid, t), tuple(mid)
+
+
+def _fock_operator(fk_i, fk_f, op, qx, qy, mutate=None):
+    """Excitation operators from explicit fermion operators.  qy is the physical momentum transfer across the
+    legs; for the rung current the final sector's ky is shifted by one more (the current is odd under the leg
+    mirror).  `mutate` plants a mistake (a dropped fermion sign, a flipped endpoint sign, a missing i, a
+    rung current without the mirror sign) to show that the comparison catches it."""
     L, N = fk_i.L, fk_i.N
     O = np.zeros((len(fk_f.states), len(fk_i.states)), np.complex128)
     for c, st in enumerate(fk_i.states):
@@ -147,6 +161,31 @@ def _fock_operator(fk_i, fk_f, op, qx, qy):
             if op == oracle.OP_SZQ:
                 if st[p] != 0:
                     O[fk_f.index[st], c] += ph * (0.5 if st[p] == 2 else -0.5) / sqrt(N)
+            elif op == oracle.OP_NQ:
+                # N(q) = N^-1/2 sum_p e^{-iq.r_p} n_p
+                if st[p] != 0:
+                    O[fk_f.index[st], c] += ph / sqrt(N)
+            elif op == oracle.OP_CURRENT_RUNG:
+                # i sum_x e^{-i qx x} (c^dag_{x,1} c_{x,0} - (-1)^qy c^dag_{x,0} c_{x,1})
+                x, y = divmod(p, 2)
+                coef = 1j * np.exp(-2j * np.pi * qx * x / L)
+                if y == 1 and mutate != "rung_mirror":
+                    coef *= -((-1) ** qy)
+                res = _hop(st, 2 * x + (1 - y), p)
+                if res is not None:
+                    O[fk_f.index[res[1]], c] += coef * res[0]
+            elif op == oracle.OP_CURRENT_LEG:
+                # i sum_p e^{-iq.r_p} (c^dag_{p+x} c_p - c^dag_{p-x} c_p), fermion signs from the algebra
+                x, y = divmod(p, 2)
+                for dx, e in ((1, 1.0), (-1, -1.0)):
+                    if mutate == "leg_endpoint" and dx == -1:
+                        e = 1.0
+                    res = _hop(st, 2 * ((x + dx) % L) + y, p)
+                    if res is None:
+                        continue
+                    sg = 1 if mutate == "no_sign" else res[0]
+                    i_factor = 1.0 if mutate == "no_i" else 1j
+                    O[fk_f.index[res[1]], c] += i_factor * ph * e * sg
             elif op in (oracle.OP_APLUS_DOWN, oracle.OP_APLUS_UP):
                 if st[p] == 0:
                     new = list(st)
@@ -169,22 +208,52 @@ OPS = [((3, 2, 2, 0, 0), oracle.OP_SZQ, (1, 1), (3, 2, 2, 1, 1)),
        ((4, 1, 3, 0, 1), oracle.OP_AMINUS_UP, (1, 0), (4, 2, 3, 1, 1))]
 
 
-@pytest.mark.parametrize("case", OPS)
-def test_operators(case):
+# N(q) and the two currents (matrix_setup.pyx:248-279).  The rung current is called with qy + 1, as
+# setup_excitation_operator does (matrix_setup.pyx:251), and its final sector has ky + qy + 1.
+OPS += [((3, 1, 2, 0, 0), oracle.OP_NQ, (1, 0), (3, 1, 2, 1, 0)),
+        ((4, 2, 3, 1, 1), oracle.OP_NQ, (2, 1), (4, 2, 3, -1, 0)),
+        ((3, 1, 2, 0, 0), oracle.OP_CURRENT_RUNG, (0, 0), (3, 1, 2, 0, 1)),
+        ((4, 2, 3, 1, 0), oracle.OP_CURRENT_RUNG, (1, 1), (4, 2, 3, 2, 0)),
+        ((4, 1, 3, 0, 1), oracle.OP_CURRENT_RUNG, (2, 0), (4, 1, 3, 2, 0)),
+        ((3, 1, 2, 0, 0), oracle.OP_CURRENT_LEG, (1, 0), (3, 1, 2, 1, 0)),
+        ((4, 2, 3, 1, 1), oracle.OP_CURRENT_LEG, (1, 1), (4, 2, 3, 2, 0)),
+        ((4, 1, 3, 0, 0), oracle.OP_CURRENT_LEG, (2, 1), (4, 1, 3, 2, 1)),
+        ((3, 2, 2, 1, 0), oracle.OP_CURRENT_LEG, (0, 0), (3, 2, 2, 1, 0))]
+
+
+def _operator_pair(case, mutate=None):
+    """(brute-force operator in the symmetric bases, the oracle's operator matrix)."""
     si, op, (qx, qy), sf = case
     fi, ff = Fock(si[0], si[1], si[2]), Fock(sf[0], sf[1], sf[2])
     _, Bi = fi.symmetric_basis(si[3], si[4])
     _, Bf = ff.symmetric_basis(sf[3], sf[4])
-    Os = Bf.conj().T @ _fock_operator(fi, ff, op, qx, qy) @ Bi
+    Os = Bf.conj().T @ _fock_operator(fi, ff, op, qx, qy, mutate) @ Bi
     oi, of = OracleSector(*si), OracleSector(*sf)
-    ptr, rows, vals = oi.op_columns(of, op, qx, qy)
+    ptr, rows, vals = oi.op_columns(of, op, qx, qy + (1 if op == oracle.OP_CURRENT_RUNG else 0))
     M = np.zeros((of.dim, oi.dim), np.complex128)
     for i in range(oi.dim):
         for k in range(ptr[i], ptr[i + 1]):
             M[rows[k], i] += vals[k]
-    # the reference's q.r uses single-precision division (~1e-7 phase error)
+    return Os, M
+
+
+@pytest.mark.parametrize("case", OPS)
+def test_operators(case):
+    Os, M = _operator_pair(case)
+    # the reference's q.r uses single-precision division (~1e-7 phase error; pinned bit for bit against the
+    # reference's own get_q_dot_r in test_oracle_ref.py)
     assert np.max(np.abs(M - Os)) < 2e-6, case
     assert np.max(np.abs(Os)) > 0.05
+
+
+@pytest.mark.parametrize("case,mutate", [(OPS[-3], "no_sign"), (OPS[-2], "leg_endpoint"), (OPS[-4], "no_i"),
+                                         (OPS[-5], "rung_mirror"), (OPS[-7], "rung_mirror")])
+def test_operator_check_catches_planted_mistakes(case, mutate):
+    """A dropped fermion sign, a flipped endpoint weight, a missing factor i or a rung current without its
+    mirror sign in the brute force moves it away from the oracle: the comparison above has the power to
+    see such a slip on either side."""
+    Os, M = _operator_pair(case, mutate)
+    assert np.max(np.abs(M - Os)) > 1e-2, (case, mutate)
 
 
 def _site(L, i, leg, rung):

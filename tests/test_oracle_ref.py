@@ -71,3 +71,84 @@ def test_reference_sparse_dot_matches_oracle():
         oracle.ref_csr_dot(values, vinds, minor, major, x, y_ref, w[t], first, threads=4)
         first = False
     assert np.array_equal(y_oracle, y_ref)
+
+
+def test_q_dot_r_bit_exact_against_reference():
+    """The oracle's q.r equals the reference's compiled get_q_dot_r (general/group/symmetry_generator.h:87-102)
+    bit for bit on every site of every ladder up to 2x16 and every integer q in [-L, L] x [-1, 2]."""
+    n_differ_from_double = 0
+    for L in range(2, 17):
+        for site in range(2 * L):
+            x, y = divmod(site, 2)
+            for qx in range(-L, L + 1):
+                for qy in (-1, 0, 1, 2):
+                    re, im = oracle.ref_q_dot_r(L, site, qx, qy)
+                    assert im == 0.0
+                    ours = oracle.q_dot_r(L, site, qx, qy)
+                    assert np.float64(ours).tobytes() == np.float64(re).tobytes(), (L, site, qx, qy)
+                    # the same sum in double precision differs in the last bits at many sites: a
+                    # double-precision q.r in the oracle would fail the assertion above
+                    if (qx * x / L + qy * y / 2) * 2.0 * np.pi != re:
+                        n_differ_from_double += 1
+    assert n_differ_from_double > 1000
+
+
+def _reference_iqr_weights(L, sites, base, q):
+    """get_iqr_weights (general/matrix_rules_utils.pyx:90-119) restated in numpy around the reference's own q.r."""
+    out = np.empty((len(sites), 2), np.complex128)
+    for i, site in enumerate(sites):
+        re, im = oracle.ref_q_dot_r(L, site, q[0], q[1])
+        out[i] = np.array(base) * np.exp(-1j * complex(re, im))
+    return out
+
+
+@pytest.mark.parametrize("L", [3, 4, 6, 7, 12])
+def test_operator_weights_against_reference_q_dot_r(L):
+    """The weights of Szq, A+-, N(q) and the currents (matrix_setup.pyx:235-325) as the reference's Python forms
+    them from its compiled q.r (np.exp, the product of start and end weights in itertools.product order), against
+    the oracle's weights, bit for bit."""
+    N = 2 * L
+    sites = list(range(N))
+    for qx in range(-(L - 1) // 2, L // 2 + 1):
+        for qy in (0, 1, 2):
+            expect = {
+                oracle.OP_SZQ: _reference_iqr_weights(L, sites, (0, 0.5 / np.sqrt(N)), (qx, qy)),
+                oracle.OP_APLUS_DOWN: _reference_iqr_weights(L, sites, (0, 1 / np.sqrt(N)), (qx, qy)),
+                oracle.OP_AMINUS_UP: _reference_iqr_weights(L, sites, (0, 1 / np.sqrt(N)), (qx, qy)),
+            }
+            start = _reference_iqr_weights(L, sites, (0, 1j), (qx, qy))
+            expect[oracle.OP_CURRENT_RUNG] = np.array([s * e for s, e in itertools.product(start, np.array([(0, 1)]))])
+            expect[oracle.OP_CURRENT_LEG] = np.array(
+                [s * e for s, e in itertools.product(start, np.array([(0, 1), (0, -1)]))])
+            for op, w in expect.items():
+                got = oracle.op_weights(L, op, qx, qy)
+                assert got.shape == w.shape, (op, got.shape, w.shape)
+                assert np.array_equal(got.view(np.float64), w.view(np.float64)), (L, op, qx, qy)
+
+
+def test_row_prefix_csr_and_reference_dot_rows():
+    """The CPU baseline's building blocks: a term's CSR built for a row prefix (build_csr_rows), read back with
+    term_csr (major holds exactly n_rows + 1 pointers) and applied by the reference's row kernels
+    (ref_csr_dot_rows), equals those rows of the full upper triangle applied densely."""
+    p = (6, 2, 5, 1, 0)
+    full, pref = OracleSector(*p), OracleSector(*p)
+    D = full.dim
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=D) + 1j * rng.normal(size=D)
+    for term, n_rows in ((0, 257), (2, 700), (3, D)):
+        values, vinds, minor, major = full.term_csr(term)
+        U = np.zeros((D, D), np.complex128)
+        for r in range(D):
+            for k in range(major[r], major[r + 1]):
+                U[r, minor[k]] += values[vinds[k]]
+        Up = U.copy()
+        Up[n_rows:] = 0
+        w = 0.37 - 0.21j
+        expect = w * (Up @ x + (np.triu(Up, 1).conj().T @ x))
+        pref.build_csr_rows(term, 0, n_rows)
+        pv, pvi, pmin, pmaj = pref.term_csr(term)
+        assert len(pmaj) == n_rows + 1 and pmaj[-1] == len(pmin) == major[n_rows]
+        assert np.array_equal(pmin, minor[:major[n_rows]])
+        y = np.zeros(D, np.complex128)
+        oracle.ref_csr_dot_rows(D, 0, pv, pvi, pmin, pmaj, x, y, w, threads=4)
+        assert np.max(np.abs(y - expect)) < 1e-12 * max(1.0, np.max(np.abs(expect))), term
