@@ -23,7 +23,7 @@ PREC_C128, PREC_C64, PREC_R64, PREC_R32 = range(4)
 EXPORTED = [
     "ed_last_error", "ed_version", "ed_sector_create", "ed_sector_destroy", "ed_basis_build", "ed_basis_size",
     "ed_basis_export", "ed_lookup", "ed_lowest", "ed_term_columns", "ed_op_columns", "ed_hv", "ed_op_apply",
-    "ed_lanczos_run", "ed_cf_eval", "ed_cf_eval_tail", "ed_vec_dots", "ed_vec_axpys", "ed_vec_combine", "ed_vec_norm2", "ed_vec_scale",
+    "ed_lanczos_run", "ed_cf_eval", "ed_cf_eval_tail", "ed_vec_dots", "ed_vec_dots_noalloc", "ed_vec_axpys", "ed_vec_combine", "ed_vec_norm2", "ed_vec_scale",
     "ed_mlp_create", "ed_mlp_destroy", "ed_mlp_num_params", "ed_mlp_get_params", "ed_mlp_set_params",
     "ed_mlp_forward", "ed_mlp_gradients", "ed_mlp_train_step", "ed_gemm_bf16",
     "ed_batch_create", "ed_batch_destroy", "ed_batch_offsets", "ed_batch_hv", "ed_batch_lanczos", "ed_batch_tune",
@@ -61,6 +61,7 @@ _SIGNATURES = {
     "ed_cf_eval": (ct.c_int, [_dp, _dp, ct.c_int, _dp, ct.c_int, ct.c_double, ct.c_double, ct.c_double, _dp]),
     "ed_cf_eval_tail": (ct.c_int, [_dp, _dp, ct.c_int, _dp, ct.c_int] + [ct.c_double] * 5 + [_dp]),
     "ed_vec_dots": (ct.c_int, [_vp, ct.c_int, _i64, _vp, _i64, _dp, _vp]),
+    "ed_vec_dots_noalloc": (ct.c_int, [_vp, ct.c_int, _i64, _vp, _i64, _dp, _vp]),
     "ed_vec_axpys": (ct.c_int, [_vp, ct.c_int, _i64, _dp, _vp, _i64, _vp]),
     "ed_vec_combine": (ct.c_int, [_vp, ct.c_int, _i64, _dp, _vp, _i64, _vp]),
     "ed_vec_norm2": (ct.c_int, [_vp, _i64, _dp, _vp]),

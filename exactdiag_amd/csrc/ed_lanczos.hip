@@ -10,6 +10,7 @@
 // 64 steps.  All reductions are fixed-order, so runs are reproducible.
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "ed_internal.h"
@@ -468,23 +469,82 @@ int ed_cf_eval_tail(const double* a, const double* b, int n, const double* ws, i
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+constexpr int DOTS_MAX_WGS = 1024;  // workgroups of dots_kernel per vector
+
+int dots_workgroups(int64_t D) { return int(std::min<int64_t>(std::max<int64_t>((D + BS - 1) / BS, 1), DOTS_MAX_WGS)); }
+
+// out[k] = <V_k, w>: partials (2 * nwg * nv doubles) and the reduced dots (2 * nv) in device memory the caller
+// provides, then copied to the host out.
+hipError_t vec_dots_into(const double* d_V, int nv, int64_t ld, const double* d_w, int64_t D, double* partial,
+                         double* dout, double* out, hipStream_t st) {
+    const int nwg = dots_workgroups(D);
+    hipLaunchKernelGGL(dots_kernel, dim3(unsigned(nwg), unsigned(nv)), dim3(BS), 0, st, d_V, ld, d_w, D, partial, nwg);
+    hipLaunchKernelGGL(dots_reduce_kernel, dim3(unsigned(nv)), dim3(BS), 0, st, partial, nwg, dout);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, size_t(2 * nv) * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
+
+// Fixed scratch of ed_vec_dots_noalloc: the partials of at most NOALLOC_MAX_VECTORS vectors, then their
+// reduced dots (2 MiB + 2 KiB).  Every GPU the library runs on holds its own copy until the process ends.
+constexpr int NOALLOC_MAX_VECTORS = 128;
+constexpr int64_t NOALLOC_PARTIALS = int64_t(2) * DOTS_MAX_WGS * NOALLOC_MAX_VECTORS;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS + 2 * NOALLOC_MAX_VECTORS];
+
+// Device address of `scratch` on the current device, looked up once per device.  The array's name on the
+// host is only a stand-in for the symbol: kernels and copies get this address.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+double* noalloc_scratch() {
+    int dev = 0;
+    ED_HIP(hipGetDevice(&dev));
+    ED_REQUIRE(dev >= 0 && dev < MAX_DEVICES, ED_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        ED_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ED_REQUIRE(p, ED_ERR_HIP, "no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
 int ed_vec_dots(const double* d_V, int nv, int64_t ld, const double* d_w, int64_t D, double* out, void* stream) {
     return guarded([&] {
         ED_REQUIRE(nv >= 0 && nv <= 65535 && D >= 0, ED_ERR_ARG, "bad sizes");
         if (nv == 0) return;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        const int nwg = int(std::min<int64_t>(std::max<int64_t>((D + BS - 1) / BS, 1), 1024));
+        const int nwg = dots_workgroups(D);
         double* partial = dev_alloc<double>(size_t(2) * nwg * nv);
         double* dout = dev_alloc<double>(size_t(2 * nv));
-        hipLaunchKernelGGL(dots_kernel, dim3(unsigned(nwg), unsigned(nv)), dim3(BS), 0, st, d_V, ld, d_w, D, partial,
-                           nwg);
-        hipLaunchKernelGGL(dots_reduce_kernel, dim3(unsigned(nv)), dim3(BS), 0, st, partial, nwg, dout);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, size_t(2 * nv) * sizeof(double), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        const hipError_t e = vec_dots_into(d_V, nv, ld, d_w, D, partial, dout, out, st);
         dev_free(partial);
         dev_free(dout);
         ED_HIP(e);
+    });
+}
+
+int ed_vec_dots_noalloc(const double* d_V, int nv, int64_t ld, const double* d_w, int64_t D, double* out,
+                        void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(nv >= 0 && D >= 0, ED_ERR_ARG, "bad sizes");
+        ED_REQUIRE(nv <= NOALLOC_MAX_VECTORS, ED_ERR_CAPACITY,
+                   "ed_vec_dots_noalloc takes at most 128 vectors (its fixed scratch holds their partials)");
+        if (nv == 0) return;
+        double* scr = noalloc_scratch();
+        // dots_kernel writes every partial it reads back, so nothing in the scratch needs clearing
+        ED_HIP(vec_dots_into(d_V, nv, ld, d_w, D, scr, scr + NOALLOC_PARTIALS, out, static_cast<hipStream_t>(stream)));
     });
 }
 

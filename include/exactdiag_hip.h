@@ -380,6 +380,15 @@ int ed_cf_eval_tail(const double* a, const double* b, int n, const double* ws, i
  * ed_vec_axpys: w -= sum_k c[k] V_k (c host).  ed_vec_combine: out = sum_k
  * c[k] V_k (c host).  ed_vec_norm2: host out = ||w||^2.  ed_vec_scale: w *= c. */
 int ed_vec_dots(const double* d_V, int nv, int64_t ld, const double* d_w, int64_t D, double* out, void* stream);
+/* ed_vec_dots without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * partials live in a fixed 2 MiB array per GPU, held until the process ends;
+ * at most 128 vectors (ED_ERR_CAPACITY beyond).  Results equal ed_vec_dots bit
+ * for bit.  Calls that overlap on one GPU (other threads or streams) share
+ * that array and race. */
+int ed_vec_dots_noalloc(const double* d_V, int nv, int64_t ld, const double* d_w, int64_t D, double* out,
+                        void* stream);
 int ed_vec_axpys(const double* d_V, int nv, int64_t ld, const double* c, double* d_w, int64_t D, void* stream);
 int ed_vec_combine(const double* d_V, int nv, int64_t ld, const double* c, double* d_out, int64_t D, void* stream);
 int ed_vec_norm2(const double* d_w, int64_t D, double* out, void* stream);
