@@ -175,6 +175,47 @@ def sqrt_terminator(a, b, n=N_IN):
     return np.asarray(a[:n]), np.asarray(b[:n]), (c, 0.5 * w)
 
 
+def evaluate_allq(model, L=12, h=2, nd=11, weights=(1.0, 1.0, 0.4, 0.4), ws=None, eta=0.05, nsteps_ref=1000):
+    """S(q, w) of every distinct q of a ladder closed at 64 levels, with the network and with the square-root
+    terminator, each against the 1000-step FP64 spectrum (stored H, ground state to 1e-12).  Returns per-q
+    spectral L1 errors and the seconds the network's forward passes took."""
+    import time
+
+    from exactdiag_amd import _lib
+    from exactdiag_amd.allq import _gs, distinct_qs, final_momentum, spectral_l1
+    from exactdiag_amd.general.krylov import vec_norm
+    from exactdiag_amd.sector import Sector, cf_eval
+
+    ws = np.linspace(0, 4, 2001) if ws is None else np.asarray(ws)
+    w = list(weights)
+    s0 = Sector(L, h, nd, 0, 0)
+    hm0 = s0.store(w)
+    e0, psi = _gs(hm0.hv, s0.dim, 1e-12)
+    hm0.close()
+    per_q, t_net = {}, 0.0
+    for qx, qy in distinct_qs(L):
+        s1 = Sector(L, h, nd, *final_momentum(L, 0, 0, qx, qy))
+        phi = s0.op_apply(s1, _lib.OP_SZQ, qx, qy, psi)
+        n2 = vec_norm(phi) ** 2
+        if n2 < 1e-14:
+            s1.close()
+            continue
+        hm = s1.store(w)
+        a, b = hm.lanczos(phi, nsteps_ref)
+        hm.close()
+        s1.close()
+        f = -n2 / np.pi
+        ref = cf_eval(a, b, ws, e0, eta, f)
+        at, bt, tail = sqrt_terminator(a, b)
+        t1 = time.perf_counter()
+        ae, be, tail2 = model.extend(a, b)  # reads the first 64 levels only
+        t_net += time.perf_counter() - t1
+        per_q[(qx, qy)] = (spectral_l1(cf_eval(at, bt, ws, e0, eta, f, tail=tail), ref, ws),
+                           spectral_l1(cf_eval(ae, be, ws, e0, eta, f, tail=tail2), ref, ws))
+    s0.close()
+    return per_q, t_net
+
+
 # -- training data ----------------------------------------------------------------
 
 def lanczos_sequences(L, h, nd, weights, nsteps=1000, ops=("Szq", "Nq", "A+", "A-"), limited=True, seed=0):

@@ -24,6 +24,7 @@
 // order, in two passes (count, fill).
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -138,46 +139,99 @@ __global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __res
     }
 }
 
-// Order each row's stored elements by column (ties by value index and sign),
-// so that the lanes of a wavefront reading slot k gather from nearby columns.
-// One wavefront per 64-row slice, one lane per row: a row of up to 32
-// elements is sorted by a bitonic network in registers; one of up to
-// SORT_CAP is copied into the lane's own LDS column (K[k][lane], no bank
-// conflicts), insertion-sorted there and written back; a longer row is
-// insertion-sorted in place in HBM.  The key is the packed word, or
-// (column << 16) | code for the two-array form.
+// Order each row's stored elements so that the 64 lanes of a wavefront, which
+// read slot k of 64 consecutive rows f, gather x[f + d] with one offset d where
+// possible: their addresses are then consecutive (16 lines for complex128
+// instead of up to 64).  One wavefront per 64-row slice, one lane per row.
+// ALIGNED: every element's offset d = column - f is counted over the slice (an
+// LDS hash table), and a row's elements are sorted by that count, largest
+// first, then by column; offsets shared by all rows of a slice (exchange on
+// bonds above the spins that vary within the slice, whose rank change does not
+// depend on those spins) come first and in the same order in every row.
+// Otherwise the rows are sorted by column alone.  A row of up to 32 elements is
+// sorted by a bitonic network in registers; one of up to SORT_CAP is copied
+// into the lane's own LDS column (K[k][lane], no bank conflicts) and
+// insertion-sorted there; a longer row is insertion-sorted in place in HBM.
+// The key is (255 - count) << 32 | packed word, or (255 - count) << 48 |
+// column << 16 | code for the two-array form (count 0 when not ALIGNED).
 constexpr int SORT_CAP = 64;
+constexpr int ALIGN_HASH = 2048;  // offsets tracked per slice (power of two); more are counted as unshared
+constexpr int32_t NO_OFFSET = INT32_MIN;
 
-template <bool PACKED>
+__device__ inline uint32_t offset_slot(int32_t d) { return (uint32_t(d) * 2654435761u) >> 21; }  // 11 bits
+
+template <bool PACKED, bool ALIGNED>
 __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* __restrict__ sptr,
                                                      const uint8_t* __restrict__ cnt, int32_t* col, uint16_t* code,
-                                                     uint32_t* pack) {
-    using Key = typename std::conditional<PACKED, uint32_t, uint64_t>::type;
-    __shared__ Key K[SORT_CAP][64];
+                                                     uint32_t* pack, int shift) {
+    __shared__ uint64_t K[SORT_CAP][64];
+    __shared__ int32_t hkey[ALIGNED ? ALIGN_HASH : 1];
+    __shared__ int32_t hcnt[ALIGNED ? ALIGN_HASH : 1];
     const int lane = threadIdx.x;
     const int64_t f = int64_t(blockIdx.x) * 64 + lane;
-    if (f >= D) return;
-    const int64_t base = sptr[f >> 6] + lane;
-    const int n = cnt[f];
-    auto key_at = [&](int k) -> Key {
+    const bool live = f < D;
+    const int64_t base = sptr[blockIdx.x] + lane;
+    const int n = live ? int(cnt[f]) : 0;
+    auto word_at = [&](int k) -> uint64_t {  // packed word, or column << 16 | code
         const int64_t at = base + int64_t(k) * 64;
         if constexpr (PACKED) return pack[at];
         else return (uint64_t(uint32_t(col[at])) << 16) | code[at];
     };
-    auto put = [&](int k, Key v) {
+    auto column_of = [&](uint64_t w) -> int64_t {
+        if constexpr (PACKED) return int64_t(uint32_t(w) >> shift);
+        else return int64_t(w >> 16);
+    };
+    auto put = [&](int k, uint64_t key) {
         const int64_t at = base + int64_t(k) * 64;
         if constexpr (PACKED) {
-            pack[at] = v;
+            pack[at] = uint32_t(key);
         } else {
-            col[at] = int32_t(uint32_t(v >> 16));
-            code[at] = uint16_t(v & 0xffffu);
+            col[at] = int32_t(uint32_t((key >> 16) & 0xffffffffu));
+            code[at] = uint16_t(key & 0xffffu);
         }
     };
+    if constexpr (ALIGNED) {
+        for (int i = lane; i < ALIGN_HASH; i += 64) {
+            hkey[i] = NO_OFFSET;
+            hcnt[i] = 0;
+        }
+        __syncthreads();
+        for (int k = 0; k < n; ++k) {
+            const int32_t d = int32_t(column_of(word_at(k)) - f);
+            uint32_t h = offset_slot(d);
+            for (int probe = 0; probe < ALIGN_HASH; ++probe, h = (h + 1) & (ALIGN_HASH - 1)) {
+                const int32_t old = atomicCAS(&hkey[h], NO_OFFSET, d);
+                if (old == NO_OFFSET || old == d) {
+                    atomicAdd(&hcnt[h], 1);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+    auto key_of = [&](uint64_t w) -> uint64_t {
+        uint64_t c = 0;
+        if constexpr (ALIGNED) {
+            const int32_t d = int32_t(column_of(w) - f);
+            uint32_t h = offset_slot(d);
+            for (int probe = 0; probe < ALIGN_HASH; ++probe, h = (h + 1) & (ALIGN_HASH - 1)) {
+                const int32_t kk = hkey[h];
+                if (kk == d) {
+                    c = uint64_t(min(hcnt[h], 255));
+                    break;
+                }
+                if (kk == NO_OFFSET) break;
+            }
+        }
+        return ((255u - c) << (PACKED ? 32 : 48)) | w;
+    };
+    auto low = [&](uint64_t key) -> uint64_t { return PACKED ? (key & 0xffffffffull) : (key & ((1ull << 48) - 1)); };
     if (n <= 32) {
         // up to 32 keys: a bitonic network in registers (slots past n padded with the largest key)
-        Key r[32];
+        uint64_t r[32];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) r[k] = k < n ? key_at(k) : Key(~Key(0));
+        for (int k = 0; k < 32; ++k) r[k] = k < n ? key_of(word_at(k)) : ~uint64_t(0);
 #pragma unroll
         for (int size = 2; size <= 32; size <<= 1) {
 #pragma unroll
@@ -186,7 +240,7 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
                 for (int i = 0; i < 32; ++i) {
                     const int j = i ^ stride;
                     if (j > i) {
-                        const Key a = r[i], b = r[j];
+                        const uint64_t a = r[i], b = r[j];
                         const bool up = (i & size) == 0;
                         r[i] = up ? (a < b ? a : b) : (a < b ? b : a);
                         r[j] = up ? (a < b ? b : a) : (a < b ? a : b);
@@ -196,11 +250,11 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
         }
 #pragma unroll
         for (int k = 0; k < 32; ++k)
-            if (k < n) put(k, r[k]);
+            if (k < n) put(k, low(r[k]));
     } else if (n <= SORT_CAP) {
-        for (int k = 0; k < n; ++k) K[k][lane] = key_at(k);
+        for (int k = 0; k < n; ++k) K[k][lane] = key_of(word_at(k));
         for (int k = 1; k < n; ++k) {
-            const Key v = K[k][lane];
+            const uint64_t v = K[k][lane];
             int q = k - 1;
             while (q >= 0 && K[q][lane] > v) {
                 K[q + 1][lane] = K[q][lane];
@@ -208,16 +262,16 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
             }
             K[q + 1][lane] = v;
         }
-        for (int k = 0; k < n; ++k) put(k, K[k][lane]);
+        for (int k = 0; k < n; ++k) put(k, low(K[k][lane]));
     } else {
         for (int k = 1; k < n; ++k) {
-            const Key v = key_at(k);
+            const uint64_t w = word_at(k), v = key_of(w);
             int q = k - 1;
-            while (q >= 0 && key_at(q) > v) {
-                put(q + 1, key_at(q));
+            while (q >= 0 && key_of(word_at(q)) > v) {
+                put(q + 1, word_at(q));
                 --q;
             }
-            put(q + 1, v);
+            put(q + 1, w);
         }
     }
 }
@@ -339,14 +393,15 @@ StoredView hmat_view(const ed_hmat* m) {
 
 // One 256-row tile per workgroup: the fastest setting measured for the 2x12
 // sector (ed_hmat_tune, product plus the reduction of its partials).
-// Rows sorted by column after the fill (the default; ED_HMAT_SORTED_ROWS=0,
-// read at ed_hmat_create, keeps the walk order).  On the 2x12, 2-hole sector
-// the product takes 0.471 ms instead of 0.529 ms: with each row's elements in
-// column order, the 64 lanes reading slot k gather from nearby columns more
-// often.
-bool hmat_sorted_rows() {
-    const char* e = std::getenv("ED_HMAT_SORTED_ROWS");
-    return !(e && std::atoi(e) == 0);
+// Order of each row's elements after the fill (ED_HMAT_ROW_ORDER, read at
+// ed_hmat_create): 0 keeps the walk order, 1 (default) sorts by column, 2
+// puts offsets shared across the slice first (hm_sort_kernel).  On the 2x12,
+// 2-hole sector the column sort took the product from 0.529 ms to 0.471 ms:
+// the 64 lanes reading slot k gather from nearby columns more often.
+int hmat_row_order() {
+    const char* e = std::getenv("ED_HMAT_ROW_ORDER");
+    const int v = e ? std::atoi(e) : 1;
+    return v < 0 || v > 2 ? 1 : v;
 }
 
 int hmat_default_tiles(int64_t) { return 1; }
@@ -567,10 +622,13 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
             hipLaunchKernelGGL(hm_fill_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
                                d_map, m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift);
             ED_HIP(hipGetLastError());
-            if (hmat_sorted_rows())
-                hipLaunchKernelGGL(m->shift ? hm_sort_kernel<true> : hm_sort_kernel<false>,
-                                   dim3(unsigned(m->nslices)), dim3(64), 0, st, D, m->d_sptr, m->d_cnt, m->d_col,
-                                   m->d_code, m->d_pack);
+            const int order = hmat_row_order();
+            if (order) {
+                auto kernel = m->shift ? (order == 2 ? hm_sort_kernel<true, true> : hm_sort_kernel<true, false>)
+                                       : (order == 2 ? hm_sort_kernel<false, true> : hm_sort_kernel<false, false>);
+                hipLaunchKernelGGL(kernel, dim3(unsigned(m->nslices)), dim3(64), 0, st, D, m->d_sptr, m->d_cnt,
+                                   m->d_col, m->d_code, m->d_pack, m->shift);
+            }
             ED_HIP(hipGetLastError());
             ED_HIP(hipStreamSynchronize(st));
         } else {

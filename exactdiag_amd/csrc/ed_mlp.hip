@@ -12,6 +12,7 @@
 // accumulator j = 0..15.
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -67,6 +68,111 @@ __global__ void __launch_bounds__(64) gemm_bf16_kernel(int M, int N, int K, cons
             *c = accumulate ? *c + acc[j] : acc[j];
         }
     }
+}
+
+// The same product on 128 x 128 tiles of C per 256-thread workgroup, staged through LDS.  Each wavefront
+// owns a 64 x 64 quarter (2 x 2 MFMA tiles).  Per 32-wide step of K the workgroup loads the A and B tiles
+// with 16-byte global loads of 8 consecutive elements along whichever dimension is contiguous in memory
+// (A_K: A's k stride is 1, else its m stride is 1; B_K: B's k stride is 1, else its n stride is 1) and stores
+// them in LDS as As[m][k] and Bs[n][k] (k contiguous, row pitch GLDK), so that every MFMA operand is one
+// 16-byte LDS read.  The next step's global loads are issued before the current step's MFMAs.  Chunks that
+// cross the matrix edge are read element by element with zero fill; the host launches this kernel only when
+// the operands are 16-byte aligned and their other stride is a multiple of 8.
+constexpr int GBM = 128, GBN = 128, GBK = 32, GLDK = GBK + 8;
+
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+// 8 consecutive elements p[0], p[step], ... with p + i * step in range for i < n_in (n_in >= 8: one vector load)
+__device__ inline u16x8 load8(const uint16_t* __restrict__ p, int n_in, bool ok) {
+    u16x8 v;
+    if (ok && n_in >= 8) return *reinterpret_cast<const u16x8*>(p);
+    for (int i = 0; i < 8; ++i) v[i] = (ok && i < n_in) ? p[i] : uint16_t(0);
+    return v;
+}
+
+template <bool A_K, bool B_K>
+__global__ void __launch_bounds__(256) gemm_bf16_tiled_kernel(int M, int N, int K, const uint16_t* __restrict__ A,
+                                                              int64_t sam, int64_t sak,
+                                                              const uint16_t* __restrict__ B, int64_t sbk,
+                                                              int64_t sbn, float* __restrict__ C, int64_t ldc,
+                                                              int accumulate) {
+    __shared__ __align__(16) uint16_t As[GBM * GLDK];
+    __shared__ __align__(16) uint16_t Bs[GBN * GLDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int r = lane & 31, half = lane >> 5;
+    f32x16 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+    u16x8 ra[2], rb[2];
+    // chunk c of 512 per operand and step: k-contiguous tiles are 4 chunks per row of 32 k (row = c / 4,
+    // k = 8 (c % 4)); m- or n-contiguous tiles are 16 chunks per k (k = c / 16, row = 8 (c % 16))
+    auto fetch = [&](int k0) {
+        for (int i = 0; i < 2; ++i) {
+            const int c = tid + 256 * i;
+            if constexpr (A_K) {
+                const int m = m0 + c / 4, k = k0 + 8 * (c % 4);
+                ra[i] = load8(A + int64_t(m) * sam + k, K - k, m < M && k < K);
+            } else {
+                const int k = k0 + c / 16, m = m0 + 8 * (c % 16);
+                ra[i] = load8(A + int64_t(k) * sak + m, M - m, k < K && m < M);
+            }
+            if constexpr (B_K) {
+                const int n = n0 + c / 4, k = k0 + 8 * (c % 4);
+                rb[i] = load8(B + int64_t(n) * sbn + k, K - k, n < N && k < K);
+            } else {
+                const int k = k0 + c / 16, n = n0 + 8 * (c % 16);
+                rb[i] = load8(B + int64_t(k) * sbk + n, N - n, k < K && n < N);
+            }
+        }
+    };
+    auto stash = [&]() {
+        for (int i = 0; i < 2; ++i) {
+            const int c = tid + 256 * i;
+            if constexpr (A_K) {
+                *reinterpret_cast<u16x8*>(&As[(c / 4) * GLDK + 8 * (c % 4)]) = ra[i];
+            } else {
+                const int k = c / 16, m = 8 * (c % 16);
+                for (int e = 0; e < 8; ++e) As[(m + e) * GLDK + k] = ra[i][e];
+            }
+            if constexpr (B_K) {
+                *reinterpret_cast<u16x8*>(&Bs[(c / 4) * GLDK + 8 * (c % 4)]) = rb[i];
+            } else {
+                const int k = c / 16, n = 8 * (c % 16);
+                for (int e = 0; e < 8; ++e) Bs[(n + e) * GLDK + k] = rb[i][e];
+            }
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+        stash();
+        __syncthreads();
+        if (k0 + GBK < K) fetch(k0 + GBK);
+        for (int kk = 0; kk < GBK; kk += 16) {
+            bf16x8 a[2], b[2];
+            for (int t = 0; t < 2; ++t) {
+                a[t] = __builtin_bit_cast(bf16x8,
+                                          *reinterpret_cast<const u16x8*>(&As[(wm + 32 * t + r) * GLDK + kk + 8 * half]));
+                b[t] = __builtin_bit_cast(bf16x8,
+                                          *reinterpret_cast<const u16x8*>(&Bs[(wn + 32 * t + r) * GLDK + kk + 8 * half]));
+            }
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int e = 0; e < 16; ++e) {
+                const int m = m0 + wm + 32 * i + 8 * (e >> 2) + 4 * half + (e & 3);
+                const int n = n0 + wn + 32 * j + r;
+                if (m < M && n < N) {
+                    float* c = &C[int64_t(m) * ldc + n];
+                    *c = accumulate ? *c + acc[i][j][e] : acc[i][j][e];
+                }
+            }
 }
 
 __device__ inline float gelu(float x) {
@@ -168,8 +274,39 @@ __global__ void reduce_double_kernel(const double* p, int n, double* out) {
 
 inline unsigned nblk(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
 
+// Which GEMM kernel a product uses: 1 = LDS-tiled (128 x 128 tiles, vector loads), 0 = one wavefront per
+// 32 x 32 tile.  ED_GEMM_KERNEL=0 or 1 forces one (for measurements); otherwise the tiled kernel whenever the
+// operands allow its 16-byte loads.
+int gemm_kernel_choice() {
+    static const int v = [] {
+        const char* e = std::getenv("ED_GEMM_KERNEL");
+        return e ? std::atoi(e) : -1;
+    }();
+    return v;
+}
+
 void gemm(hipStream_t st, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
           int64_t sbk, int64_t sbn, float* C, int64_t ldc, int accumulate) {
+    // 16-byte loads of 8 elements: one unit stride per operand, the other a multiple of 8, aligned bases
+    const bool a_k = sak == 1 && sam % 8 == 0, a_m = sam == 1 && sak % 8 == 0;
+    const bool b_k = sbk == 1 && sbn % 8 == 0, b_n = sbn == 1 && sbk % 8 == 0;
+    const bool aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+    const int choice = gemm_kernel_choice();
+    if (choice != 0 && aligned && (a_k || a_m) && (b_k || b_n)) {
+        const dim3 grid{unsigned((N + GBN - 1) / GBN), unsigned((M + GBM - 1) / GBM), 1u};
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate);
+        };
+        if (a_k && b_k)
+            go(gemm_bf16_tiled_kernel<true, true>);
+        else if (a_k)
+            go(gemm_bf16_tiled_kernel<true, false>);
+        else if (b_k)
+            go(gemm_bf16_tiled_kernel<false, true>);
+        else
+            go(gemm_bf16_tiled_kernel<false, false>);
+        return;
+    }
     const dim3 grid{unsigned((N + 31) / 32), unsigned((M + 31) / 32), 1u};
     hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(64), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc,
                        accumulate);

@@ -230,40 +230,14 @@ def test_c4_complex64_against_complex128(torch):
 
 
 def test_c5_learned_closure_on_2x12_all_q(torch):
-    from exactdiag_amd import _lib, closure
-    from exactdiag_amd.allq import _gs, distinct_qs, final_momentum, spectral_l1
-    from exactdiag_amd.general import krylov
-    from exactdiag_amd.sector import Sector, cf_eval
+    from exactdiag_amd import closure
 
     if not closure.MODEL_PATH.exists():
         pytest.skip("closure model not trained")
-    model = closure.Closure.load()
-    L, h, nd = 12, 2, 11
-    s0 = Sector(L, h, nd, 0, 0)
-    hm0 = s0.store(W)
-    e0, psi = _gs(hm0.hv, s0.dim, 1e-12)
-    hm0.close()
-    errs = []
-    for qx, qy in distinct_qs(L):
-        s1 = Sector(L, h, nd, *final_momentum(L, 0, 0, qx, qy))
-        phi = s0.op_apply(s1, _lib.OP_SZQ, qx, qy, psi)
-        n2 = krylov.vec_norm(phi) ** 2
-        if n2 < 1e-14:
-            s1.close()
-            continue
-        hm = s1.store(W)
-        a, b = hm.lanczos(phi, 1000)
-        hm.close()
-        s1.close()
-        f = -n2 / np.pi
-        ref = cf_eval(a, b, WS, e0, ETA, f)
-        at, bt, tail = closure.sqrt_terminator(a, b)
-        ae, be, tail2 = model.extend(a, b)  # reads the first 64 levels only
-        e_term = spectral_l1(cf_eval(at, bt, WS, e0, ETA, f, tail=tail), ref, WS)
-        e_nn = spectral_l1(cf_eval(ae, be, WS, e0, ETA, f, tail=tail2), ref, WS)
-        errs.append((e_term, e_nn))
-        print(f"C5: q=({qx},{qy}): terminator {e_term:.2e}, learned {e_nn:.2e}", flush=True)
-    errs = np.array(errs)
+    per_q, _ = closure.evaluate_allq(closure.Closure.load(), 12, 2, 11, W, WS, ETA)
+    for q, (e_term, e_nn) in per_q.items():
+        print(f"C5: q={q}: terminator {e_term:.2e}, learned {e_nn:.2e}", flush=True)
+    errs = np.array(list(per_q.values()))
     assert len(errs) == 13
     gain = np.exp(np.mean(np.log(errs[:, 0] / errs[:, 1])))
     print(f"C5: mean L1 terminator {errs[:, 0].mean():.2e}, learned {errs[:, 1].mean():.2e}, geometric gain "

@@ -47,6 +47,35 @@ def test_gemm_lane_layout_and_values(torch, shape):
     assert np.max(np.abs(Ct.cpu().numpy() - ref.T)) < 1e-4 * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("shape", [(200, 136, 88), (264, 512, 1024), (128, 128, 32), (8, 520, 40)])
+@pytest.mark.parametrize("a_kcontig,b_kcontig", [(True, False), (True, True), (False, False), (False, True)])
+def test_tiled_gemm_all_operand_layouts(torch, shape, a_kcontig, b_kcontig):
+    """The LDS-tiled kernel (taken when a stride is 1 and the other a multiple of 8) for every layout of A and
+    B, with tiles that cross the matrix edges, against float64 products of the same BF16 values."""
+    from exactdiag_amd._lib import check, lib
+    from oracle import mlp_oracle as mo
+
+    M, N, K = shape
+    rng = np.random.default_rng(M * N + K)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    ref = mo.bf16(A).astype(np.float64) @ mo.bf16(B).astype(np.float64)
+    # A stored as [m][k] (k contiguous) or as its transpose [k][m]; B as [n][k] (k contiguous) or [k][n]
+    Ast = _bits(A) if a_kcontig else _bits(np.ascontiguousarray(A.T))
+    Bst = _bits(np.ascontiguousarray(B.T)) if b_kcontig else _bits(B)
+    sam, sak = (K, 1) if a_kcontig else (1, M)
+    sbk, sbn = (1, K) if b_kcontig else (N, 1)
+    dA = torch.as_tensor(Ast.astype(np.uint16).view(np.int16), device="cuda")
+    dB = torch.as_tensor(Bst.astype(np.uint16).view(np.int16), device="cuda")
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    s = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(lib().ed_gemm_bf16(M, N, K, ct.c_void_p(dA.data_ptr()), sam, sak, ct.c_void_p(dB.data_ptr()), sbk, sbn,
+                             ct.c_void_p(C.data_ptr()), N, s))
+    got = C.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - ref)) < 1e-4 * max(1.0, np.max(np.abs(ref)))
+
+
 def test_forward_and_gradients_match_oracle(torch):
     from exactdiag_amd.closure import GpuMLP
     from oracle import mlp_oracle as mo
