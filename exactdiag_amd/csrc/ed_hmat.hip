@@ -141,13 +141,13 @@ __global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __res
 
 // Order each row's stored elements so that the 64 lanes of a wavefront, which
 // read slot k of 64 consecutive rows f, gather x[f + d] with one offset d where
-// possible: their addresses are then consecutive (16 lines for complex128
-// instead of up to 64).  One wavefront per 64-row slice, one lane per row.
-// ALIGNED: every element's offset d = column - f is counted over the slice (an
-// LDS hash table), and a row's elements are sorted by that count, largest
-// first, then by column; offsets shared by all rows of a slice (exchange on
-// bonds above the spins that vary within the slice, whose rank change does not
-// depend on those spins) come first and in the same order in every row.
+// possible: their addresses are then consecutive.  One wavefront per 64-row
+// slice, one lane per row.  ALIGNED: every element's pair (offset d = column -
+// f, code) is counted over the slice (an LDS hash table), and a row's elements
+// are sorted by that count, largest first, then by column; pairs shared by all
+// rows of a slice (exchange on bonds above the spins that vary within the
+// slice, whose rank change does not depend on those spins) come first and in
+// the same order in every row, where hm_uniform_kernel finds them.
 // Otherwise the rows are sorted by column alone.  A row of up to 32 elements is
 // sorted by a bitonic network in registers; one of up to SORT_CAP is copied
 // into the lane's own LDS column (K[k][lane], no bank conflicts) and
@@ -155,17 +155,19 @@ __global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __res
 // The key is (255 - count) << 32 | packed word, or (255 - count) << 48 |
 // column << 16 | code for the two-array form (count 0 when not ALIGNED).
 constexpr int SORT_CAP = 64;
-constexpr int ALIGN_HASH = 2048;  // offsets tracked per slice (power of two); more are counted as unshared
-constexpr int32_t NO_OFFSET = INT32_MIN;
+constexpr int ALIGN_HASH = 2048;  // pairs tracked per slice (power of two); more are counted as unshared
+constexpr unsigned long long NO_PAIR = ~0ull;  // codes use at most 31 bits, so no pair has this key
 
-__device__ inline uint32_t offset_slot(int32_t d) { return (uint32_t(d) * 2654435761u) >> 21; }  // 11 bits
+__device__ inline uint32_t pair_slot(unsigned long long p) {  // 11 bits
+    return ((uint32_t(p >> 32) * 2654435761u) ^ (uint32_t(p) * 0x9e3779b1u)) >> 21;
+}
 
 template <bool PACKED, bool ALIGNED>
 __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* __restrict__ sptr,
                                                      const uint8_t* __restrict__ cnt, int32_t* col, uint16_t* code,
                                                      uint32_t* pack, int shift) {
     __shared__ uint64_t K[SORT_CAP][64];
-    __shared__ int32_t hkey[ALIGNED ? ALIGN_HASH : 1];
+    __shared__ unsigned long long hkey[ALIGNED ? ALIGN_HASH : 1];
     __shared__ int32_t hcnt[ALIGNED ? ALIGN_HASH : 1];
     const int lane = threadIdx.x;
     const int64_t f = int64_t(blockIdx.x) * 64 + lane;
@@ -181,6 +183,11 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
         if constexpr (PACKED) return int64_t(uint32_t(w) >> shift);
         else return int64_t(w >> 16);
     };
+    auto pair_of = [&](uint64_t w) -> unsigned long long {
+        const uint32_t d = uint32_t(int32_t(column_of(w) - f));
+        const uint32_t c = PACKED ? uint32_t(w) & ((1u << shift) - 1u) : uint32_t(w & 0xffffu);
+        return (uint64_t(d) << 32) | c;
+    };
     auto put = [&](int k, uint64_t key) {
         const int64_t at = base + int64_t(k) * 64;
         if constexpr (PACKED) {
@@ -192,16 +199,16 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
     };
     if constexpr (ALIGNED) {
         for (int i = lane; i < ALIGN_HASH; i += 64) {
-            hkey[i] = NO_OFFSET;
+            hkey[i] = NO_PAIR;
             hcnt[i] = 0;
         }
         __syncthreads();
         for (int k = 0; k < n; ++k) {
-            const int32_t d = int32_t(column_of(word_at(k)) - f);
-            uint32_t h = offset_slot(d);
+            const unsigned long long d = pair_of(word_at(k));
+            uint32_t h = pair_slot(d);
             for (int probe = 0; probe < ALIGN_HASH; ++probe, h = (h + 1) & (ALIGN_HASH - 1)) {
-                const int32_t old = atomicCAS(&hkey[h], NO_OFFSET, d);
-                if (old == NO_OFFSET || old == d) {
+                const unsigned long long old = atomicCAS(&hkey[h], NO_PAIR, d);
+                if (old == NO_PAIR || old == d) {
                     atomicAdd(&hcnt[h], 1);
                     break;
                 }
@@ -213,15 +220,15 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
     auto key_of = [&](uint64_t w) -> uint64_t {
         uint64_t c = 0;
         if constexpr (ALIGNED) {
-            const int32_t d = int32_t(column_of(w) - f);
-            uint32_t h = offset_slot(d);
+            const unsigned long long d = pair_of(w);
+            uint32_t h = pair_slot(d);
             for (int probe = 0; probe < ALIGN_HASH; ++probe, h = (h + 1) & (ALIGN_HASH - 1)) {
-                const int32_t kk = hkey[h];
+                const unsigned long long kk = hkey[h];
                 if (kk == d) {
                     c = uint64_t(min(hcnt[h], 255));
                     break;
                 }
-                if (kk == NO_OFFSET) break;
+                if (kk == NO_PAIR) break;
             }
         }
         return ((255u - c) << (PACKED ? 32 : 48)) | w;
@@ -274,6 +281,96 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
             put(q + 1, w);
         }
     }
+}
+
+// After the ALIGNED sort: u[s] = the number of leading slots k at which every
+// row of the full slice s holds the same (column offset, code); width2[s] =
+// the widest row's remaining count | (sum of the remaining counts) << 16.
+// One wavefront per slice.  A slice with fewer than 64 rows keeps all of its
+// elements per row.
+template <bool PACKED>
+__global__ void __launch_bounds__(64) hm_uniform_kernel(int64_t D, const int64_t* __restrict__ sptr,
+                                                        const uint8_t* __restrict__ cnt, const int32_t* __restrict__ col,
+                                                        const uint16_t* __restrict__ code,
+                                                        const uint32_t* __restrict__ pack, int shift, int32_t* ucount,
+                                                        int32_t* width2) {
+    const int lane = threadIdx.x;
+    const int64_t s = blockIdx.x;
+    const int64_t f = s * 64 + lane;
+    const bool full = s * 64 + 64 <= D;
+    const int n = f < D ? int(cnt[f]) : 0;
+    int nmin = n, nmax = n, nsum = n;
+    for (int o = 32; o > 0; o >>= 1) {
+        nmin = min(nmin, __shfl_xor(nmin, o, 64));
+        nmax = max(nmax, __shfl_xor(nmax, o, 64));
+        nsum += __shfl_xor(nsum, o, 64);
+    }
+    int u = 0;
+    if (full) {
+        const int64_t base = sptr[s] + lane;
+        for (; u < nmin; ++u) {
+            const int64_t at = base + int64_t(u) * 64;
+            int64_t c;
+            uint32_t cd;
+            if constexpr (PACKED) {
+                const uint32_t w = pack[at];
+                c = int64_t(w >> shift);
+                cd = w & ((1u << shift) - 1u);
+            } else {
+                c = col[at];
+                cd = code[at];
+            }
+            const int32_t d = int32_t(c - f);
+            const bool same = d == __shfl(d, 0, 64) && cd == uint32_t(__shfl(int(cd), 0, 64));
+            if (!__all(same)) break;
+        }
+    }
+    if (lane == 0) {
+        ucount[s] = u;
+        width2[s] = (nmax - u) | ((nsum - 64 * u) << 16);
+    }
+}
+
+// Move the uniform slots of slice s into uni[uptr[s] ..] (from row s * 64) and
+// the remaining elements of each row into the new sliced-ELL arrays; cnt[f]
+// becomes the remaining count.
+template <bool PACKED>
+__global__ void __launch_bounds__(64) hm_compact_kernel(int64_t D, const int64_t* __restrict__ sptr,
+                                                        const int64_t* __restrict__ sptr2,
+                                                        const int64_t* __restrict__ uptr, uint8_t* cnt,
+                                                        const int32_t* __restrict__ col,
+                                                        const uint16_t* __restrict__ code,
+                                                        const uint32_t* __restrict__ pack, int shift, int32_t* col2,
+                                                        uint16_t* code2, uint32_t* pack2, uint2* uni) {
+    const int lane = threadIdx.x;
+    const int64_t s = blockIdx.x;
+    const int64_t f = s * 64 + lane;
+    const int u = int(uptr[s + 1] - uptr[s]);
+    for (int k = lane; k < u; k += 64) {
+        const int64_t at = sptr[s] + int64_t(k) * 64;  // row s * 64 (lane 0 of the slice)
+        uint32_t d, cd;
+        if constexpr (PACKED) {
+            const uint32_t w = pack[at];
+            d = uint32_t(int32_t(int64_t(w >> shift) - s * 64));
+            cd = w & ((1u << shift) - 1u);
+        } else {
+            d = uint32_t(int32_t(int64_t(col[at]) - s * 64));
+            cd = code[at];
+        }
+        uni[uptr[s] + k] = uint2{d, cd};
+    }
+    if (f >= D) return;
+    const int n = cnt[f];
+    const int64_t from = sptr[s] + lane, to = sptr2[s] + lane;
+    for (int k = u; k < n; ++k) {
+        if constexpr (PACKED) {
+            pack2[to + int64_t(k - u) * 64] = pack[from + int64_t(k) * 64];
+        } else {
+            col2[to + int64_t(k - u) * 64] = col[from + int64_t(k) * 64];
+            code2[to + int64_t(k - u) * 64] = code[from + int64_t(k) * 64];
+        }
+    }
+    cnt[f] = uint8_t(n - u);
 }
 
 // The product over all rows (stored_row, ed_hmatrow.h).  Persistent
@@ -369,13 +466,19 @@ struct ed_hmat {
     uint16_t* d_diag = nullptr;
     double* d_vals = nullptr;
     double* d_partial = nullptr;
+    int64_t* d_uptr = nullptr;  // slice-uniform entries (nullptr: none)
+    uint2* d_uni = nullptr;
+    int64_t n_uni = 0;
 
     int element_bytes() const { return shift ? 4 : 6; }
-    int64_t bytes() const { return slots * element_bytes() + D * 3 + (nslices + 1) * 8 + int64_t(n_vals) * 16; }
+    int64_t bytes() const {
+        return slots * element_bytes() + D * 3 + (nslices + 1) * 8 + int64_t(n_vals) * 16 +
+               (d_uptr ? (nslices + 1) * 8 + n_uni * 8 : 0);
+    }
 
     ~ed_hmat() {
         for (void* p : {(void*)d_sptr, (void*)d_col, (void*)d_code, (void*)d_pack, (void*)d_cnt, (void*)d_diag,
-                        (void*)d_vals, (void*)d_partial})
+                        (void*)d_vals, (void*)d_partial, (void*)d_uptr, (void*)d_uni})
             dev_free(p);
     }
 };
@@ -388,7 +491,7 @@ HvWeights hmat_weights(const ed_hmat* m) { return m->w; }
 
 StoredView hmat_view(const ed_hmat* m) {
     return StoredView{m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift, m->n_vals, m->d_cnt, m->d_diag,
-                      m->s->d_orbit, m->s->d_sqrt, m->d_vals};
+                      m->s->d_orbit, m->s->d_sqrt, m->d_vals, m->d_uptr, m->d_uni};
 }
 
 // One 256-row tile per workgroup: the fastest setting measured for the 2x12
@@ -405,6 +508,84 @@ int hmat_row_order() {
 }
 
 int hmat_default_tiles(int64_t) { return 1; }
+
+// Slice-uniform entries kept once per slice (the default; ED_HMAT_UNIFORM=0,
+// read at ed_hmat_create, keeps every element per row).
+bool hmat_uniform() {
+    const char* e = std::getenv("ED_HMAT_UNIFORM");
+    return !(e && std::atoi(e) == 0);
+}
+
+// Moves the leading slots that every row of a slice shares into the per-slice
+// table (hm_uniform_kernel, hm_compact_kernel) and rebuilds the sliced-ELL
+// arrays for the remaining elements.
+void extract_uniform(ed_hmat* m, hipStream_t st) {
+    const int64_t D = m->D, ns = m->nslices;
+    std::vector<void*> tmp;      // the new arrays, owned by the matrix once complete
+    std::vector<void*> scratch;  // freed in any case
+    auto cleanup = [&] {
+        for (void* p : tmp) dev_free(p);
+        for (void* p : scratch) dev_free(p);
+        tmp.clear();
+        scratch.clear();
+    };
+    try {
+        int32_t* d_u = dev_alloc<int32_t>(size_t(ns));
+        scratch.push_back(d_u);
+        int32_t* d_w = dev_alloc<int32_t>(size_t(ns));
+        scratch.push_back(d_w);
+        hipLaunchKernelGGL(m->shift ? hm_uniform_kernel<true> : hm_uniform_kernel<false>, dim3(unsigned(ns)), dim3(64),
+                           0, st, D, m->d_sptr, m->d_cnt, m->d_col, m->d_code, m->d_pack, m->shift, d_u, d_w);
+        ED_HIP(hipGetLastError());
+        std::vector<int32_t> u(static_cast<size_t>(ns)), w(static_cast<size_t>(ns));
+        ED_HIP(hipMemcpyAsync(u.data(), d_u, u.size() * 4, hipMemcpyDeviceToHost, st));
+        ED_HIP(hipMemcpyAsync(w.data(), d_w, w.size() * 4, hipMemcpyDeviceToHost, st));
+        ED_HIP(hipStreamSynchronize(st));
+        std::vector<int64_t> uptr(static_cast<size_t>(ns + 1), 0), sptr2(static_cast<size_t>(ns + 1), 0);
+        for (int64_t i = 0; i < ns; ++i) {
+            uptr[size_t(i + 1)] = uptr[size_t(i)] + u[size_t(i)];
+            sptr2[size_t(i + 1)] = sptr2[size_t(i)] + int64_t(w[size_t(i)] & 0xffff) * 64;
+        }
+        m->n_uni = uptr.back();
+        const int64_t slots2 = sptr2.back();
+        int64_t* d_uptr = dev_upload(uptr, st);
+        tmp.push_back(d_uptr);
+        int64_t* d_sptr2 = dev_upload(sptr2, st);
+        tmp.push_back(d_sptr2);
+        uint2* d_uni = dev_alloc<uint2>(size_t(std::max<int64_t>(m->n_uni, 1)));
+        tmp.push_back(d_uni);
+        int32_t* col2 = nullptr;
+        uint16_t* code2 = nullptr;
+        uint32_t* pack2 = nullptr;
+        if (m->shift) {
+            pack2 = dev_alloc<uint32_t>(size_t(std::max<int64_t>(slots2, 1)));
+            tmp.push_back(pack2);
+        } else {
+            col2 = dev_alloc<int32_t>(size_t(std::max<int64_t>(slots2, 1)));
+            tmp.push_back(col2);
+            code2 = dev_alloc<uint16_t>(size_t(std::max<int64_t>(slots2, 1)));
+            tmp.push_back(code2);
+        }
+        hipLaunchKernelGGL(m->shift ? hm_compact_kernel<true> : hm_compact_kernel<false>, dim3(unsigned(ns)), dim3(64),
+                           0, st, D, m->d_sptr, d_sptr2, d_uptr, m->d_cnt, m->d_col, m->d_code, m->d_pack, m->shift,
+                           col2, code2, pack2, d_uni);
+        ED_HIP(hipGetLastError());
+        ED_HIP(hipStreamSynchronize(st));
+        for (void* p : {(void*)m->d_sptr, (void*)m->d_col, (void*)m->d_code, (void*)m->d_pack}) dev_free(p);
+        m->d_sptr = d_sptr2;
+        m->d_col = col2;
+        m->d_code = code2;
+        m->d_pack = pack2;
+        m->d_uptr = d_uptr;
+        m->d_uni = d_uni;
+        m->slots = slots2;
+        tmp.clear();  // now owned by the matrix
+        cleanup();
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+}
 
 template <class VT, int U>
 void launch_spmv(const ed_hmat* m, const void* x, void* y, double* partial, int64_t nwg, int64_t tiles,
@@ -622,13 +803,16 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
             hipLaunchKernelGGL(hm_fill_kernel, dim3(nwg), dim3(BS), lds ? lay.bytes : 0, st, v, m->w, lay, lds,
                                d_map, m->d_sptr, m->d_col, m->d_code, m->d_pack, m->shift);
             ED_HIP(hipGetLastError());
-            const int order = hmat_row_order();
+            const bool uniform = !coded && hmat_uniform();
+            const int order = uniform ? 2 : hmat_row_order();
             if (order) {
                 auto kernel = m->shift ? (order == 2 ? hm_sort_kernel<true, true> : hm_sort_kernel<true, false>)
                                        : (order == 2 ? hm_sort_kernel<false, true> : hm_sort_kernel<false, false>);
                 hipLaunchKernelGGL(kernel, dim3(unsigned(m->nslices)), dim3(64), 0, st, D, m->d_sptr, m->d_cnt,
                                    m->d_col, m->d_code, m->d_pack, m->shift);
             }
+            ED_HIP(hipGetLastError());
+            if (uniform) extract_uniform(m, st);
             ED_HIP(hipGetLastError());
             ED_HIP(hipStreamSynchronize(st));
         } else {

@@ -4,9 +4,14 @@
 // Row f of the sliced-ELL gather form: y_f = sqrt(n_f) * sum_k (+-V[code_k])
 // x[col_k] + V[diag_f] x_f, element k at sptr[f / 64] + 64 k + f % 64, as
 // packed 32-bit words (column << shift | value index << 1 | negate) or as an
-// int32 column with a uint16 code (bit 15 = negate).  U elements are loaded,
-// then their U gathers issued together.  V is the value table (staged in LDS
-// by the caller).
+// int32 column with a uint16 code (bit 15 = negate).  Elements that every row
+// of a 64-row slice has with the same column offset d = column - f and the
+// same code are kept once per slice, in uni[uptr[s] .. uptr[s + 1]) as
+// (d, code): the 64 lanes of the wavefront that owns the slice read each such
+// entry at one address and gather x[f + d] from 64 consecutive states.  U
+// elements are loaded, then their U gathers issued together.  V is the value
+// table (staged in LDS by the caller).  The caller's wavefronts must each
+// cover the rows of one slice (row ranges start at multiples of 64).
 #pragma once
 
 #include "ed_internal.h"
@@ -25,7 +30,29 @@ struct StoredView {
     const uint8_t* orbit;
     const double* sq;
     const double* vals;
+    const int64_t* uptr;  // per slice: first slice-uniform entry (nullptr: none stored)
+    const uint2* uni;     // slice-uniform entries (column offset, code in the element's format)
 };
+
+template <bool PACKED>
+__device__ inline void decode_code(uint32_t c, int& vi, double& sg) {
+    if constexpr (PACKED) {
+        vi = int(c >> 1);
+        sg = (c & 1u) ? -1.0 : 1.0;
+    } else {
+        vi = int(c & 0x7fffu);
+        sg = (c & 0x8000u) ? -1.0 : 1.0;
+    }
+}
+
+// Range [u0, u1) of the slice-uniform entries of the wavefront's slice (one value for all its lanes).
+__device__ inline void uniform_range(const StoredView& m, int64_t f, int64_t& u0, int64_t& u1) {
+    u0 = u1 = 0;
+    if (!m.uptr) return;
+    const int s = __builtin_amdgcn_readfirstlane(int(f >> 6));
+    u0 = m.uptr[s];
+    u1 = m.uptr[s + 1];
+}
 
 // Read-once streams can be loaded with a non-temporal hint (NT) so that L2
 // keeps the gathered x_j.
@@ -67,8 +94,33 @@ template <class VT, int U, bool NT, bool PACKED>
 __device__ inline C2 stored_row(const StoredView& m, const double* V, const VT* __restrict__ x, int64_t f, C2& xf) {
     const int64_t base = m.sptr[f >> 6] + (f & 63);
     const int n = ld_stream<NT>(m.cnt + f);
+    int64_t u0, u1;
+    uniform_range(m, f, u0, u1);
     if constexpr (is_real_vec<VT>::value) {
         double acc = 0.0;
+        int64_t q = u0;
+        for (; q + U <= u1; q += U) {
+            int vi[U];
+            double sg[U], xv[U];
+            int32_t j[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint2 e = m.uni[q + u];
+                j[u] = int32_t(f) + int32_t(e.x);
+                decode_code<PACKED>(e.y, vi[u], sg[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = double(x[j[u]]);
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += sg[u] * V[2 * vi[u]] * xv[u];
+        }
+        for (; q < u1; ++q) {
+            const uint2 e = m.uni[q];
+            int vi;
+            double sg;
+            decode_code<PACKED>(e.y, vi, sg);
+            acc += sg * V[2 * vi] * double(x[int32_t(f) + int32_t(e.x)]);
+        }
         int k = 0;
         for (; k + U <= n; k += U) {
             int32_t j[U];
@@ -95,6 +147,34 @@ __device__ inline C2 stored_row(const StoredView& m, const double* V, const VT* 
         return C2{sqf * acc + V[2 * dv] * xf.re, 0.0};
     }
     C2 acc{0.0, 0.0};
+    int64_t q = u0;
+    for (; q + U <= u1; q += U) {
+        int vi[U];
+        double sg[U];
+        int32_t j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint2 e = m.uni[q + u];
+            j[u] = int32_t(f) + int32_t(e.x);
+            decode_code<PACKED>(e.y, vi[u], sg[u]);
+        }
+        C2 xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = ld_vec(x, j[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const C2 v{sg[u] * V[2 * vi[u]], sg[u] * V[2 * vi[u] + 1]};
+            acc = cadd(acc, cmul(v, xv[u]));
+        }
+    }
+    for (; q < u1; ++q) {
+        const uint2 e = m.uni[q];
+        int vi;
+        double sg;
+        decode_code<PACKED>(e.y, vi, sg);
+        const C2 xv = ld_vec(x, int32_t(f) + int32_t(e.x));
+        acc = cadd(acc, cmul(C2{sg * V[2 * vi], sg * V[2 * vi + 1]}, xv));
+    }
     int k = 0;
     for (; k + U <= n; k += U) {
         int32_t j[U];
