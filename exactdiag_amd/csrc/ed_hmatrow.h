@@ -2,7 +2,8 @@
 // (ed_hmat.hip) and the batched kernels (ed_batch.hip).
 //
 // Row f of the sliced-ELL gather form: y_f = sqrt(n_f) * sum_k (+-V[code_k])
-// x[col_k] + V[diag_f] x_f, element k at sptr[f / 64] + 64 k + f % 64, as
+// x[col_k] + V[diag_f] x_f, element k at sptr[f / 64] + 64 k + f % 64 (local
+// segment first, stored_row), as
 // packed 32-bit words (column << shift | value index << 1 | negate) or as an
 // int32 column with a uint16 code (bit 15 = negate).  Elements that every row
 // of a 64-row slice has with the same column offset d = column - f and the
@@ -32,6 +33,9 @@ struct StoredView {
     const double* vals;
     const int64_t* uptr;  // per slice: first slice-uniform entry (nullptr: none stored)
     const uint2* uni;     // slice-uniform entries (column offset, code in the element's format)
+    const uint8_t* lcnt;  // per row: local elements (|column - row| <= radius); nullptr: none
+    const uint8_t* lwid;  // per slice: slots of the local segment (the far segment follows)
+    int radius;           // reach of the local elements and of the staged window
 };
 
 template <bool PACKED>
@@ -90,118 +94,92 @@ __device__ inline C2 ld_vec(const VT* __restrict__ x, int64_t j) {
 
 // y_f of the stored product; xf receives x_f for the fused <x, Hx>.
 // Real vectors (VT double or float) use only the real parts of the values.
-template <class VT, int U, bool NT, bool PACKED>
-__device__ inline C2 stored_row(const StoredView& m, const double* V, const VT* __restrict__ x, int64_t f, C2& xf) {
-    const int64_t base = m.sptr[f >> 6] + (f & 63);
-    const int n = ld_stream<NT>(m.cnt + f);
+// Row f's per-lane elements sit in two segments of its slice: local ones
+// (|column - f| <= m.radius) in slots [0, lwid[s]), far ones from slot
+// lwid[s] on.  WIN: x[win0 + i] is staged in LDS as win[i] for every column
+// the local elements, the local slice-uniform entries and x_f can reach; the
+// other elements gather from x.  Without WIN every element gathers from x.
+template <class VT, int U, bool NT, bool PACKED, bool WIN = false>
+__device__ inline C2 stored_row(const StoredView& m, const double* V, const VT* __restrict__ x, int64_t f, C2& xf,
+                                const VT* win = nullptr, int64_t win0 = 0) {
+    constexpr bool REAL = is_real_vec<VT>::value;
+    const int64_t s = f >> 6;
+    const int64_t base = m.sptr[s] + (f & 63);
+    const int nf = ld_stream<NT>(m.cnt + f);
+    const int nl = m.lcnt ? int(ld_stream<NT>(m.lcnt + f)) : 0;
+    const int64_t far0 = base + (m.lwid ? int64_t(m.lwid[s]) * 64 : 0);
     int64_t u0, u1;
     uniform_range(m, f, u0, u1);
-    if constexpr (is_real_vec<VT>::value) {
-        double acc = 0.0;
-        int64_t q = u0;
-        for (; q + U <= u1; q += U) {
-            int vi[U];
-            double sg[U], xv[U];
-            int32_t j[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint2 e = m.uni[q + u];
-                j[u] = int32_t(f) + int32_t(e.x);
-                decode_code<PACKED>(e.y, vi[u], sg[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = double(x[j[u]]);
-#pragma unroll
-            for (int u = 0; u < U; ++u) acc += sg[u] * V[2 * vi[u]] * xv[u];
-        }
-        for (; q < u1; ++q) {
-            const uint2 e = m.uni[q];
-            int vi;
-            double sg;
-            decode_code<PACKED>(e.y, vi, sg);
-            acc += sg * V[2 * vi] * double(x[int32_t(f) + int32_t(e.x)]);
-        }
-        int k = 0;
-        for (; k + U <= n; k += U) {
-            int32_t j[U];
-            int vi[U];
-            double sg[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) ld_elem<PACKED, NT>(m, base + int64_t(k + u) * 64, j[u], vi[u], sg[u]);
-            double xv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = double(x[j[u]]);
-#pragma unroll
-            for (int u = 0; u < U; ++u) acc += sg[u] * V[2 * vi[u]] * xv[u];
-        }
-        for (; k < n; ++k) {
-            int32_t j;
-            int vi;
-            double sg;
-            ld_elem<PACKED, NT>(m, base + int64_t(k) * 64, j, vi, sg);
-            acc += sg * V[2 * vi] * double(x[j]);
-        }
-        const double sqf = m.sq[m.orbit[f]];
-        xf = C2{double(x[f]), 0.0};
-        const int dv = ld_stream<NT>(m.diag + f);
-        return C2{sqf * acc + V[2 * dv] * xf.re, 0.0};
-    }
     C2 acc{0.0, 0.0};
+    auto add = [&](int vi, double sg, const C2& xv) {
+        if constexpr (REAL) {
+            acc.re += sg * V[2 * vi] * xv.re;
+        } else {
+            const C2 v{sg * V[2 * vi], sg * V[2 * vi + 1]};
+            acc = cadd(acc, cmul(v, xv));
+        }
+    };
+    // slice-uniform entries (offset and code the same for the whole wavefront, so is the branch)
     int64_t q = u0;
     for (; q + U <= u1; q += U) {
         int vi[U];
         double sg[U];
-        int32_t j[U];
+        int32_t d[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint2 e = m.uni[q + u];
-            j[u] = int32_t(f) + int32_t(e.x);
+            d[u] = int32_t(e.x);
             decode_code<PACKED>(e.y, vi[u], sg[u]);
         }
         C2 xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = ld_vec(x, j[u]);
-#pragma unroll
         for (int u = 0; u < U; ++u) {
-            const C2 v{sg[u] * V[2 * vi[u]], sg[u] * V[2 * vi[u] + 1]};
-            acc = cadd(acc, cmul(v, xv[u]));
+            const int32_t j = int32_t(f) + d[u];
+            if (WIN && d[u] >= -m.radius && d[u] <= m.radius)
+                xv[u] = ld_vec(win, int64_t(j) - win0);
+            else
+                xv[u] = ld_vec(x, j);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) add(vi[u], sg[u], xv[u]);
     }
     for (; q < u1; ++q) {
         const uint2 e = m.uni[q];
         int vi;
         double sg;
         decode_code<PACKED>(e.y, vi, sg);
-        const C2 xv = ld_vec(x, int32_t(f) + int32_t(e.x));
-        acc = cadd(acc, cmul(C2{sg * V[2 * vi], sg * V[2 * vi + 1]}, xv));
+        const int32_t d = int32_t(e.x), j = int32_t(f) + d;
+        add(vi, sg, (WIN && d >= -m.radius && d <= m.radius) ? ld_vec(win, int64_t(j) - win0) : ld_vec(x, j));
     }
-    int k = 0;
-    for (; k + U <= n; k += U) {
-        int32_t j[U];
-        int vi[U];
-        double sg[U];
+    // per-lane elements: the local segment (from the window when staged), then the far one
+    auto segment = [&](int64_t first, int n, bool local) {
+        int k = 0;
+        for (; k + U <= n; k += U) {
+            int32_t j[U];
+            int vi[U];
+            double sg[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) ld_elem<PACKED, NT>(m, base + int64_t(k + u) * 64, j[u], vi[u], sg[u]);
-        C2 xv[U];
+            for (int u = 0; u < U; ++u) ld_elem<PACKED, NT>(m, first + int64_t(k + u) * 64, j[u], vi[u], sg[u]);
+            C2 xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = ld_vec(x, j[u]);
+            for (int u = 0; u < U; ++u) xv[u] = (WIN && local) ? ld_vec(win, int64_t(j[u]) - win0) : ld_vec(x, j[u]);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const C2 v{sg[u] * V[2 * vi[u]], sg[u] * V[2 * vi[u] + 1]};
-            acc = cadd(acc, cmul(v, xv[u]));
+            for (int u = 0; u < U; ++u) add(vi[u], sg[u], xv[u]);
         }
-    }
-    for (; k < n; ++k) {
-        int32_t j;
-        int vi;
-        double sg;
-        ld_elem<PACKED, NT>(m, base + int64_t(k) * 64, j, vi, sg);
-        const C2 xv = ld_vec(x, j);
-        acc = cadd(acc, cmul(C2{sg * V[2 * vi], sg * V[2 * vi + 1]}, xv));
-    }
+        for (; k < n; ++k) {
+            int32_t j;
+            int vi;
+            double sg;
+            ld_elem<PACKED, NT>(m, first + int64_t(k) * 64, j, vi, sg);
+            add(vi, sg, (WIN && local) ? ld_vec(win, int64_t(j) - win0) : ld_vec(x, j));
+        }
+    };
+    segment(base, nl, true);
+    segment(far0, nf, false);
     const double sqf = m.sq[m.orbit[f]];
-    xf = ld_vec(x, f);
+    xf = WIN ? ld_vec(win, f - win0) : ld_vec(x, f);
     const int dv = ld_stream<NT>(m.diag + f);
+    if constexpr (REAL) return C2{sqf * acc.re + V[2 * dv] * xf.re, 0.0};
     return cadd(C2{sqf * acc.re, sqf * acc.im}, cmul(C2{V[2 * dv], V[2 * dv + 1]}, xf));
 }
 
