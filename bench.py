@@ -302,6 +302,61 @@ def allq_section():
     }
 
 
+PUBLISHED_SECONDS = 3 * 3600.0  # "almost three hours to calculate on my 14 cores"
+
+
+def published_workload():
+    """The reference's one published timing, run through the drop-in API on its own configuration: all-q
+    S(q,w) (`get_Szq_spectra`, limited_qs) of the 2x12 ladder with 2 holes, k=(0,0), weights tl=0.45,
+    tr=0.36, jl=0.15, jr=0.12, 5 eigenpairs, 250 Lanczos vectors, eta=0.05, 1000 w on [0, 0.7]
+    (tests/tJ_spin_half_ladder/manual/paulina_spin_susceptibility_visual.py:117-121, config :162-178).
+    Cache-cold like the quoted figure (a fresh calculations/ folder: bases, eigenpairs and spectra are
+    computed and written), then once more from the disk cache the first run wrote."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from exactdiag_amd.tJ_spin_half_ladder import api, configs
+
+    def config():
+        return configs.Full_Spectrum_Config(
+            hamiltonian={"num_rungs": 12, "num_holes": 2, "weights": {"tl": 0.45, "tr": 0.36, "jl": 0.15, "jr": 0.12},
+                         "symmetry_qs": {"leg": 0, "rung": 0}, "num_threads": 14},
+            spectrum={"name": "Szq", "operator_symmetry_qs": None, "broadening": 5e-2, "omega_max": 0.7,
+                      "num_threads": 14},
+            eigenpair={"num_eigenpairs": 5, "num_threads": 14})
+
+    cwd = os.getcwd()
+    tmp = tempfile.mkdtemp(prefix="ed_published_")
+    try:
+        os.chdir(tmp)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ws, spectra, _ = api.get_Szq_spectra(config(), limited_qs=True)
+        torch.cuda.synchronize()
+        t_cold = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ws2, spectra2, _ = api.get_Szq_spectra(config(), limited_qs=True)
+        t_warm = time.perf_counter() - t0
+        same = bool(np.array_equal(spectra, spectra2))
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(tmp, ignore_errors=True)
+    log(f"published workload: {t_cold:.2f}s cache-cold, {t_warm:.2f}s from the cache; spectra "
+        f"{spectra.shape}, min {spectra.min():.2e}, max {spectra.max():.3f}")
+    return {"workload": "api.get_Szq_spectra(limited_qs=True) on the reference's published configuration: 2x12 "
+                        "ladder, 2 holes, k=(0,0), tl=0.45 tr=0.36 jl=0.15 jr=0.12, 5 eigenpairs, 250 Lanczos "
+                        "vectors, eta=0.05, 1000 w on [0,0.7]; fresh calculations/ folder",
+            "seconds_cold": t_cold, "seconds_from_cache": t_warm, "cache_reproduces": same,
+            "reference": {"seconds": PUBLISHED_SECONDS, "cores": 14,
+                          "source": "'almost three hours to calculate on my 14 cores' "
+                                    "(tests/tJ_spin_half_ladder/manual/paulina_spin_susceptibility_visual.py:117-121)"},
+            "speedup_vs_published": PUBLISHED_SECONDS / t_cold,
+            "spectra_finite_nonnegative": bool(np.all(np.isfinite(spectra)) and spectra.min() > -1e-12)}
+
+
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 
 
@@ -609,6 +664,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
     ap.add_argument("--no-allq", action="store_true", help="skip the all-q time-to-accuracy section")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the reference's published all-q workload run through the drop-in API")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batching, closure (with its GEMM rate) and precision results")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -738,6 +795,9 @@ def main():
     if not args.no_forms and prec == 0:
         torch.cuda.empty_cache()
         line["forms"] = forms_section()
+    if not args.no_published and prec == 0:
+        torch.cuda.empty_cache()
+        line["published"] = published_workload()
     print(json.dumps(line), flush=True)
 
 
