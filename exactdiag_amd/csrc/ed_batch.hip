@@ -74,21 +74,7 @@ __device__ inline int job_index(int nblocks) {
     return wg;
 }
 
-__device__ inline void wg_sum2(double& re, double& im) {
-    __shared__ double red[2 * BS];
-    red[2 * threadIdx.x] = re;
-    red[2 * threadIdx.x + 1] = im;
-    __syncthreads();
-    for (int st = BS / 2; st > 0; st >>= 1) {
-        if (int(threadIdx.x) < st) {
-            red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
-            red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
-        }
-        __syncthreads();
-    }
-    re = red[0];
-    im = red[1];
-}
+__device__ inline void wg_sum2(double& re, double& im) { workgroup_sum2<BS>(re, im); }
 
 __device__ inline bool inactive(const BLz& lz, int inst, int step) {
     return lz.flag && (lz.flag[2 * inst] || step >= lz.nsteps[inst]);

@@ -32,6 +32,9 @@ struct SectorView {
     int n_hole_ranks;  // C(N, h)
     int n_blocks;
     int any_k;         // 1: every lowest image is a basis state (no momentum compatibility test)
+    int flip;          // spin-flip parity z = +1 or -1 (S^z = 0 only), 0 = no spin flip (the reference's group)
+    int G;             // group elements: N, or 2 N with the spin flip (p + N = the flip after p)
+    int NO;            // largest orbit size (N, or 2 N), the amplitude table's n range
     const uint32_t* comb;         // C(n, k) for n, k <= 32
     const uint32_t* canon_gmask;  // per hole rank: bit p set if image p gives the minimal hole mask
     const int32_t* canon_block;   // per hole rank: block of the minimal image (-1 = no basis states)
@@ -42,8 +45,8 @@ struct SectorView {
     const int32_t* table;         // spin rank -> dense index or -1, for blocks with a symmetric hole pattern
     const uint64_t* key;          // per dense index: (H << 32) | Dn of the representative
     const uint8_t* orbit;         // per dense index: orbit size n
-    const double* amp;            // A[((m * 2 + b) * (N + 1) + n) * 2 + {re, im}] = F[m][b] / sqrt(n)
-    const double* sqrt_tab;       // sqrt(n) for n = 0..N, computed on the host
+    const double* amp;            // A[(p * (NO + 1) + n) * 2 + {re, im}] = F[p] / sqrt(n), p = 2m + b (+ N)
+    const double* sqrt_tab;       // sqrt(n) for n = 0..NO, computed on the host
 };
 
 __host__ __device__ inline uint32_t rotl_n(uint32_t x, int s, int N, uint32_t full) {
@@ -61,6 +64,9 @@ __host__ __device__ inline uint32_t image_mask(uint32_t x, int p, int N, uint32_
 }
 
 __host__ __device__ inline uint64_t make_key(uint32_t H, uint32_t Dn) { return (uint64_t(H) << 32) | Dn; }
+
+// Down mask after flipping every spin of the state (holes stay).
+__host__ __device__ inline uint32_t flip_downs(uint32_t H, uint32_t Dn, uint32_t full) { return ~H & full & ~Dn; }
 
 // Workgroup w of a grid of nwg runs on XCD w % 8.  xcd_order gives the piece
 // of work it takes: XCD x walks chunks x, x + 8, ... of `chunk` consecutive

@@ -634,6 +634,7 @@ extern "C" {
 int ed_csr_from_sector(const ed_sector* s, int term, int triangle, int row_major, ed_csr** out, void* stream) {
     return guarded([&] {
         ED_REQUIRE(s && out, ED_ERR_ARG, "null argument");
+        require_no_flip(s, "the reference's matrix folders");
         ED_REQUIRE(term >= 0 && term < 4, ED_ERR_ARG, "term must be 0..3 (tl, tr, jl, jr)");
         *out = build_csr(s, s, hamiltonian_rule(s->L, term), triangle ? 1 : 0, row_major ? 1 : 0, triangle ? 1 : 0,
                          row_major ? 1 : 0, static_cast<hipStream_t>(stream));
@@ -644,6 +645,8 @@ int ed_csr_from_operator(const ed_sector* init, const ed_sector* fin, int op, in
                          void* stream) {
     return guarded([&] {
         ED_REQUIRE(init && fin && out, ED_ERR_ARG, "null argument");
+        require_no_flip(init, "the reference's matrix folders");
+        require_no_flip(fin, "the reference's matrix folders");
         ED_REQUIRE(init->L == fin->L, ED_ERR_ARG, "sectors of different ladders");
         *out = build_csr(init, fin, operator_rule(init->L, op, qx, qy), 0, 0, 0, 0, static_cast<hipStream_t>(stream));
     });
@@ -653,6 +656,7 @@ int ed_csr_from_position_operator(const ed_sector* s, int kind, const int* shift
                                   void* stream) {
     return guarded([&] {
         ED_REQUIRE(s && out && (shifts || n_shifts == 0), ED_ERR_ARG, "null argument");
+        require_no_flip(s, "the reference's matrix folders");
         *out = build_csr(s, s, position_rule(s->L, kind, shifts, n_shifts), 0, 0, 0, 0,
                          static_cast<hipStream_t>(stream));
     });

@@ -131,44 +131,34 @@ __global__ void __launch_bounds__(BS) hm_fill_kernel(SectorView s, HvWeights w, 
 }
 
 // present[n] = 1 for every orbit size n in the basis.
-__global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __restrict__ orbit, int64_t D, int N,
+__global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __restrict__ orbit, int64_t D, int NO,
                                                             int* present) {
     for (int64_t f = int64_t(blockIdx.x) * BS + threadIdx.x; f < D; f += int64_t(gridDim.x) * BS) {
         const int n = orbit[f];
-        if (n <= N && !present[n]) present[n] = 1;
+        if (n <= NO && !present[n]) present[n] = 1;
     }
 }
 
 // Order each row's stored elements so that the 64 lanes of a wavefront, which
 // read slot k of 64 consecutive rows f, gather x[f + d] with one offset d where
 // possible: their addresses are then consecutive.  One wavefront per 64-row
-// slice, one lane per row.  ALIGNED: every element's pair (offset d = column -
-// f, code) is counted over the slice (an LDS hash table), and a row's elements
-// are sorted by that count, largest first, then by column; pairs shared by all
-// rows of a slice (exchange on bonds above the spins that vary within the
-// slice, whose rank change does not depend on those spins) come first and in
-// the same order in every row, where hm_uniform_kernel finds them.
-// Otherwise the rows are sorted by column alone.  A row of up to 32 elements is
-// sorted by a bitonic network in registers; one of up to SORT_CAP is copied
-// into the lane's own LDS column (K[k][lane], no bank conflicts) and
-// insertion-sorted there; a longer row is insertion-sorted in place in HBM.
-// The key is (255 - count) << 32 | packed word, or (255 - count) << 48 |
-// column << 16 | code for the two-array form (count 0 when not ALIGNED).
-constexpr int SORT_CAP = 64;
-constexpr int ALIGN_HASH = 2048;  // pairs tracked per slice (power of two); more are counted as unshared
-constexpr unsigned long long NO_PAIR = ~0ull;  // codes use at most 31 bits, so no pair has this key
-
-__device__ inline uint32_t pair_slot(unsigned long long p) {  // 11 bits
-    return ((uint32_t(p >> 32) * 2654435761u) ^ (uint32_t(p) * 0x9e3779b1u)) >> 21;
-}
-
+// slice, one lane per row.  ALIGNED (full slices whose rows hold at most 32
+// elements): every (offset d = column - f, code) pair of row 0 that all 64
+// rows hold is ranked by its place in row 0, and each row puts its elements
+// with those pairs first, in that order (exchange on bonds above the spins that
+// vary within the slice has the same rank change in every row), then the rest
+// by column; hm_uniform_kernel then finds the shared leading slots.  Otherwise
+// the rows are sorted by column alone.  A row of up to 32 elements is sorted
+// by a bitonic network in registers; a longer one (0.07 % of the rows of the
+// 2x12, 2-hole sector) is insertion-sorted in place in HBM.  The key is
+// rank << 32 | packed word, or rank << 48 | column << 16 | code for the
+// two-array form (rank 255 for elements sorted by column).
 template <bool PACKED, bool ALIGNED>
 __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* __restrict__ sptr,
                                                      const uint8_t* __restrict__ cnt, int32_t* col, uint16_t* code,
                                                      uint32_t* pack, int shift) {
-    __shared__ uint64_t K[SORT_CAP][64];
-    __shared__ unsigned long long hkey[ALIGNED ? ALIGN_HASH : 1];
-    __shared__ int32_t hcnt[ALIGNED ? ALIGN_HASH : 1];
+    constexpr int HB = PACKED ? 32 : 48;  // where the rank sits above the word
+    const uint64_t LOW = (uint64_t(1) << HB) - 1u;
     const int lane = threadIdx.x;
     const int64_t f = int64_t(blockIdx.x) * 64 + lane;
     const bool live = f < D;
@@ -179,66 +169,62 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
         if constexpr (PACKED) return pack[at];
         else return (uint64_t(uint32_t(col[at])) << 16) | code[at];
     };
-    auto column_of = [&](uint64_t w) -> int64_t {
-        if constexpr (PACKED) return int64_t(uint32_t(w) >> shift);
-        else return int64_t(w >> 16);
-    };
-    auto pair_of = [&](uint64_t w) -> unsigned long long {
-        const uint32_t d = uint32_t(int32_t(column_of(w) - f));
-        const uint32_t c = PACKED ? uint32_t(w) & ((1u << shift) - 1u) : uint32_t(w & 0xffffu);
-        return (uint64_t(d) << 32) | c;
-    };
     auto put = [&](int k, uint64_t key) {
         const int64_t at = base + int64_t(k) * 64;
         if constexpr (PACKED) {
-            pack[at] = uint32_t(key);
+            pack[at] = uint32_t(key & LOW);
         } else {
             col[at] = int32_t(uint32_t((key >> 16) & 0xffffffffu));
             code[at] = uint16_t(key & 0xffffu);
         }
     };
-    if constexpr (ALIGNED) {
-        for (int i = lane; i < ALIGN_HASH; i += 64) {
-            hkey[i] = NO_PAIR;
-            hcnt[i] = 0;
-        }
-        __syncthreads();
-        for (int k = 0; k < n; ++k) {
-            const unsigned long long d = pair_of(word_at(k));
-            uint32_t h = pair_slot(d);
-            for (int probe = 0; probe < ALIGN_HASH; ++probe, h = (h + 1) & (ALIGN_HASH - 1)) {
-                const unsigned long long old = atomicCAS(&hkey[h], NO_PAIR, d);
-                if (old == NO_PAIR || old == d) {
-                    atomicAdd(&hcnt[h], 1);
-                    break;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (!live) return;
-    auto key_of = [&](uint64_t w) -> uint64_t {
-        uint64_t c = 0;
-        if constexpr (ALIGNED) {
-            const unsigned long long d = pair_of(w);
-            uint32_t h = pair_slot(d);
-            for (int probe = 0; probe < ALIGN_HASH; ++probe, h = (h + 1) & (ALIGN_HASH - 1)) {
-                const unsigned long long kk = hkey[h];
-                if (kk == d) {
-                    c = uint64_t(min(hcnt[h], 255));
-                    break;
-                }
-                if (kk == NO_PAIR) break;
-            }
-        }
-        return ((255u - c) << (PACKED ? 32 : 48)) | w;
+    // (offset, code) of a word for row `row`
+    auto offset_of = [&](uint64_t w, int64_t row) -> int32_t {
+        if constexpr (PACKED) return int32_t(int64_t(uint32_t(w) >> shift) - row);
+        else return int32_t(int64_t(w >> 16) - row);
     };
-    auto low = [&](uint64_t key) -> uint64_t { return PACKED ? (key & 0xffffffffull) : (key & ((1ull << 48) - 1)); };
-    if (n <= 32) {
+    auto code_of = [&](uint64_t w) -> uint32_t {
+        if constexpr (PACKED) return uint32_t(w) & ((1u << shift) - 1u);
+        else return uint32_t(w & 0xffffu);
+    };
+    int nmax = n;
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    const bool align = ALIGNED && nmax <= 32 && int64_t(blockIdx.x) * 64 + 64 <= D;
+    if (n <= 32 || (align && live)) {
         // up to 32 keys: a bitonic network in registers (slots past n padded with the largest key)
         uint64_t r[32];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) r[k] = k < n ? key_of(word_at(k)) : ~uint64_t(0);
+        for (int k = 0; k < 32; ++k) r[k] = k < n ? ((uint64_t(255) << HB) | word_at(k)) : ~uint64_t(0);
+        if (align) {
+            // row 0's elements (the candidates) through LDS, so that the register array is only indexed by
+            // unrolled constants
+            __shared__ uint64_t row0[32];
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < 32; ++k) row0[k] = r[k];
+            }
+            __syncthreads();
+            const int n0 = __shfl(n, 0, 64);
+            const int64_t f0 = int64_t(blockIdx.x) * 64;
+            for (int c = 0; c < 32; ++c) {
+                if (c < n0) {  // uniform: n0 is lane 0's count
+                    const uint64_t w0 = row0[c];
+                    const int32_t d0 = offset_of(w0 & LOW, f0);
+                    const uint32_t c0 = code_of(w0 & LOW);
+                    int hit = -1;
+#pragma unroll
+                    for (int k = 0; k < 32; ++k)
+                        if (hit < 0 && k < n && (r[k] >> HB) == 255u && offset_of(r[k] & LOW, f) == d0 &&
+                            code_of(r[k] & LOW) == c0)
+                            hit = k;
+                    if (__all(hit >= 0)) {
+#pragma unroll
+                        for (int k = 0; k < 32; ++k)
+                            if (k == hit) r[k] = (uint64_t(c) << HB) | (r[k] & LOW);
+                    }
+                }
+            }
+        }
 #pragma unroll
         for (int size = 2; size <= 32; size <<= 1) {
 #pragma unroll
@@ -257,28 +243,16 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
         }
 #pragma unroll
         for (int k = 0; k < 32; ++k)
-            if (k < n) put(k, low(r[k]));
-    } else if (n <= SORT_CAP) {
-        for (int k = 0; k < n; ++k) K[k][lane] = key_of(word_at(k));
-        for (int k = 1; k < n; ++k) {
-            const uint64_t v = K[k][lane];
-            int q = k - 1;
-            while (q >= 0 && K[q][lane] > v) {
-                K[q + 1][lane] = K[q][lane];
-                --q;
-            }
-            K[q + 1][lane] = v;
-        }
-        for (int k = 0; k < n; ++k) put(k, low(K[k][lane]));
+            if (k < n) put(k, r[k]);
     } else {
         for (int k = 1; k < n; ++k) {
-            const uint64_t w = word_at(k), v = key_of(w);
+            const uint64_t v = word_at(k);
             int q = k - 1;
-            while (q >= 0 && key_of(word_at(q)) > v) {
+            while (q >= 0 && word_at(q) > v) {
                 put(q + 1, word_at(q));
                 --q;
             }
-            put(q + 1, w);
+            put(q + 1, v);
         }
     }
 }
@@ -461,20 +435,10 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(StoredView m, int64_t D, co
         if constexpr (WIN) __syncthreads();  // the next tile's window overwrites this one
     }
     if (partial) {
-        __shared__ double red[2 * BS];
-        red[2 * threadIdx.x] = dot.re;
-        red[2 * threadIdx.x + 1] = dot.im;
-        __syncthreads();
-        for (int st = BS / 2; st > 0; st >>= 1) {
-            if (int(threadIdx.x) < st) {
-                red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
-                red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
-            }
-            __syncthreads();
-        }
+        workgroup_sum2<BS>(dot.re, dot.im);
         if (threadIdx.x == 0) {
-            partial[2 * wg] = red[0];
-            partial[2 * wg + 1] = red[1];
+            partial[2 * wg] = dot.re;
+            partial[2 * wg + 1] = dot.im;
         }
     }
 }
@@ -778,40 +742,43 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
             uid.emplace(key, id);
             return id;
         };
-        const int ncombo = 4 * N * (N + 1);
+        // element codes (elem_code): (term * G + p) * (NO + 1) + n_j, G group elements and orbits up to NO
+        const int G = s->flip ? 2 * N : N, NO = G;
+        const int ncombo = 4 * G * (NO + 1);
         std::vector<uint16_t> map(size_t(ncombo), 0);
         if (coded) {
-            ED_REQUIRE(N <= 32, ED_ERR_ARG, "coded elements need at most 32 group elements");
+            ED_REQUIRE(N <= 32 && !s->flip, ED_ERR_ARG,
+                       "coded elements need at most 32 group elements (no spin flip)");
             vals.assign(2 * 128, 0.0);
             for (int t = 0; t < 4; ++t)
                 for (int p = 0; p < N; ++p)
                     for (int nj = 0; nj <= N; ++nj) map[size_t((t * N + p) * (N + 1) + nj)] = uint16_t(t * 32 + p);
         }
-        // orbit sizes a target can have: those of the basis states (N for generic blocks); values of other
+        // orbit sizes a target can have: those of the basis states (NO for generic blocks); values of other
         // sizes are never addressed, and leaving them out keeps the table small enough for 4-byte elements
-        std::vector<int> nj_present(size_t(N) + 1, 0);
-        nj_present[size_t(N)] = 1;
+        std::vector<int> nj_present(size_t(NO) + 1, 0);
+        nj_present[size_t(NO)] = 1;
         if (!coded && s->D > 0) {
-            int* d_present = dev_alloc<int>(size_t(N) + 1);
+            int* d_present = dev_alloc<int>(size_t(NO) + 1);
             tmp.push_back(d_present);
-            ED_HIP(hipMemsetAsync(d_present, 0, (size_t(N) + 1) * sizeof(int), st));
+            ED_HIP(hipMemsetAsync(d_present, 0, (size_t(NO) + 1) * sizeof(int), st));
             hipLaunchKernelGGL(hm_orbit_sizes_kernel, dim3(unsigned(std::min<int64_t>((s->D + BS - 1) / BS, 4096))),
-                               dim3(BS), 0, st, s->d_orbit, s->D, N, d_present);
+                               dim3(BS), 0, st, s->d_orbit, s->D, NO, d_present);
             ED_HIP(hipGetLastError());
-            std::vector<int> present(size_t(N) + 1, 0);
+            std::vector<int> present(size_t(NO) + 1, 0);
             ED_HIP(hipMemcpyAsync(present.data(), d_present, present.size() * sizeof(int), hipMemcpyDeviceToHost, st));
             ED_HIP(hipStreamSynchronize(st));
-            for (int n = 0; n <= N; ++n) nj_present[size_t(n)] |= present[size_t(n)];
+            for (int n = 0; n <= NO; ++n) nj_present[size_t(n)] |= present[size_t(n)];
         }
         for (int t = 0; t < 4 && !coded; ++t) {
             if (!(m->w.active & (1 << t))) continue;
             const double st_ = t < 2 ? 1.0 : 0.5;
-            for (int p = 0; p < N; ++p)
-                for (int nj = 0; nj <= N; ++nj) {
+            for (int p = 0; p < G; ++p)
+                for (int nj = 0; nj <= NO; ++nj) {
                     if (!nj_present[size_t(nj)]) continue;
-                    const int idx = p * (N + 1) + nj;
+                    const int idx = p * (NO + 1) + nj;
                     const double ar = s->amp[2 * idx], ai = s->amp[2 * idx + 1];
-                    map[size_t((t * N + p) * (N + 1) + nj)] =
+                    map[size_t((t * G + p) * (NO + 1) + nj)] =
                         uint16_t(intern(cmul(m->w.w[t], C2{st_ * ar, -(st_ * ai)})));
                 }
         }
@@ -827,7 +794,7 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
         m->n_vals = int(vals.size() / 2);
         ED_REQUIRE(m->n_vals < 0x8000, ED_ERR_CAPACITY, "more than 32767 unique matrix values");
         if (!coded) {
-            // the product stages the value table (and the window) in LDS, next to its 4 KiB reduction buffer
+            // the product stages the value table (and the window) in LDS, next to its reduction buffer
             int dev = 0, max_lds = 0;
             ED_HIP(hipGetDevice(&dev));
             ED_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));

@@ -20,12 +20,13 @@
 namespace ed {
 namespace {
 
-template <class VT, int BS, int G>
+template <class VT, int BS, int G, bool USE_LDS>
 __global__ void __launch_bounds__(BS) hv_lds_kernel(SectorView s, HvWeights w, const VT* __restrict__ x,
                                                     VT* __restrict__ y, double* __restrict__ partial, LdsLayout lay,
                                                     int64_t ntiles, int64_t tiles_per_wg, int chunk) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const Tabs T = stage_tabs<BS>(s, lay, lds);
+    // a compile-time choice, so that the table reads compile to LDS reads
+    const Tabs T = USE_LDS ? stage_tabs<BS>(s, lay, lds) : global_tabs(s);
     __syncthreads();
     // the eight XCDs on neighbouring chunks of row ranges
     const int wg = xcd_order(blockIdx.x, gridDim.x, chunk);
@@ -44,20 +45,10 @@ __global__ void __launch_bounds__(BS) hv_lds_kernel(SectorView s, HvWeights w, c
         dot = cadd(dot, cmul(C2{xf.re, -xf.im}, acc));
     }
     if (partial) {
-        __shared__ double red[2 * BS];
-        red[2 * threadIdx.x] = dot.re;
-        red[2 * threadIdx.x + 1] = dot.im;
-        __syncthreads();
-        for (int st = BS / 2; st > 0; st >>= 1) {
-            if (int(threadIdx.x) < st) {
-                red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
-                red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
-            }
-            __syncthreads();
-        }
+        workgroup_sum2<BS>(dot.re, dot.im);
         if (threadIdx.x == 0) {
-            partial[2 * wg] = red[0];
-            partial[2 * wg + 1] = red[1];
+            partial[2 * wg] = dot.re;
+            partial[2 * wg + 1] = dot.im;
         }
     }
 }
@@ -75,15 +66,17 @@ void launch_g(int G, int64_t nwg, const LdsLayout& lay, hipStream_t st, const Se
     const VT* xx = static_cast<const VT*>(x);
     VT* yy = static_cast<VT*>(y);
     const dim3 grid{unsigned(nwg), 1u, 1u};
-    if (G == 2)
-        hipLaunchKernelGGL((hv_lds_kernel<VT, BS, 2>), grid, dim3(BS), lay.bytes, st, v, w, xx, yy, partial, lay,
-                           ntiles, tpw, chunk);
+    auto go = [&](auto kernel, size_t bytes) {
+        hipLaunchKernelGGL(kernel, grid, dim3(BS), bytes, st, v, w, xx, yy, partial, lay, ntiles, tpw, chunk);
+    };
+    if (lay.bytes > LDS_CAP)  // only sectors with the spin flip get here: tables from global memory
+        go(hv_lds_kernel<VT, BS, 4, false>, 0);
+    else if (G == 2)
+        go(hv_lds_kernel<VT, BS, 2, true>, size_t(lay.bytes));
     else if (G == 8)
-        hipLaunchKernelGGL((hv_lds_kernel<VT, BS, 8>), grid, dim3(BS), lay.bytes, st, v, w, xx, yy, partial, lay,
-                           ntiles, tpw, chunk);
+        go(hv_lds_kernel<VT, BS, 8, true>, size_t(lay.bytes));
     else
-        hipLaunchKernelGGL((hv_lds_kernel<VT, BS, 4>), grid, dim3(BS), lay.bytes, st, v, w, xx, yy, partial, lay,
-                           ntiles, tpw, chunk);
+        go(hv_lds_kernel<VT, BS, 4, true>, size_t(lay.bytes));
 }
 
 }  // namespace
@@ -101,12 +94,14 @@ int hv_default_tiles(const ed_sector* s) {
 }
 
 // Persistent LDS-table H·v; returns the number of partials written, or -1
-// when the sector's tables do not fit in LDS (caller falls back).
+// when the sector's tables do not fit in LDS (caller falls back).  Sectors
+// with the spin flip, whose amplitude table is four times larger, read their
+// tables from global memory instead (the fallback kernels have no spin flip).
 int64_t launch_hv_lds(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
                       hipStream_t st) {
     constexpr int BS = 256;
     const LdsLayout lay = lds_layout(s);
-    if (lay.bytes > LDS_CAP) return -1;
+    if (lay.bytes > LDS_CAP && !s->flip) return -1;
     const int64_t ntiles = (s->D + BS - 1) / BS;
     if (ntiles == 0) return 0;
     const int64_t tpw = s->tune_tiles > 0 ? s->tune_tiles : hv_default_tiles(s);

@@ -32,8 +32,9 @@ inline LdsLayout lds_layout(const ed_sector* s) {
         return at;
     };
     l.comb = take(COMB_W * COMB_W * 4);
-    l.amp = take(s->L * 2 * (s->N + 1) * 16);
-    l.sq = take((s->N + 1) * 8);
+    const int G = s->flip ? 2 * s->N : s->N;
+    l.amp = take(G * (G + 1) * 16);
+    l.sq = take((G + 1) * 8);
     l.gm = take(int(s->n_hole) * 4);
     l.cb = take(int(s->n_hole) * 4);
     l.bb = take(int(s->block_rank.size()) * 8);
@@ -71,9 +72,9 @@ __device__ inline Tabs stage_tabs(const SectorView& s, const LdsLayout& lay, uns
     int64_t* s_bt = reinterpret_cast<int64_t*>(lds + lay.bt);
     const int t = threadIdx.x;
     for (int i = t; i < COMB_W * COMB_W; i += BS) s_comb[i] = s.comb[i];
-    const int namp = s.L * 2 * (s.N + 1) * 2;
+    const int namp = s.G * (s.NO + 1) * 2;
     for (int i = t; i < namp; i += BS) s_amp[i] = s.amp[i];
-    for (int i = t; i <= s.N; i += BS) s_sq[i] = s.sqrt_tab[i];
+    for (int i = t; i <= s.NO; i += BS) s_sq[i] = s.sqrt_tab[i];
     for (int i = t; i < s.n_hole_ranks; i += BS) {
         s_gm[i] = s.canon_gmask[i];
         s_cb[i] = s.canon_block[i];
@@ -135,37 +136,30 @@ struct Elem {
     uint32_t code;  // stored-form element code (elem_code), NO_ELEM for an empty slot
 };
 
-__device__ inline C2 weighted_conj(const Tabs& T, int N, int L, int ne_even, uint32_t full, uint32_t Hsrc, int p,
-                                   int nj, double scale, C2 W) {
-    const int idx = ((p >> 1) * 2 + (p & 1)) * (N + 1) + nj;
+template <bool FLIP>
+__device__ inline C2 weighted_conj(const SectorView& s, const Tabs& T, uint32_t Hsrc, int p, int nj, double scale,
+                                   C2 W) {
+    const int idx = p * ((FLIP ? s.NO : s.N) + 1) + nj;
     const double ar = T.amp[2 * idx], ai = T.amp[2 * idx + 1];
-    const double sg = double(image_sign(Hsrc, p, L, ne_even, full)) * scale;
+    const double sg = double(group_sign<FLIP>(s, Hsrc, p)) * scale;
     return cmul(W, C2{sg * ar, -(sg * ai)});
 }
 
 // Lowest image among the group elements gm (hole part already minimal),
 // then its row: generic blocks by rank, symmetric blocks by table.
+template <bool FLIP>
 __device__ inline bool locate(const SectorView& s, const Tabs& T, uint32_t Hsrc, uint32_t D2, uint32_t gm, int blk,
                               int& p_out, int64_t& j_out, int& nj_out) {
-    int bp = ctz32(gm);
-    const uint32_t Hc = image_mask(Hsrc, bp, s.N, s.full);
-    uint32_t best = image_mask(D2, bp, s.N, s.full);
-    gm &= gm - 1;
-    while (gm) {
-        const int p = ctz32(gm);
-        const uint32_t d = image_mask(D2, p, s.N, s.full);
-        if (d < best) {
-            best = d;
-            bp = p;
-        }
-        gm &= gm - 1;
-    }
+    const uint32_t Hc = image_mask(Hsrc, ctz32(gm), s.N, s.full);
+    uint32_t best;
+    int bp;
+    lowest_downs<FLIP>(s, Hsrc, D2, gm, best, bp);
     const int32_t sr = rank_spins32(Hc, best, T.comb);
     const int64_t off = T.bt[blk];
     p_out = bp;
     if (off < 0) {
         j_out = T.bb[blk] + sr;
-        nj_out = s.N;
+        nj_out = FLIP ? s.NO : s.N;
         return true;
     }
     const int32_t j = s.table[off + sr];
@@ -181,22 +175,24 @@ __device__ inline C2 ldx(const VT* __restrict__ x, int64_t j) {
     return {double(v.x), double(v.y)};
 }
 
-// Element code for the stored form: combo = (term * N + p) * (N + 1) + n_j
-// (term 0..3 = tl, tr, jl, jr; p the group element; n_j the target's orbit
-// size) in the low 15 bits, bit 15 set when the element's sign is negative.
-// NO_ELEM marks an empty slot of a group.
+// Element code for the stored form: combo = (term * G + p) * (NO + 1) + n_j
+// (term 0..3 = tl, tr, jl, jr; p the group element of G; n_j the target's
+// orbit size, at most NO) in the low 15 bits, bit 15 set when the element's
+// sign is negative.  NO_ELEM marks an empty slot of a group.
 constexpr uint32_t NO_ELEM = 0xffffffffu;
 
-__device__ inline uint32_t elem_code(int term, int N, int p, int nj, bool neg) {
-    return uint32_t((term * N + p) * (N + 1) + nj) | (neg ? 0x8000u : 0u);
+template <bool FLIP>
+__device__ inline uint32_t elem_code(const SectorView& s, int term, int p, int nj, bool neg) {
+    const int G = FLIP ? s.G : s.N, NO = FLIP ? s.NO : s.N;
+    return uint32_t((term * G + p) * (NO + 1) + nj) | (neg ? 0x8000u : 0u);
 }
 
 // Walk the elements of row f in a fixed order (jl bonds, jr bonds, hops onto
 // each hole), G exchange elements or three hops per group, and hand each
 // group to the sink: sink.diag(term, -count/2) for the exchange diagonal of
 // term 2 (jl) and 3 (jr), sink.group(e) for the elements.
-template <int G, class Sink>
-__device__ inline void walk_row(const SectorView& s, const Tabs& T, const HvWeights& w, int64_t f, Sink& sink) {
+template <int G, bool FLIP, class Sink>
+__device__ inline void walk_row_t(const SectorView& s, const Tabs& T, const HvWeights& w, int64_t f, Sink& sink) {
     const int N = s.N, L = s.L, ne_even = s.ne_even;
     const uint32_t full = s.full;
     const uint64_t key = s.key[f];
@@ -207,28 +203,32 @@ __device__ inline void walk_row(const SectorView& s, const Tabs& T, const HvWeig
     const uint32_t gm_self = T.gm[hr];
     const int blk_self = T.cb[hr];
     const bool generic = (gm_self & (gm_self - 1u)) == 0u;
+    // with the spin flip a generic representative has its top electron up: an exchange that turns that spin
+    // down reaches the flipped pattern instead
+    const uint32_t top_e = FLIP ? (1u << (31 - __clz(int(E)))) : 0u;
+    const int NO = FLIP ? s.NO : N;
     // an exchange element: generic blocks by window rank change (coefficient cg, the same for every bond of the
     // term), others by the stabiliser search
     auto exchange = [&](int term, uint32_t D2, int lo, int hi, bool wrap, C2 W, C2 cg, Elem& e) {
-        if (generic) {
+        if (generic && !(D2 & top_e)) {
             e.j = wrap ? T.bb[blk_self] + rank_spins32(H, D2, T.comb) : f + window_delta32(H, Dn, D2, lo, hi, T.comb);
             e.c = cg;
-            e.code = elem_code(term, N, 0, N, false);
+            e.code = elem_code<FLIP>(s, term, 0, NO, false);
             return;
         }
         int p, nj;
         int64_t j;
-        if (locate(s, T, H, D2, gm_self, blk_self, p, j, nj)) {
+        if (locate<FLIP>(s, T, H, D2, gm_self, blk_self, p, j, nj)) {
             e.j = j;
-            e.c = weighted_conj(T, N, L, ne_even, full, H, p, nj, 0.5 * sqf, W);
-            e.code = elem_code(term, N, p, nj, image_sign(H, p, L, ne_even, full) < 0);
+            e.c = weighted_conj<FLIP>(s, T, H, p, nj, 0.5 * sqf, W);
+            e.code = elem_code<FLIP>(s, term, p, nj, group_sign<FLIP>(s, H, p) < 0);
         }
     };
     if (w.active & 4) {  // jl: bonds (i, i + 2)
         const uint32_t d2 = rotl_n(Dn, N - 2, N, full), u2 = rotl_n(Up, N - 2, N, full);
         const uint32_t anti = N == 2 ? 0u : (((Dn & u2) | (Up & d2)) & full);
         const C2 W = w.w[2];
-        const C2 cg = weighted_conj(T, N, L, ne_even, full, H, 0, N, 0.5 * sqf, W);
+        const C2 cg = weighted_conj<FLIP>(s, T, H, 0, NO, 0.5 * sqf, W);
         sink.diag(2, -0.5 * double(popc32(anti)));
         for (int g = 0; g < N; g += G) {
             Elem e[G];
@@ -248,7 +248,7 @@ __device__ inline void walk_row(const SectorView& s, const Tabs& T, const HvWeig
     if (w.active & 8) {  // jr: bonds (2x, 2x + 1)
         const uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
         const C2 W = w.w[3];
-        const C2 cg = weighted_conj(T, N, L, ne_even, full, H, 0, N, 0.5 * sqf, W);
+        const C2 cg = weighted_conj<FLIP>(s, T, H, 0, NO, 0.5 * sqf, W);
         sink.diag(3, -0.5 * double(popc32(anti)));
         for (int g = 0; g < L; g += G) {
             Elem e[G];
@@ -287,15 +287,25 @@ __device__ inline void walk_row(const SectorView& s, const Tabs& T, const HvWeig
                 if (blk < 0) continue;
                 int p, nj;
                 int64_t j;
-                if (locate(s, T, H2, D2, T.gm[hr2], blk, p, j, nj)) {
+                if (locate<FLIP>(s, T, H2, D2, T.gm[hr2], blk, p, j, nj)) {
                     e[k].j = j;
-                    e[k].c = weighted_conj(T, N, L, ne_even, full, H2, p, nj, par ? sqf : -sqf, W);
-                    e[k].code = elem_code(k < 2 ? 0 : 1, N, p, nj, (image_sign(H2, p, L, ne_even, full) < 0) == bool(par));
+                    e[k].c = weighted_conj<FLIP>(s, T, H2, p, nj, par ? sqf : -sqf, W);
+                    e[k].code = elem_code<FLIP>(s, k < 2 ? 0 : 1, p, nj, (group_sign<FLIP>(s, H2, p) < 0) == bool(par));
                 }
             }
             sink.group(e);
         }
     }
+}
+
+// walk_row_t for the sector's group (the spin-flip code only where the
+// sector has the spin flip; the branch is the same for every lane).
+template <int G, class Sink>
+__device__ inline void walk_row(const SectorView& s, const Tabs& T, const HvWeights& w, int64_t f, Sink& sink) {
+    if (s.flip)
+        walk_row_t<G, true>(s, T, w, f, sink);
+    else
+        walk_row_t<G, false>(s, T, w, f, sink);
 }
 
 // Matrix-free sink: gathers of one group are issued together, then

@@ -106,6 +106,34 @@ T* dev_upload(const std::vector<T>& v, hipStream_t st) {
     return p;
 }
 
+// Sum of (re, im) over a workgroup of NT threads (a multiple of 64) in a
+// fixed order: each wavefront adds its 64 values by lane shuffles (xor
+// butterfly; lane 0's association order is fixed), then the NT / 64
+// wavefront sums are added in wavefront order from LDS.  Every thread
+// returns the total; the trailing barrier lets a kernel call it again.
+template <int NT>
+__device__ inline void workgroup_sum2(double& re, double& im) {
+    static_assert(NT % 64 == 0, "whole wavefronts");
+    for (int o = 32; o > 0; o >>= 1) {
+        re += __shfl_xor(re, o, 64);
+        im += __shfl_xor(im, o, 64);
+    }
+    __shared__ double wsum[2 * (NT / 64)];
+    const int w = int(threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wsum[2 * w] = re;
+        wsum[2 * w + 1] = im;
+    }
+    __syncthreads();
+    re = wsum[0];
+    im = wsum[1];
+    for (int k = 1; k < NT / 64; ++k) {
+        re += wsum[2 * k];
+        im += wsum[2 * k + 1];
+    }
+    __syncthreads();
+}
+
 inline void dev_free(void* p) {
     if (p) (void)hipFree(p);
 }
@@ -142,10 +170,11 @@ int shifted_site(int N, int i, int leg, int rung);
 struct ed_sector {
     int L = 0, N = 0, h = 0, ne = 0, nd = 0, kx = 0, ky = 0;
     int any_k = 0;  // union of the momentum sectors: see ed_sector_create_union
+    int flip = 0;   // spin-flip parity (+1 / -1), 0 = without the spin flip: see ed_sector_create_flip
     int64_t n_spin = 0, n_hole = 0;
     std::vector<uint32_t> comb;        // COMB_W * COMB_W
     std::vector<double> amp;           // see SectorView::amp
-    std::vector<double> sqrt_tab;      // N + 1
+    std::vector<double> sqrt_tab;      // NO + 1 (SectorView::NO)
     std::vector<uint32_t> hole_by_rank;
     std::vector<uint32_t> canon_gmask;
     std::vector<int32_t> canon_block, block_of_rank;

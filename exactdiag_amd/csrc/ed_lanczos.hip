@@ -59,19 +59,7 @@ constexpr int UPD_WGS = 2048;  // workgroups of the grid-stride vector kernels (
 
 template <int NT = BS>
 __device__ inline void block_sum2(double& re, double& im) {
-    __shared__ double red[2 * NT];
-    red[2 * threadIdx.x] = re;
-    red[2 * threadIdx.x + 1] = im;
-    __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if (int(threadIdx.x) < st) {
-            red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
-            red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
-        }
-        __syncthreads();
-    }
-    re = red[0];
-    im = red[1];
+    workgroup_sum2<NT>(re, im);
 }
 
 // partial[2 wg] = squared norm of the stored start vector; for real vectors

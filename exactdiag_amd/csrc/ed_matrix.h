@@ -34,5 +34,9 @@ void launch_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& 
 int column_capacity();
 void apply_columns(const ed_sector* ini, const ed_sector* fin, const RuleHost& rule, const double* d_x, double* d_y,
                    hipStream_t st);
+// Sectors with the spin flip: an excitation operator between them must carry a definite spin-flip parity
+// (Szq reverses z, N(q) and the currents keep it; A+- change S^z); other entry points take none (ED_ERR_ARG).
+void require_flip_operator(const ed_sector* ini, const ed_sector* fin, int op);
+void require_no_flip(const ed_sector* s, const char* what);
 
 }  // namespace ed

@@ -397,20 +397,10 @@ __global__ void __launch_bounds__(BS) hv_kernel(SectorView s, HvWeights w, const
     }
     if (partial) {
         // deterministic workgroup reduction of <x, y>
-        __shared__ double red[2 * BS];
-        red[2 * threadIdx.x] = dot.re;
-        red[2 * threadIdx.x + 1] = dot.im;
-        __syncthreads();
-        for (int st = BS / 2; st > 0; st >>= 1) {
-            if (int(threadIdx.x) < st) {
-                red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
-                red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
-            }
-            __syncthreads();
-        }
+        workgroup_sum2<BS>(dot.re, dot.im);
         if (threadIdx.x == 0) {
-            partial[2 * bid] = red[0];
-            partial[2 * bid + 1] = red[1];
+            partial[2 * bid] = dot.re;
+            partial[2 * bid + 1] = dot.im;
         }
     }
 }
@@ -571,20 +561,10 @@ __global__ void __launch_bounds__(BS) hv_lockstep_kernel(SectorView s, HvWeights
         dot = cmul(C2{xf.re, -xf.im}, acc);
     }
     if (partial) {
-        __shared__ double red[2 * BS];
-        red[2 * threadIdx.x] = dot.re;
-        red[2 * threadIdx.x + 1] = dot.im;
-        __syncthreads();
-        for (int st = BS / 2; st > 0; st >>= 1) {
-            if (int(threadIdx.x) < st) {
-                red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
-                red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
-            }
-            __syncthreads();
-        }
+        workgroup_sum2<BS>(dot.re, dot.im);
         if (threadIdx.x == 0) {
-            partial[2 * bid] = red[0];
-            partial[2 * bid + 1] = red[1];
+            partial[2 * bid] = dot.re;
+            partial[2 * bid + 1] = dot.im;
         }
     }
 }
@@ -764,22 +744,12 @@ __global__ void zero_kernel(double* p, int64_t n) {
 // Sum of n (re, im) partials in a fixed order: 1024 threads, each with four
 // independent 16-byte loads in flight, then a tree.
 __global__ void __launch_bounds__(1024) reduce_partials_kernel(const double* partial, int64_t n, double* out) {
-    __shared__ double red[2048];
     double re, im;
     sum_partials<1024>(partial, n, re, im);
-    red[2 * threadIdx.x] = re;
-    red[2 * threadIdx.x + 1] = im;
-    __syncthreads();
-    for (int st = 512; st > 0; st >>= 1) {
-        if (int(threadIdx.x) < st) {
-            red[2 * threadIdx.x] += red[2 * (threadIdx.x + st)];
-            red[2 * threadIdx.x + 1] += red[2 * (threadIdx.x + st) + 1];
-        }
-        __syncthreads();
-    }
+    workgroup_sum2<1024>(re, im);
     if (threadIdx.x == 0) {
-        out[0] = red[0];
-        out[1] = red[1];
+        out[0] = re;
+        out[1] = im;
     }
 }
 
@@ -875,7 +845,7 @@ int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y
         const char* e = std::getenv("ED_HV_KERNEL");
         return e ? std::atoi(e) : 2;
     }();
-    if (variant == 2) {
+    if (variant == 2 || s->flip) {
         const int64_t n = launch_hv_lds(s, w, x, y, partial, precision, st);
         if (n >= 0) return n;
     }
@@ -903,6 +873,19 @@ int64_t hv_partials(const ed_sector* s) { return (s->D + 255) / 256; }
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st) {
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, st, partial, n, out);
     ED_HIP(hipGetLastError());
+}
+
+void require_flip_operator(const ed_sector* ini, const ed_sector* fin, int op) {
+    if (!ini->flip && !fin->flip) return;
+    const bool keeps = op == ED_OP_NQ || op == ED_OP_CURRENT_RUNG || op == ED_OP_CURRENT_LEG;
+    ED_REQUIRE(op == ED_OP_SZQ || keeps, ED_ERR_ARG,
+               "between spin-flip sectors only Szq, N(q) and the currents (A+- change S^z)");
+    ED_REQUIRE(ini->flip && fin->flip && fin->flip == (keeps ? ini->flip : -ini->flip), ED_ERR_ARG,
+               "the final spin-flip parity must be the initial one (N(q), currents) or its opposite (Szq)");
+}
+
+void require_no_flip(const ed_sector* s, const char* what) {
+    ED_REQUIRE(!s || !s->flip, ED_ERR_ARG, std::string(what) + " is not available for sectors with the spin flip");
 }
 
 HvWeights make_weights(const double* w) {
@@ -934,6 +917,8 @@ int ed_op_columns(const ed_sector* ini, const ed_sector* fin, int op, int qx, in
                   int32_t* d_count, int64_t* d_rows, double* d_vals, void* stream) {
     return guarded([&] {
         ED_REQUIRE(ini && fin, ED_ERR_ARG, "sector is null");
+        require_flip_operator(ini, fin, op);
+        ED_REQUIRE(ini && fin, ED_ERR_ARG, "sector is null");
         run_columns(ini, fin, operator_rule(ini->L, op, qx, qy), 0, max_per_col, d_count, d_rows, d_vals,
                     static_cast<hipStream_t>(stream));
     });
@@ -955,6 +940,8 @@ int ed_hv(const ed_sector* s, const double* w, const void* d_x, void* d_y, doubl
 int ed_op_apply(const ed_sector* ini, const ed_sector* fin, int op, int qx, int qy, const double* d_x, double* d_y,
                 void* stream) {
     return guarded([&] {
+        ED_REQUIRE(ini && fin, ED_ERR_ARG, "sector is null");
+        require_flip_operator(ini, fin, op);
         ED_REQUIRE(ini && fin && ini->D >= 0 && fin->D >= 0, ED_ERR_STATE, "basis not built");
         hipStream_t st = static_cast<hipStream_t>(stream);
         const RuleHost rule = operator_rule(ini->L, op, qx, qy);
@@ -976,6 +963,7 @@ int ed_op_apply(const ed_sector* ini, const ed_sector* fin, int op, int qx, int 
 int ed_position_op_columns(const ed_sector* s, int kind, const int* shifts, int n_shifts, int max_per_col,
                            int32_t* d_count, int64_t* d_rows, double* d_vals, void* stream) {
     return guarded([&] {
+        require_no_flip(s, "position correlations");
         ED_REQUIRE(s && (shifts || n_shifts == 0), ED_ERR_ARG, "null argument");
         run_columns(s, s, position_rule(s->L, kind, shifts, n_shifts), 0, max_per_col, d_count, d_rows, d_vals,
                     static_cast<hipStream_t>(stream));
@@ -985,6 +973,7 @@ int ed_position_op_columns(const ed_sector* s, int kind, const int* shifts, int 
 int ed_position_op_apply(const ed_sector* s, int kind, const int* shifts, int n_shifts, const double* d_x,
                          double* d_y, void* stream) {
     return guarded([&] {
+        require_no_flip(s, "position correlations");
         ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
         ED_REQUIRE(shifts || n_shifts == 0, ED_ERR_ARG, "null shifts");
         if (s->D == 0) return;
@@ -995,6 +984,7 @@ int ed_position_op_apply(const ed_sector* s, int kind, const int* shifts, int n_
 int ed_position_expectation(const ed_sector* s, int kind, const int* shifts, int n_shifts, const double* d_psi,
                             double* out, void* stream) {
     return guarded([&] {
+        require_no_flip(s, "position correlations");
         ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
         ED_REQUIRE(out && (shifts || n_shifts == 0), ED_ERR_ARG, "null argument");
         out[0] = out[1] = 0.0;
@@ -1021,6 +1011,7 @@ int ed_position_expectation(const ed_sector* s, int kind, const int* shifts, int
 int ed_position_correlations(const ed_sector* s, int kind, const int* fixed, int n_fixed, const double* d_psi,
                              double* out, void* stream) {
     return guarded([&] {
+        require_no_flip(s, "position correlations");
         ED_REQUIRE(s && s->D >= 0, ED_ERR_STATE, "basis not built");
         ED_REQUIRE(out && (fixed || n_fixed == 0), ED_ERR_ARG, "null argument");
         ED_REQUIRE(kind == ED_POS_HOLES || kind == ED_POS_SZ, ED_ERR_ARG, "kind must be holes or S^z");

@@ -511,6 +511,7 @@ void launch_mk(const ed_mk* m, const VT* x, VT* y, double* partial, hipStream_t 
 void check_slot_sector(const ed_mk* m, int slot, const ed_sector* sk) {
     ED_REQUIRE(slot >= 0 && slot < m->n_slots, ED_ERR_ARG, "slot out of range");
     ED_REQUIRE(sk && sk->D >= 0 && sk->d_key, ED_ERR_STATE, "sector basis not built");
+    require_no_flip(sk, "the multi-momentum product");
     ED_REQUIRE(sk->L == m->u->L && sk->h == m->u->h && sk->nd == m->u->nd && !sk->any_k, ED_ERR_ARG,
                "sector is not one of this ladder's momentum sectors");
     const int L = m->u->L;

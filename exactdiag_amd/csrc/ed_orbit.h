@@ -5,10 +5,19 @@
 
 namespace ed {
 
-// Amplitude table access: A[m][b][n] = F[m][b] / sqrt(n), F[m][b][1] == F.
+// Amplitude table access: A[p][n] = F[p] / sqrt(n), F[p][1] == F; with the
+// spin flip F[p + N] = z F[p].
 __host__ __device__ inline C2 amp_at(const SectorView& s, int p, int n) {
-    const int idx = ((p >> 1) * 2 + (p & 1)) * (s.N + 1) + n;
+    const int idx = p * (s.NO + 1) + n;
     return {s.amp[2 * idx], s.amp[2 * idx + 1]};
+}
+
+// Fermion sign of group element p (the spin flip relabels spins without
+// reordering any operator: sign +1).  FLIP: the sector may have the spin flip.
+template <bool FLIP = true>
+__host__ __device__ inline int group_sign(const SectorView& s, uint32_t H, int p) {
+    if constexpr (FLIP) p = p < s.N ? p : p - s.N;
+    return image_sign(H, p, s.L, s.ne_even, s.full);
 }
 
 struct OrbitResult {
@@ -16,6 +25,36 @@ struct OrbitResult {
     int best_p;     // first group element mapping the state to its lowest image
     uint64_t best;  // key of the lowest image
 };
+
+// The orbit under translations x leg mirror x spin flip (no counterpart in the
+// reference): every one of the 2N elements, p + N being the flip after p.
+// The stabiliser elements (images equal to the state) must all have
+// character x sign 1, else the state has no component in the sector; the
+// orbit size is 2N / |stabiliser|; the first element reaching the lowest
+// image is kept.
+__host__ __device__ inline OrbitResult orbit_walk_flip(const SectorView& s, uint32_t H, uint32_t Dn,
+                                                       bool check_lowest) {
+    const uint64_t k0 = make_key(H, Dn);
+    const uint32_t Df = flip_downs(H, Dn, s.full);
+    uint64_t best = k0;
+    int best_p = 0, stab = 1;
+    for (int q = 1; q < 2 * s.N; ++q) {
+        const int p = q < s.N ? q : q - s.N;
+        const uint64_t kt = make_key(image_mask(H, p, s.N, s.full), image_mask(q < s.N ? Dn : Df, p, s.N, s.full));
+        if (check_lowest && kt < k0) return {0, 0, 0};
+        if (kt == k0) {
+            const C2 F = amp_at(s, q, 1);
+            const double sg = double(image_sign(H, p, s.L, s.ne_even, s.full));
+            const double dr = sg * F.re - 1.0, di = sg * F.im;
+            if (!s.any_k && dr * dr + di * di > 1e-16) return {0, 0, 0};
+            ++stab;
+        } else if (kt < best) {
+            best = kt;
+            best_p = q;
+        }
+    }
+    return {2 * s.N / stab, best_p, best};
+}
 
 // The reference's orbit walk, tJ_spin_half_ladder/symmetry_ladder.h:113-235:
 // group elements in the order p = 1 .. N-1, optional rejection as soon as an
@@ -27,7 +66,9 @@ struct OrbitResult {
 // sign * F[m][b] exactly as the reference's running product produces them
 // (the running product differs from the unsigned one only by exact sign
 // flips).
+// With the spin flip, orbit_walk_flip instead.
 __host__ __device__ inline OrbitResult orbit_walk(const SectorView& s, uint32_t H, uint32_t Dn, bool check_lowest) {
+    if (s.flip) return orbit_walk_flip(s, H, Dn, check_lowest);
     const uint64_t k0 = make_key(H, Dn);
     bool mirror_sym = false;
     int count = 1;
@@ -79,7 +120,7 @@ __host__ __device__ inline OrbitResult orbit_walk(const SectorView& s, uint32_t 
 // (general/symmetry.h:240-269): sign(H, p) * F[m][b] / sqrt(n).
 __host__ __device__ inline C2 lowest_amplitude(const SectorView& s, uint32_t H, int p, int n) {
     const C2 a = amp_at(s, p, n);
-    const double sg = double(image_sign(H, p, s.L, s.ne_even, s.full));
+    const double sg = double(group_sign(s, H, p));
     return {sg * a.re, sg * a.im};
 }
 
@@ -97,7 +138,7 @@ __device__ inline Found lookup_rep(const SectorView& s, uint32_t Hr, uint32_t Dr
     if (blk < 0) return {-1, 0};
     const int64_t sr = rank_spins(Hr, Dr, s.comb);
     const int64_t off = s.block_table[blk];
-    if (off < 0) return {s.block_base[blk] + sr, s.N};
+    if (off < 0) return {s.block_base[blk] + sr, s.NO};
     const int32_t j = s.table[off + sr];
     if (j < 0) return {-1, 0};
     return {int64_t(j), int(s.orbit[j])};
@@ -109,28 +150,43 @@ struct Rep {
     int blk;
 };
 
-// Lowest image of an arbitrary state using the per-hole-rank table of group
-// elements that minimise the hole pattern.  Among them the down pattern is
-// compared in increasing p, first minimum wins, which is the element the
-// reference's orbit walk stores first.
-__device__ inline Rep find_rep(const SectorView& s, uint32_t H, uint32_t Dn) {
-    const int64_t hr = rank_holes(H, s.comb);
-    uint32_t gm = s.canon_gmask[hr];
-    const int blk = s.canon_block[hr];
-    const int p0 = ctz32(gm);
-    const uint32_t Hc = image_mask(H, p0, s.N, s.full);
-    uint32_t best = image_mask(Dn, p0, s.N, s.full);
-    int bp = p0;
-    gm &= gm - 1;
+// Lowest image of the down pattern among the group elements gm (bit p: image
+// p gives the minimal hole pattern): compared in increasing p, first minimum
+// wins, which is the element the reference's orbit walk stores first.  With
+// the spin flip each element's flipped image (element p + N) follows it.
+template <bool FLIP = true>
+__host__ __device__ inline void lowest_downs(const SectorView& s, uint32_t H, uint32_t Dn, uint32_t gm,
+                                             uint32_t& best, int& bp) {
+    const uint32_t Ef = FLIP && s.flip ? ~H & s.full : 0u;
+    bp = -1;
     while (gm) {
         const int p = ctz32(gm);
         const uint32_t d = image_mask(Dn, p, s.N, s.full);
-        if (d < best) {
+        if (bp < 0 || d < best) {
             best = d;
             bp = p;
         }
+        if (FLIP && s.flip) {
+            const uint32_t df = image_mask(Ef, p, s.N, s.full) & ~d;
+            if (df < best) {
+                best = df;
+                bp = p + s.N;
+            }
+        }
         gm &= gm - 1;
     }
+}
+
+// Lowest image of an arbitrary state using the per-hole-rank table of group
+// elements that minimise the hole pattern.
+__device__ inline Rep find_rep(const SectorView& s, uint32_t H, uint32_t Dn) {
+    const int64_t hr = rank_holes(H, s.comb);
+    const uint32_t gm = s.canon_gmask[hr];
+    const int blk = s.canon_block[hr];
+    const uint32_t Hc = image_mask(H, ctz32(gm), s.N, s.full);
+    uint32_t best;
+    int bp;
+    lowest_downs(s, H, Dn, gm, best, bp);
     return {Hc, best, bp, blk};
 }
 

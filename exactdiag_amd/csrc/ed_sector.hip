@@ -375,6 +375,9 @@ ed::SectorView ed_sector::host_view() const {
     v.n_hole_ranks = int(n_hole);
     v.n_blocks = int(block_rank.size());
     v.any_k = any_k;
+    v.flip = flip;
+    v.G = flip ? 2 * N : N;
+    v.NO = v.G;
     v.comb = comb.data();
     v.amp = amp.data();
     v.sqrt_tab = sqrt_tab.data();
@@ -426,7 +429,7 @@ int ed_version(void) { return 1; }
 namespace ed {
 namespace {
 
-int sector_create(int L, int h, int nd, int kx, int ky, int any_k, ed_sector** out) {
+int sector_create(int L, int h, int nd, int kx, int ky, int any_k, int flip, ed_sector** out) {
     return guarded([&] {
         ED_REQUIRE(out != nullptr, ED_ERR_ARG, "out is null");
         ED_REQUIRE(L >= 1 && 2 * L <= MAX_N, ED_ERR_ARG, "num_rungs must be in 1..16");
@@ -434,7 +437,12 @@ int sector_create(int L, int h, int nd, int kx, int ky, int any_k, ed_sector** o
         ED_REQUIRE(h >= 0 && h <= N, ED_ERR_ARG, "num_holes out of range");
         ED_REQUIRE(nd >= 0 && nd <= N - h, ED_ERR_ARG, "num_down_spins out of range");
         ED_REQUIRE(ky == 0 || ky == 1, ED_ERR_ARG, "rung quantum number must be 0 or 1");
+        ED_REQUIRE(flip == 0 || flip == 1 || flip == -1, ED_ERR_ARG, "spin-flip parity must be +1 or -1 (0: none)");
+        ED_REQUIRE(!flip || (2 * nd == N - h && nd > 0), ED_ERR_ARG,
+                   "the spin flip needs S^z = 0 (as many down as up spins) and at least one electron");
+        ED_REQUIRE(!(flip && any_k), ED_ERR_ARG, "no spin flip in the union of momentum sectors");
         auto* s = new ed_sector;
+        s->flip = flip;
         s->L = L;
         s->N = N;
         s->h = h;
@@ -462,20 +470,24 @@ int sector_create(int L, int h, int nd, int kx, int ky, int any_k, ed_sector** o
         std::vector<cd> E(L + 1);
         E[0] = 1;
         for (int m = 1; m <= L; ++m) E[m] = E[m - 1] * rung_exp;
-        s->amp.assign(size_t(L) * 2 * (N + 1) * 2, 0.0);
-        s->sqrt_tab.assign(N + 1, 0.0);
-        for (int n = 0; n <= N; ++n) s->sqrt_tab[n] = std::sqrt(double(n));
-        for (int m = 0; m < L; ++m)
-            for (int b = 0; b < 2; ++b) {
-                const cd F = b ? E[m] * mirror_exp : E[m];
-                for (int n = 1; n <= N; ++n) {
-                    cd a = F;
-                    a /= std::sqrt(n);
-                    const size_t idx = (size_t(m) * 2 + b) * (N + 1) + n;
-                    s->amp[2 * idx] = a.real();
-                    s->amp[2 * idx + 1] = a.imag();
+        // with the spin flip: elements p + N (the flip after p) with character z F[p], orbits up to 2N
+        const int G = flip ? 2 * N : N, NO = G;
+        s->amp.assign(size_t(G) * (NO + 1) * 2, 0.0);
+        s->sqrt_tab.assign(NO + 1, 0.0);
+        for (int n = 0; n <= NO; ++n) s->sqrt_tab[n] = std::sqrt(double(n));
+        for (int f = 0; f < (flip ? 2 : 1); ++f)
+            for (int m = 0; m < L; ++m)
+                for (int b = 0; b < 2; ++b) {
+                    cd F = b ? E[m] * mirror_exp : E[m];
+                    if (f && flip < 0) F = -F;
+                    for (int n = 1; n <= NO; ++n) {
+                        cd a = F;
+                        a /= std::sqrt(n);
+                        const size_t idx = (size_t(f) * N + size_t(m) * 2 + b) * (NO + 1) + n;
+                        s->amp[2 * idx] = a.real();
+                        s->amp[2 * idx + 1] = a.imag();
+                    }
                 }
-            }
         // hole patterns: eligibility prefilter (symmetry_ladder.h:363-405),
         // minimising group elements, blocks
         SectorView hv = s->host_view();
@@ -500,7 +512,9 @@ int sector_create(int L, int h, int nd, int kx, int ky, int any_k, ed_sector** o
             s->canon_gmask[r] = gm;
             canon_rank[r] = rank_holes(best, s->comb.data());
             bool eligible;
-            if (h <= 1 || h >= N - 1) {
+            if (flip) {
+                eligible = best == H;  // the pattern is the minimum of its orbit; states are tested one by one
+            } else if (h <= 1 || h >= N - 1) {
                 eligible = r == 0;
             } else {
                 const uint32_t free = ~H & hv.full;
@@ -537,10 +551,16 @@ int sector_create(int L, int h, int nd, int kx, int ky, int any_k, ed_sector** o
 extern "C" {
 
 int ed_sector_create(int L, int h, int nd, int kx, int ky, ed_sector** out) {
-    return sector_create(L, h, nd, kx, ky, 0, out);
+    return sector_create(L, h, nd, kx, ky, 0, 0, out);
 }
 
-int ed_sector_create_union(int L, int h, int nd, ed_sector** out) { return sector_create(L, h, nd, 0, 0, 1, out); }
+int ed_sector_create_flip(int L, int h, int nd, int kx, int ky, int z, ed_sector** out) {
+    return sector_create(L, h, nd, kx, ky, 0, z, out);
+}
+
+int ed_sector_flip(const ed_sector* s) { return s ? s->flip : 0; }
+
+int ed_sector_create_union(int L, int h, int nd, ed_sector** out) { return sector_create(L, h, nd, 0, 0, 1, 0, out); }
 
 int ed_sector_destroy(ed_sector* s) {
     return guarded([&] {
@@ -594,7 +614,10 @@ int ed_basis_build(ed_sector* s, void* stream) {
                 acc += hc[b * cpb + c];
                 in_block += hc[b * cpb + c];
             }
-            if (!s->block_sym[b] && in_block != s->n_spin) {
+            // a pattern without stabiliser keeps every spin configuration; with the spin flip those whose top
+            // electron is up (the lower of each flipped pair), the first C(ne - 1, nd) spin ranks
+            const int64_t generic = s->flip ? int64_t(s->comb[s->nd * COMB_W + s->ne - 1]) : s->n_spin;
+            if (!s->block_sym[b] && in_block != generic) {
                 dev_free(flags);
                 dev_free(counts);
                 throw Fail{ED_ERR_STATE, "hole pattern without stabiliser lost spin configurations"};

@@ -103,7 +103,8 @@ def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=3
         if res <= tol * max(1.0, abs(e)) or len(a) < min(nsteps, dim):
             return e, x, restart, res
         v = x
-    return e, x, max_restarts, res
+    raise RuntimeError(f"two-pass ground state: residual {res:.3e} above {tol:.1e} x max(1, |E|) after "
+                       f"{max_restarts} restarts (E = {e:.12f})")
 
 
 def precision_form(L=14, h=2, nd=13, k=(7, 0), q=(7, 1), weights=(1.0, 1.0, 0.4, 0.4), nsteps=300,

@@ -125,7 +125,8 @@ def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=
     Returns (eigenvalues ascending, device tensor (nev, D) of eigenvectors, restarts).
     Convergence: residual norm of every wanted Ritz pair below tol * max(1, |theta|).
     real=True: ``apply`` is a real symmetric operator on float64 vectors; the basis is kept in float64 (half the
-    bytes), the real part of v0 starts it, and the eigenvectors come back as complex128 with zero imaginary part.
+    bytes), v0 rotated by its global phase starts it (a seeded real vector when v0 stays complex), and the
+    eigenvectors come back as complex128 with zero imaginary part.
     The vector helpers then see each real vector as complex pairs (length padded to even): the real part of a
     pair-view dot is the real dot, and real coefficients give the real combinations.
     """
@@ -137,6 +138,16 @@ def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=
     else:
         v = torch.as_tensor(start_vector(D) if v0 is None else np.asarray(v0, np.complex128), device="cuda")
     if real:
+        # a complex guess is rotated by its global phase first (its real part alone could vanish); a guess that
+        # stays complex gives way to a seeded real start vector
+        from exactdiag_amd.sector import real_phase
+
+        rotated = real_phase(v) if v0 is not None else v
+        if rotated is None:
+            rotated = torch.as_tensor(np.random.default_rng(0).normal(size=D), device="cuda").to(torch.complex128)
+        v = rotated
+        if float(torch.linalg.vector_norm(v.real)) == 0.0:
+            raise ValueError("the start vector of the real Krylov basis is zero")
         Dp = D + (D & 1)
         Vr = torch.zeros((m + 1, Dp), dtype=torch.float64, device="cuda")
         V = torch.view_as_complex(Vr.view(m + 1, Dp // 2, 2))

@@ -55,16 +55,23 @@ def weights_array(weights) -> np.ndarray:
 class Sector:
     """Basis of one (L, h, nd, kx, ky) sector, built on the GPU."""
 
-    def __init__(self, num_rungs: int, num_holes: int, num_down_spins: int, kx: int, ky: int, *, _union=False):
+    def __init__(self, num_rungs: int, num_holes: int, num_down_spins: int, kx: int, ky: int, *, flip: int = 0,
+                 _union=False):
+        """``flip`` = +1 or -1 adds the global spin flip with that parity to the symmetry group (S^z = 0 sectors
+        only; no counterpart in the reference, see ed_sector_create_flip); 0 keeps the reference's group."""
         self.L, self.h, self.nd, self.kx, self.ky = num_rungs, num_holes, num_down_spins, int(kx), int(ky)
         self.N = 2 * num_rungs
         self.any_k = bool(_union)
+        self.flip = int(flip)
         self._lib = lib()
         torch = _torch()
         self.device = torch.device("cuda", torch.cuda.current_device())
         h = ct.c_void_p()
         if _union:
             check(self._lib.ed_sector_create_union(num_rungs, num_holes, num_down_spins, ct.byref(h)))
+        elif self.flip:
+            check(self._lib.ed_sector_create_flip(num_rungs, num_holes, num_down_spins, self.kx, self.ky, self.flip,
+                                                  ct.byref(h)))
         else:
             check(self._lib.ed_sector_create(num_rungs, num_holes, num_down_spins, self.kx, self.ky, ct.byref(h)))
         self._h = h

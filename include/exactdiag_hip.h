@@ -70,6 +70,17 @@ int ed_version(void);
  * Host tables (binomials, amplitudes, hole patterns) are built here. */
 int ed_sector_create(int L, int h, int nd, int kx, int ky, ed_sector** out);
 int ed_sector_destroy(ed_sector* s);
+/* A sector of the group extended by the global spin flip, with parity z = +1
+ * or -1 (S^z = 0 only).  No counterpart in the reference, whose ladder group is
+ * leg translations x leg mirror (tJ_spin_half_ladder/symmetry_ladder.h:55-70):
+ * optional, for callers that want the halved dimension; the sectors of
+ * ed_sector_create keep the reference's bases.  The spin flip relabels spins
+ * without a fermion sign; the H·v, stored-H, Lanczos and batched entry points
+ * and the operators Szq (z -> -z), N(q) and the currents (z -> z) take these
+ * sectors; A+-, position correlations, the reference's matrix folders and
+ * the multi-momentum product refuse them (ED_ERR_ARG). */
+int ed_sector_create_flip(int L, int h, int nd, int kx, int ky, int z, ed_sector** out);
+int ed_sector_flip(const ed_sector* s); /* z, or 0 without the spin flip */
 
 /* Enumerate the symmetric basis on the GPU.  Replaces calc_basis_map
  * (general/basis_indexing.h:188-242). */

@@ -209,3 +209,33 @@ def test_thick_restart_in_real_vectors(torch, oracle_mod):
         if s.dim <= 3000:
             ev = np.linalg.eigvalsh(oracle_mod.OracleSector(*p).toarray(WEIGHTS))
             np.testing.assert_allclose(er, ev[:3], rtol=0, atol=1e-9 * max(1.0, abs(ev[0])))
+
+
+def test_degenerate_ground_states_orthonormal_in_real_vectors(torch, oracle_mod):
+    """A real sector whose ground state is two-fold degenerate (Heisenberg exchange with static holes, 2x6 with 3
+    holes): the float64 Krylov basis returns orthonormal eigenvectors, the same eigenvalues as the complex128
+    basis and the dense spectrum, and -- where both find the pair -- the same degenerate subspace, so the
+    degenerate averages that calculate_spectrum forms agree."""
+    from exactdiag_amd.general import krylov
+    from exactdiag_amd.sector import Sector
+
+    p, w = (6, 3, 4, 0, 0), [0.0, 0.0, 1.0, 1.0]
+    s = Sector(*p)
+    m = s.store(w)
+    assert m.is_real
+    ev = np.linalg.eigvalsh(np.conj(oracle_mod.OracleSector(*p).toarray(w)))
+    assert ev[1] - ev[0] < 1e-9 < ev[2] - ev[1]
+    out = {}
+    for real in (True, False):
+        vals, X, _ = krylov.lowest_eigenpairs(m.hv, s.dim, 3, tol=1e-11, real=real)
+        G = (X.conj() @ X.T).cpu().numpy()
+        assert np.max(np.abs(G - np.eye(len(vals)))) < 1e-10, real
+        for e in vals:
+            assert np.min(np.abs(ev - e)) < 1e-9
+        assert abs(vals[0] - ev[0]) < 1e-10
+        deg = X[np.abs(vals - ev[0]) < 1e-6]
+        out[real] = (vals, deg)
+        print("real" if real else "complex", vals, "degenerate copies found:", deg.shape[0], flush=True)
+    if out[True][1].shape[0] == out[False][1].shape[0] == 2:
+        P = [(d.T @ d.conj()).cpu().numpy() for d in (out[True][1], out[False][1])]
+        assert np.max(np.abs(P[0] - P[1])) < 1e-8

@@ -8,112 +8,12 @@ explicit fermion operators.  A dropped term, a wrong sign or a transposed
 convention in the oracle fails one of these comparisons.
 """
 
-import itertools
-from math import sqrt
-
 import numpy as np
 import pytest
 
+from fock import Fock, _fock_operator, ann_sign
 from oracle import oracle
 from oracle.oracle import OracleSector
-
-
-def configs_of(N, h, nd):
-    out = []
-    for holes in itertools.combinations(range(N), h):
-        rest = [i for i in range(N) if i not in holes]
-        for downs in itertools.combinations(rest, nd):
-            st = [2] * N
-            for i in holes:
-                st[i] = 0
-            for i in downs:
-                st[i] = 1
-            out.append(tuple(st))
-    return out
-
-
-def key(st):
-    H = sum(1 << i for i, v in enumerate(st) if v == 0)
-    D = sum(1 << i for i, v in enumerate(st) if v == 1)
-    return (H << 32) | D
-
-
-def perm_site(i, L, m, b):
-    x, y = divmod(i, 2)
-    return 2 * ((x + m) % L) + (y ^ b)
-
-
-def apply_group(st, L, m, b):
-    """U(g)|st> = sign * |g st>, creation operators ordered by descending site."""
-    N = 2 * L
-    occ = [i for i in range(N - 1, -1, -1) if st[i] != 0]
-    img = [perm_site(i, L, m, b) for i in occ]
-    inv = sum(1 for a in range(len(img)) for c in range(a + 1, len(img)) if img[a] < img[c])
-    new = [0] * N
-    for i in range(N):
-        new[perm_site(i, L, m, b)] = st[i]
-    return tuple(new), (-1) ** inv
-
-
-def ann_sign(st, p):
-    """sign of moving c_p to the operator c^dagger_p in the descending string."""
-    return (-1) ** sum(1 for i in range(p + 1, len(st)) if st[i] != 0)
-
-
-class Fock:
-    def __init__(self, L, h, nd):
-        self.L, self.N = L, 2 * L
-        self.states = configs_of(self.N, h, nd)
-        self.index = {s: k for k, s in enumerate(self.states)}
-
-    def hamiltonian(self, w):
-        L, N = self.L, self.N
-        dim = len(self.states)
-        H = np.zeros((dim, dim), np.complex128)
-        leg = [(i, (i + 2) % N) for i in range(N)]
-        rung = [(2 * x, 2 * x + 1) for x in range(L)]
-        for c, st in enumerate(self.states):
-            for bonds, wt in ((leg, w[0]), (rung, w[1])):
-                for a, b in bonds:
-                    for t, s in ((a, b), (b, a)):  # c^dagger_t c_s
-                        if st[t] != 0 or st[s] == 0:
-                            continue
-                        sg = ann_sign(st, s)
-                        mid = list(st)
-                        mid[t], mid[s] = st[s], 0
-                        sg *= ann_sign(mid, t)
-                        H[self.index[tuple(mid)], c] += -wt * sg
-            for bonds, wt in ((leg, w[2]), (rung, w[3])):
-                for a, b in bonds:
-                    if st[a] == 0 or st[b] == 0 or st[a] == st[b]:
-                        continue
-                    H[c, c] += -0.5 * wt
-                    sw = list(st)
-                    sw[a], sw[b] = st[b], st[a]
-                    H[self.index[tuple(sw)], c] += 0.5 * wt
-        return H
-
-    def symmetric_basis(self, kx, ky):
-        L = self.L
-        cols, reps = [], []
-        seen = set()
-        for st in sorted(self.states, key=key):
-            orbit = {apply_group(st, L, m, b)[0] for m in range(L) for b in range(2)}
-            r = min(orbit, key=key)
-            if r in seen:
-                continue
-            seen.add(r)
-            v = np.zeros(len(self.states), np.complex128)
-            for m in range(L):
-                for b in range(2):
-                    img, sg = apply_group(r, L, m, b)
-                    v[self.index[img]] += sg * np.exp(-2j * np.pi * (kx * m / L + ky * b / 2))
-            nrm = np.linalg.norm(v)
-            if nrm > 1e-9:
-                cols.append(v / nrm)
-                reps.append(r)
-        order = np.argsort([key(r) for r in reps])
-        return [reps[k] for k in order], np.stack([cols[k] for k in order], axis=1)
 
 
 SECTORS = [(3, 1, 2, 0, 0), (3, 1, 2, 1, 1), (3, 2, 2, 1, 0), (3, 2, 1, 2, 1), (4, 2, 3, 1, 1), (4, 2, 3, 2, 0),
@@ -136,68 +36,6 @@ def test_basis_and_hamiltonian(sector):
         assert np.max(np.abs(M - Hs)) < 1e-12, (sector, t)
         y = o.hv(w, np.arange(o.dim) + 1j)
         assert np.max(np.abs(y - Hs @ (np.arange(o.dim) + 1j))) < 1e-11
-
-
-def _hop(st, t, p):
-    """c^dagger_t c_p (same spin) on a Fock state: (sign, new state) or None."""
-    if st[t] != 0 or st[p] == 0:
-        return None
-    sg = ann_sign(st, p)
-    mid = list(st)
-    mid[t], mid[p] = st[p], 0
-    return sg * ann_sign(mid, t), tuple(mid)
-
-
-def _fock_operator(fk_i, fk_f, op, qx, qy, mutate=None):
-    """Excitation operators from explicit fermion operators.  qy is the physical momentum transfer across the
-    legs; for the rung current the final sector's ky is shifted by one more (the current is odd under the leg
-    mirror).  `mutate` plants a mistake (a dropped fermion sign, a flipped endpoint sign, a missing i, a
-    rung current without the mirror sign) to show that the comparison catches it."""
-    L, N = fk_i.L, fk_i.N
-    O = np.zeros((len(fk_f.states), len(fk_i.states)), np.complex128)
-    for c, st in enumerate(fk_i.states):
-        for p in range(N):
-            ph = np.exp(-2j * np.pi * (qx * (p // 2) / L + qy * (p % 2) / 2))
-            if op == oracle.OP_SZQ:
-                if st[p] != 0:
-                    O[fk_f.index[st], c] += ph * (0.5 if st[p] == 2 else -0.5) / sqrt(N)
-            elif op == oracle.OP_NQ:
-                # N(q) = N^-1/2 sum_p e^{-iq.r_p} n_p
-                if st[p] != 0:
-                    O[fk_f.index[st], c] += ph / sqrt(N)
-            elif op == oracle.OP_CURRENT_RUNG:
-                # i sum_x e^{-i qx x} (c^dag_{x,1} c_{x,0} - (-1)^qy c^dag_{x,0} c_{x,1})
-                x, y = divmod(p, 2)
-                coef = 1j * np.exp(-2j * np.pi * qx * x / L)
-                if y == 1 and mutate != "rung_mirror":
-                    coef *= -((-1) ** qy)
-                res = _hop(st, 2 * x + (1 - y), p)
-                if res is not None:
-                    O[fk_f.index[res[1]], c] += coef * res[0]
-            elif op == oracle.OP_CURRENT_LEG:
-                # i sum_p e^{-iq.r_p} (c^dag_{p+x} c_p - c^dag_{p-x} c_p), fermion signs from the algebra
-                x, y = divmod(p, 2)
-                for dx, e in ((1, 1.0), (-1, -1.0)):
-                    if mutate == "leg_endpoint" and dx == -1:
-                        e = 1.0
-                    res = _hop(st, 2 * ((x + dx) % L) + y, p)
-                    if res is None:
-                        continue
-                    sg = 1 if mutate == "no_sign" else res[0]
-                    i_factor = 1.0 if mutate == "no_i" else 1j
-                    O[fk_f.index[res[1]], c] += i_factor * ph * e * sg
-            elif op in (oracle.OP_APLUS_DOWN, oracle.OP_APLUS_UP):
-                if st[p] == 0:
-                    new = list(st)
-                    new[p] = 1 if op == oracle.OP_APLUS_DOWN else 2
-                    O[fk_f.index[tuple(new)], c] += ph * ann_sign(st, p) / sqrt(N)
-            elif op in (oracle.OP_AMINUS_DOWN, oracle.OP_AMINUS_UP):
-                want = 1 if op == oracle.OP_AMINUS_DOWN else 2
-                if st[p] == want:
-                    new = list(st)
-                    new[p] = 0
-                    O[fk_f.index[tuple(new)], c] += ph * ann_sign(st, p) / sqrt(N)
-    return O
 
 
 OPS = [((3, 2, 2, 0, 0), oracle.OP_SZQ, (1, 1), (3, 2, 2, 1, 1)),
@@ -368,3 +206,36 @@ def test_position_correlations(sec, kind, fixed):
         Od = _fock_diagonal(fk, kind, [(d // 2, d % 2)] + list(fixed))
         ref = np.real(np.vdot(Psi, Od @ Psi))
         assert abs(got[d] - ref) < 1e-12, (sec, kind, d)
+
+
+# -- the spin flip (no counterpart in the reference; pins the brute-force construction the GPU's spin-flip
+#    sectors are checked against) ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("sector", [(3, 0, 3, 0, 0), (4, 2, 3, 1, 1), (4, 0, 4, 2, 0), (3, 2, 2, 1, 1), (4, 2, 3, 0, 0)])
+def test_spin_flip_blocks_split_the_momentum_sector(sector):
+    """The spin flip commutes with every term of H, its two parity blocks of a momentum sector hold together
+    exactly the sector's states (orthonormal, mutually orthogonal) and their spectra together are the sector's;
+    Szq turns parity z into -z and N(q) and the currents keep it."""
+    from fock import flip_spins
+
+    L, h, nd, kx, ky = sector
+    fk = Fock(L, h, nd)
+    P = np.zeros((len(fk.states), len(fk.states)))
+    for c, st in enumerate(fk.states):
+        P[fk.index[flip_spins(st)], c] = 1.0
+    w = [1.0, 0.7, 0.4, 0.3]
+    Hf = fk.hamiltonian(w)
+    assert np.max(np.abs(P @ Hf @ P.T - Hf)) < 1e-12
+    _, B = fk.symmetric_basis(kx, ky)
+    parts = [fk.symmetric_basis(kx, ky, z)[1] for z in (1, -1)]
+    assert parts[0].shape[1] + parts[1].shape[1] == B.shape[1]
+    Bz = np.concatenate(parts, axis=1)
+    assert np.max(np.abs(Bz.conj().T @ Bz - np.eye(Bz.shape[1]))) < 1e-12
+    assert np.max(np.abs(Bz @ Bz.conj().T - B @ B.conj().T)) < 1e-12  # the same subspace
+    ev = np.linalg.eigvalsh(B.conj().T @ Hf @ B)
+    evz = np.sort(np.concatenate([np.linalg.eigvalsh(Bp.conj().T @ Hf @ Bp) for Bp in parts]))
+    assert np.max(np.abs(ev - evz)) < 1e-10
+    for op, qy_shift, parity in ((oracle.OP_SZQ, 0, -1), (oracle.OP_NQ, 0, 1), (oracle.OP_CURRENT_LEG, 0, 1),
+                                 (oracle.OP_CURRENT_RUNG, 1, 1)):
+        O = _fock_operator(fk, fk, op, 1 % L, 0)
+        assert np.max(np.abs(P @ O @ P.T - parity * O)) < 1e-12, op
