@@ -452,7 +452,27 @@ def forms_section():
         log(f"closure on 2x12: {out['closure']['mean_l1']}, gain {out['closure']['geometric_gain']:.2f}")
     out["closure_gemm"] = gemm_section()
     t0 = time.perf_counter()
-    p = forms.precision_form(log=log)
+    p = forms.precision_form(log=log, keep_spectra=True)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pf = forms.precision_form(log=log, flip=True, keep_spectra=True)
+    t_flip = time.perf_counter() - t0
+    from exactdiag_amd.allq import spectral_l1
+
+    ws = np.linspace(0, 4, 2001)
+    out["precision_spin_flip"] = {
+        "workload": "the same 2x14 sector split by the optional spin-flip symmetry (no counterpart in the "
+                    "reference): ground state in the lower parity block, S(q=(pi,pi),w) in the opposite block",
+        "dims": pf["dims"], "block_dims": pf["flip"]["block_dims"], "ground_state_parity": pf["flip"]["ground_state_parity"],
+        "e0_abs_diff_vs_momentum_sector": abs(pf["c128"]["e0"] - p["c128"]["e0"]),
+        "spectrum_l1_vs_momentum_sector_c128": spectral_l1(pf["spectra"]["c128"], p["spectra"]["c128"], ws),
+        "e0_rel_error_c64": pf["e0_rel_error_c64"], "spectrum_l1_error_c64": pf["spectrum_l1_error_c64"],
+        "seconds": {"c128": pf["c128"]["gs_seconds"] + pf["c128"]["lanczos_seconds"],
+                    "c64": pf["c64"]["gs_seconds"] + pf["c64"]["lanczos_seconds"]},
+        "stored_gb": pf["stored_gb"], "seconds_total": t_flip}
+    log(f"precision with the spin flip: dims {pf['dims']}, |dE0| vs the momentum sector "
+        f"{out['precision_spin_flip']['e0_abs_diff_vs_momentum_sector']:.1e}, S L1 "
+        f"{out['precision_spin_flip']['spectrum_l1_vs_momentum_sector_c128']:.1e}")
     out["precision"] = {
         "workload": "2x14 ladder, 2 holes, Sz=0, k=(pi,0), t=1, J=0.4 (D=140,408,100); S(q=(pi,pi),w), 300 steps",
         "e0": {"c128": p["c128"]["e0"], "c64": p["c64"]["e0"]}, "e0_rel_error_c64": p["e0_rel_error_c64"],
@@ -460,7 +480,7 @@ def forms_section():
         "spectrum_max_rel_error_c64": p["spectrum_max_rel_error_c64"],
         "seconds": {"c128": p["c128"]["gs_seconds"] + p["c128"]["lanczos_seconds"],
                     "c64": p["c64"]["gs_seconds"] + p["c64"]["lanczos_seconds"]},
-        "stored_gb": p["stored_gb"], "seconds_total": time.perf_counter() - t0}
+        "stored_gb": p["stored_gb"], "seconds_total": t_plain}
     return out
 
 

@@ -108,20 +108,39 @@ def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=3
 
 
 def precision_form(L=14, h=2, nd=13, k=(7, 0), q=(7, 1), weights=(1.0, 1.0, 0.4, 0.4), nsteps=300,
-                   ws=np.linspace(0, 4, 2001), eta=0.05, log=print):
-    """complex64 vs complex128: ground-state energy and S(q, w) errors of one sector (stored H)."""
+                   ws=np.linspace(0, 4, 2001), eta=0.05, log=print, flip=False, keep_spectra=False):
+    """complex64 vs complex128: ground-state energy and S(q, w) errors of one sector (stored H).
+
+    flip: the sector split by the spin flip (ed_sector_create_flip): the ground state is the lower of the two
+    parity blocks' (found by a complex128 two-pass run in each), and Szq leads to the opposite parity block of
+    the final momentum; half the states of the momentum sector per block."""
     w = list(weights)
     t0 = time.perf_counter()
-    s0 = Sector(L, h, nd, k[0], k[1])
-    hm0 = s0.store(w)
     kx, ky = final_momentum(L, k[0], k[1], q[0], q[1])
-    s1 = Sector(L, h, nd, kx, ky)
+    out = {}
+    if flip:
+        blocks = {z: Sector(L, h, nd, k[0], k[1], flip=z) for z in (1, -1)}
+        lows = {}
+        for z, sz in blocks.items():
+            hm = sz.store(w)
+            lows[z] = two_pass_ground_state(hm, precision=0, tol=1e-10)[0]
+            hm.close()
+        z0 = 1 if lows[1] <= lows[-1] else -1
+        out["flip"] = {"ground_state_parity": z0, "block_dims": {str(z): blocks[z].dim for z in blocks},
+                       "block_lowest": {str(z): lows[z] for z in lows}}
+        s0 = blocks[z0]
+        blocks[-z0].close()
+        s1 = Sector(L, h, nd, kx, ky, flip=-z0)
+    else:
+        s0 = Sector(L, h, nd, k[0], k[1])
+        s1 = Sector(L, h, nd, kx, ky)
+    hm0 = s0.store(w)
     hm1 = s1.store(w)
     _sync()
     t_setup = time.perf_counter() - t0
-    log(f"precision form: sectors {s0.dim} and {s1.dim} states, stored H {(hm0.bytes + hm1.bytes) / 1e9:.1f} GB, "
-        f"set up in {t_setup:.1f}s")
-    out = {"dims": [s0.dim, s1.dim], "setup_s": t_setup, "stored_gb": (hm0.bytes + hm1.bytes) / 1e9}
+    log(f"precision form{' (spin flip)' if flip else ''}: sectors {s0.dim} and {s1.dim} states, stored H "
+        f"{(hm0.bytes + hm1.bytes) / 1e9:.1f} GB, set up in {t_setup:.1f}s")
+    out.update({"dims": [s0.dim, s1.dim], "setup_s": t_setup, "stored_gb": (hm0.bytes + hm1.bytes) / 1e9})
     spectra = {}
     for name, prec, tol in (("c128", 0, 1e-10), ("c64", 1, 1e-5)):
         t1 = time.perf_counter()
@@ -144,6 +163,8 @@ def precision_form(L=14, h=2, nd=13, k=(7, 0), q=(7, 1), weights=(1.0, 1.0, 0.4,
     out["spectrum_l1_error_c64"] = spectral_l1(spectra["c64"], spectra["c128"], ws)
     out["spectrum_max_rel_error_c64"] = float(np.max(np.abs(spectra["c64"] - spectra["c128"]))
                                               / np.max(np.abs(spectra["c128"])))
+    if keep_spectra:
+        out["spectra"] = spectra
     for obj in (hm1, hm0, s1, s0):
         obj.close()
     return out

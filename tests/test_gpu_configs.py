@@ -217,9 +217,17 @@ def test_c3_batched_eigenpairs_and_spectral_functions(torch):
 
 
 def test_c4_complex64_against_complex128(torch):
+    from exactdiag_amd.allq import spectral_l1
     from exactdiag_amd.forms import precision_form
 
-    out = precision_form(log=print)
+    out = precision_form(log=print, keep_spectra=True)
+    # the same sector split by the optional spin flip: half the states per block, the same ground-state energy
+    # and spectrum (to the fraction's truncation level)
+    fl = precision_form(log=print, flip=True, keep_spectra=True)
+    print(f"C4 spin flip: dims {fl['dims']}, blocks {fl['flip']['block_dims']}, E0 {fl['c128']['e0']:.12f}", flush=True)
+    assert sum(fl["flip"]["block_dims"].values()) == out["dims"][0]
+    assert abs(fl["c128"]["e0"] - out["c128"]["e0"]) < 1e-9 * abs(out["c128"]["e0"])
+    assert spectral_l1(fl["spectra"]["c128"], out["spectra"]["c128"], WS) < 1e-3
     assert out["dims"][0] == 140_408_100
     assert out["c128"]["gs_residual"] <= 1e-10 * abs(out["c128"]["e0"])
     assert out["c64"]["gs_residual"] <= 1e-5 * abs(out["c64"]["e0"])
