@@ -47,6 +47,7 @@ struct SectorView {
     const uint8_t* orbit;         // per dense index: orbit size n
     const double* amp;            // A[(p * (NO + 1) + n) * 2 + {re, im}] = F[p] / sqrt(n), p = 2m + b (+ N)
     const double* sqrt_tab;       // sqrt(n) for n = 0..NO, computed on the host
+    const uint32_t* rank_tab;     // spin ranks by 8-bit chunks (RankGeom of ne, nd)
 };
 
 __host__ __device__ inline uint32_t rotl_n(uint32_t x, int s, int N, uint32_t full) {

@@ -5,10 +5,15 @@
 //  * every table the element computation touches is copied once per
 //    workgroup into LDS, so the inner loop issues LDS reads instead of
 //    scattered vector-L1 loads;
-//  * workgroups are persistent: each walks a contiguous range of row tiles,
-//    so the table copy is amortised and the dot-product partials shrink to
+//  * each workgroup walks a short contiguous range of row tiles (two by
+//    default), so the table copy is shared and the dot-product partials are
 //    one per workgroup; workgroup w is placed so consecutive workgroups on
 //    one XCD (round-robin dispatch over 8 XCDs) take neighbouring ranges;
+//  * a wavefront whose 64 rows share one hole pattern without stabiliser
+//    (nearly all of them: such a block holds every spin configuration) does
+//    the hole-only work once, on the scalar unit, and finds exchange targets
+//    by closed-form rank changes and hop targets by chunked ranks
+//    (hv_row_uniform); other wavefronts take the per-lane walk;
 //  * the gather group size G and the tiles per workgroup are tunable per
 //    sector (ed_hv_tune times the candidates and keeps the fastest).
 
@@ -20,7 +25,211 @@
 namespace ed {
 namespace {
 
-template <class VT, int BS, int G, bool USE_LDS>
+// The down pattern D with the sites of the hole mask Hm taken out (electron
+// order).  Hm is the same in every lane, so the loop runs on the scalar unit.
+__device__ inline uint32_t electron_order(uint32_t D, uint32_t Hm) {
+    while (Hm) {
+        const int p = 31 - __builtin_clz(Hm);
+        const uint32_t lo = (1u << p) - 1u;
+        D = (D & lo) | ((D >> 1) & ~lo);
+        Hm &= lo;
+    }
+    return D;
+}
+
+// Colex rank of an electron-order down pattern from the chunked table
+// (RankGeom): one table read per 8-bit chunk instead of one binomial per
+// down spin.
+__device__ inline int32_t rank_by_chunks(uint32_t c, const uint32_t* rt, const LdsLayout& lay) {
+    uint32_t r = 0;
+    int kb = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < lay.rt_n) {
+            const uint32_t bits = (c >> (8 * k)) & 0xffu;
+            r += rt[lay.rt_base[k] + (kb << lay.rt_width[k]) + int(bits)];
+            kb += popc32(bits);
+        }
+    }
+    return int32_t(r);
+}
+
+// Spin-rank change of an exchange on the bond (i, i + span), span 1 (rung) or
+// 2 (leg, without wrap), both ends electrons with opposite spins: the down
+// spin at electron position c = i - (holes below i), with kd downs below it,
+// moves to the other end (Dn has site i) or comes from it.  Between the ends
+// sits at most one site (span 2): a hole, or an electron whose spin md shifts
+// the second binomial.  Equal to the window rank change of the general walk.
+__device__ inline int32_t exchange_delta(uint32_t H, uint32_t Dn, int i, int span, const uint32_t* comb) {
+    const uint32_t below = (1u << i) - 1u;
+    const int c = i - popc32(H & below);
+    const int kd = popc32(Dn & below);
+    int32_t d = int32_t(comb[kd * COMB_W + c]);
+    if (span == 2 && !((H >> (i + 1)) & 1u)) d += int32_t(comb[(kd + int((Dn >> (i + 1)) & 1u)) * COMB_W + c + 1]);
+    return ((Dn >> i) & 1u) ? d : -d;
+}
+
+// x_j at a 32-bit byte offset (the uniform walk needs D * sizeof(VT) < 2^32):
+// the address is the scalar base plus the lane's offset.
+template <class VT>
+__device__ inline C2 ldx32(const VT* __restrict__ x, uint32_t j) {
+    const VT v = *reinterpret_cast<const VT*>(reinterpret_cast<const char*>(x) + uint64_t(j * uint32_t(sizeof(VT))));
+    return {double(v.x), double(v.y)};
+}
+
+// Sum of x_j over one group's elements, loads issued for the lanes that
+// have an element before any is used (empty slots add zero).
+template <int K, class VT>
+__device__ inline void gather_sum(const VT* __restrict__ x, const uint32_t (&j)[K], const bool (&act)[K], C2& sum) {
+    C2 v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        v[k] = C2{0.0, 0.0};
+        if (act[k]) v[k] = ldx32(x, j[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) sum = cadd(sum, v[k]);
+}
+
+// Exchange elements of one term on the bonds listed in `bonds` (bit i: bond
+// (i, i + span), the same in every lane), G at a time: every element of the
+// term has the same coefficient, so the x_j are summed and multiplied once.
+template <int G, int SPAN, class VT>
+__device__ inline C2 exchange_sum(const VT* __restrict__ x, const Tabs& T, uint32_t H, uint32_t Dn, uint32_t anti,
+                                  uint32_t bonds, uint32_t f) {
+    C2 sum{0.0, 0.0};
+    while (bonds) {
+        uint32_t j[G];
+        bool act[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            j[k] = f;
+            act[k] = false;
+            if (bonds) {
+                const int i = ctz32(bonds);
+                bonds &= bonds - 1;
+                act[k] = (anti >> i) & 1u;
+                j[k] = f + uint32_t(exchange_delta(H, Dn, i, SPAN, T.comb));
+            }
+        }
+        gather_sum<G>(x, j, act, sum);
+    }
+    return sum;
+}
+
+// Row f of the gather form when every lane of the wavefront has the same hole
+// pattern H (a scalar) without stabiliser (block blk_self, orbit size N), in
+// a sector without spin flip: the elements and coefficients of walk_row_t,
+// with the hole-only work (bond sites, hop targets' hole patterns, group
+// elements, signs and coefficients) done once per wavefront, exchange
+// targets by exchange_delta, hop targets' ranks from the chunked table, and
+// each exchange term's x_j summed before its one coefficient is applied.
+template <class VT, int G>
+__device__ inline C2 hv_row_uniform(const SectorView& s, const Tabs& T, const LdsLayout& lay, const HvWeights& w,
+                                    const VT* __restrict__ x, uint32_t f, uint32_t H, uint32_t Dn, int blk_self,
+                                    C2& xf) {
+    const int N = s.N, L = s.L;
+    const uint32_t full = s.full;
+    const uint32_t E = ~H & full, Up = E & ~Dn;
+    GatherSink<VT> sink{x, ldx32(x, f), w, C2{0.0, 0.0}};
+    const double sqf = T.sq[N];
+    const uint32_t base_self = uint32_t(T.bb[blk_self]);
+    if (w.active & 4) {  // jl: bonds (i, i + 2)
+        const uint32_t d2 = rotl_n(Dn, N - 2, N, full), u2 = rotl_n(Up, N - 2, N, full);
+        const uint32_t anti = N == 2 ? 0u : (((Dn & u2) | (Up & d2)) & full);
+        const uint32_t both = N == 2 ? 0u : (E & rotl_n(E, N - 2, N, full));
+        const C2 cg = weighted_conj<false>(s, T, H, 0, N, 0.5 * sqf, w.w[2]);
+        sink.diag(2, -0.5 * double(popc32(anti)));
+        // bonds i <= N - 3 by the window rank change, the two that wrap round by the full rank
+        const uint32_t inner = both & (full >> 2);
+        C2 sum = exchange_sum<G, 2>(x, T, H, Dn, anti, inner, f);
+        uint32_t j[2];
+        bool act[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = N - 2 + k, kk = k;
+            j[k] = f;
+            act[k] = false;
+            if ((both >> i) & 1u) {
+                act[k] = (anti >> i) & 1u;
+                const uint32_t D2 = act[k] ? Dn ^ (1u << i) ^ (1u << kk) : Dn;
+                j[k] = base_self + uint32_t(rank_by_chunks(electron_order(D2, H), T.rt, lay));
+            }
+        }
+        gather_sum<2>(x, j, act, sum);
+        sink.acc = cadd(sink.acc, cmul(cg, sum));
+    }
+    if (w.active & 8) {  // jr: bonds (2x, 2x + 1)
+        const uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
+        const uint32_t both = E & (E >> 1) & EVEN_BITS & full;
+        const C2 cg = weighted_conj<false>(s, T, H, 0, N, 0.5 * sqf, w.w[3]);
+        sink.diag(3, -0.5 * double(popc32(anti)));
+        const C2 sum = exchange_sum<G, 1>(x, T, H, Dn, anti, both, f);
+        sink.acc = cadd(sink.acc, cmul(cg, sum));
+    }
+    if (w.active & 3) {  // hops onto each hole: leg neighbours (tl) and rung partner (tr)
+        uint32_t holes = H;
+        while (holes) {
+            const int t = ctz32(holes);
+            holes &= holes - 1;
+            uint32_t j[3];
+            C2 c[3];
+            bool act[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                j[k] = f;
+                c[k] = C2{0.0, 0.0};
+                act[k] = false;
+                if (k < 2 && !(w.active & 1)) continue;
+                if (k == 2 && !(w.active & 2)) continue;
+                const int src = k == 0 ? (t + 2 < N ? t + 2 : t + 2 - N) : k == 1 ? (t >= 2 ? t - 2 : t - 2 + N)
+                                                                                   : (t ^ 1);
+                if (!(E & (1u << src))) continue;
+                const C2 W = k < 2 ? w.w[0] : w.w[1];
+                const uint32_t bt = 1u << t, bs = 1u << src;
+                const uint32_t H2 = H ^ bt ^ bs;
+                const uint32_t D2 = (Dn & bs) ? (Dn ^ bs ^ bt) : Dn;
+                const uint32_t above_s = src >= 31 ? 0u : (~0u << (src + 1));
+                const uint32_t above_t = t >= 31 ? 0u : (~0u << (t + 1));
+                const int par = (popc32(E & above_s) + popc32(E & above_t) + (t < src ? 1 : 0)) & 1;
+                const int32_t hr2 = rank_holes32(H2, T.comb);
+                const int blk = T.cb[hr2];
+                if (blk < 0) continue;
+                const uint32_t gm = T.gm[hr2];
+                if ((gm & (gm - 1u)) == 0u && T.bt[blk] < 0) {
+                    // target pattern without stabiliser: one group element for every lane
+                    const int p = ctz32(gm);
+                    const uint32_t Hc = image_mask(H2, p, N, full);
+                    j[k] = uint32_t(T.bb[blk]) +
+                           uint32_t(rank_by_chunks(electron_order(image_mask(D2, p, N, full), Hc), T.rt, lay));
+                    c[k] = weighted_conj<false>(s, T, H2, p, N, par ? sqf : -sqf, W);
+                    act[k] = true;
+                } else {
+                    int p, nj;
+                    int64_t jj;
+                    if (locate<false>(s, T, H2, D2, gm, blk, p, jj, nj)) {
+                        j[k] = uint32_t(jj);
+                        c[k] = weighted_conj<false>(s, T, H2, p, nj, par ? sqf : -sqf, W);
+                        act[k] = true;
+                    }
+                }
+            }
+            C2 v[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v[k] = C2{0.0, 0.0};
+                if (act[k]) v[k] = ldx32(x, j[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (act[k]) sink.acc = cadd(sink.acc, cmul(c[k], v[k]));
+        }
+    }
+    xf = sink.xf;
+    return sink.acc;
+}
+
+template <class VT, int BS, int G, bool USE_LDS, bool UNIFORM>
 __global__ void __launch_bounds__(BS) hv_lds_kernel(SectorView s, HvWeights w, const VT* __restrict__ x,
                                                     VT* __restrict__ y, double* __restrict__ partial, LdsLayout lay,
                                                     int64_t ntiles, int64_t tiles_per_wg, int chunk) {
@@ -35,6 +244,33 @@ __global__ void __launch_bounds__(BS) hv_lds_kernel(SectorView s, HvWeights w, c
     C2 dot{0.0, 0.0};
     for (int64_t tile = t0; tile < t1; ++tile) {
         const int64_t f = tile * BS + threadIdx.x;
+        if constexpr (UNIFORM) {
+            // the wavefront's rows share their hole pattern except where a block ends
+            const int64_t f0 = tile * BS + int(threadIdx.x & ~63u);
+            if (f0 >= s.D) continue;
+            const bool live = f < s.D;
+            const uint64_t key = s.key[live ? f : f0];
+            const uint32_t Hl = uint32_t(key >> 32);
+            const uint32_t H = __builtin_amdgcn_readfirstlane(Hl);
+            if (__all(Hl == H)) {
+                const int32_t hr = rank_holes32(H, T.comb);
+                const uint32_t gm = T.gm[hr];
+                const int blk = T.cb[hr];
+                if ((gm & (gm - 1u)) == 0u && T.bt[blk] < 0) {
+                    C2 xf;
+                    const C2 acc =
+                        hv_row_uniform<VT, G>(s, T, lay, w, x, uint32_t(live ? f : f0), H, uint32_t(key), blk, xf);
+                    if (live) {
+                        VT o;
+                        o.x = decltype(o.x)(acc.re);
+                        o.y = decltype(o.y)(acc.im);
+                        y[f] = o;
+                        dot = cadd(dot, cmul(C2{xf.re, -xf.im}, acc));
+                    }
+                    continue;
+                }
+            }
+        }
         if (f >= s.D) continue;
         C2 xf;
         const C2 acc = hv_row<VT, G>(s, T, w, x, f, xf);
@@ -54,44 +290,56 @@ __global__ void __launch_bounds__(BS) hv_lds_kernel(SectorView s, HvWeights w, c
 }
 
 template <class VT>
-void launch_g(int G, int64_t nwg, const LdsLayout& lay, hipStream_t st, const SectorView& v, const HvWeights& w,
-              const void* x, void* y, double* partial, int64_t ntiles, int64_t tpw) {
+using HvKernel = void (*)(SectorView, HvWeights, const VT*, VT*, double*, LdsLayout, int64_t, int64_t, int);
+
+template <class VT>
+HvKernel<VT> pick_kernel(int G, const LdsLayout& lay) {
     constexpr int BS = 256;
+    if (lay.bytes > LDS_CAP) return hv_lds_kernel<VT, BS, 4, false, false>;  // spin flip: tables in global memory
+    if (lay.rt >= 0) {
+        if (G == 2) return hv_lds_kernel<VT, BS, 2, true, true>;
+        if (G == 8) return hv_lds_kernel<VT, BS, 8, true, true>;
+        return hv_lds_kernel<VT, BS, 4, true, true>;
+    }
+    if (G == 2) return hv_lds_kernel<VT, BS, 2, true, false>;
+    if (G == 8) return hv_lds_kernel<VT, BS, 8, true, false>;
+    return hv_lds_kernel<VT, BS, 4, true, false>;
+}
+
+// tiles_per_wg = 0: two row tiles per workgroup.  Many short workgroups let
+// the dispatcher balance rows of unequal cost (on the 2x12 ladder 0.78 ms
+// per complex128 product, against 1.35 ms with one round of resident
+// workgroups walking 25 tiles each).  Returns the number of workgroups
+// (partials).
+template <class VT>
+int64_t launch_g(int G, int64_t tiles_per_wg, const LdsLayout& lay, hipStream_t st, const SectorView& v,
+                 const HvWeights& w, const void* x, void* y, double* partial, int64_t ntiles) {
+    constexpr int BS = 256;
+    const HvKernel<VT> k = pick_kernel<VT>(G, lay);
+    const size_t bytes = lay.bytes > LDS_CAP ? 0 : size_t(lay.bytes);
+    const int64_t tpw = tiles_per_wg > 0 ? tiles_per_wg : 2;
+    const int64_t nwg = (ntiles + tpw - 1) / tpw;
     // XCD chunks of about 131,072 rows (ED_HV_XCD_CHUNK_ROWS overrides, 0 = one eighth per XCD)
     static const int64_t chunk_rows = [] {
         const char* e = std::getenv("ED_HV_XCD_CHUNK_ROWS");
         return e ? int64_t(std::atoll(e)) : int64_t(131072);
     }();
     const int chunk = chunk_rows > 0 ? int(std::max<int64_t>(1, chunk_rows / (tpw * BS))) : 0;
-    const VT* xx = static_cast<const VT*>(x);
-    VT* yy = static_cast<VT*>(y);
-    const dim3 grid{unsigned(nwg), 1u, 1u};
-    auto go = [&](auto kernel, size_t bytes) {
-        hipLaunchKernelGGL(kernel, grid, dim3(BS), bytes, st, v, w, xx, yy, partial, lay, ntiles, tpw, chunk);
-    };
-    if (lay.bytes > LDS_CAP)  // only sectors with the spin flip get here: tables from global memory
-        go(hv_lds_kernel<VT, BS, 4, false>, 0);
-    else if (G == 2)
-        go(hv_lds_kernel<VT, BS, 2, true>, size_t(lay.bytes));
-    else if (G == 8)
-        go(hv_lds_kernel<VT, BS, 8, true>, size_t(lay.bytes));
-    else
-        go(hv_lds_kernel<VT, BS, 4, true>, size_t(lay.bytes));
+    hipLaunchKernelGGL(k, dim3(unsigned(nwg)), dim3(BS), bytes, st, v, w, static_cast<const VT*>(x),
+                       static_cast<VT*>(y), partial, lay, ntiles, tpw, chunk);
+    return nwg;
+}
+
+// The wavefront-uniform walk unless ED_HV_UNIFORM=0.
+bool hv_uniform_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("ED_HV_UNIFORM");
+        return !(e && e[0] == '0');
+    }();
+    return on;
 }
 
 }  // namespace
-
-int hv_default_tiles(const ed_sector* s) {
-    constexpr int BS = 256;
-    const int64_t ntiles = (s->D + BS - 1) / BS;
-    int dev = 0;
-    ED_HIP(hipGetDevice(&dev));
-    static int cus = 0;
-    if (cus == 0) ED_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    // enough workgroups to fill every CU several times over, each with >= 4 tiles
-    const int64_t target = int64_t(cus) * 8;
-    return int(std::max<int64_t>(4, (ntiles + target - 1) / target));
-}
 
 // Persistent LDS-table H·v; returns the number of partials written, or -1
 // when the sector's tables do not fit in LDS (caller falls back).  Sectors
@@ -100,17 +348,14 @@ int hv_default_tiles(const ed_sector* s) {
 int64_t launch_hv_lds(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
                       hipStream_t st) {
     constexpr int BS = 256;
-    const LdsLayout lay = lds_layout(s);
+    const LdsLayout lay = hv_uniform_enabled() ? lds_layout_ranks(s) : lds_layout(s);
     if (lay.bytes > LDS_CAP && !s->flip) return -1;
     const int64_t ntiles = (s->D + BS - 1) / BS;
     if (ntiles == 0) return 0;
-    const int64_t tpw = s->tune_tiles > 0 ? s->tune_tiles : hv_default_tiles(s);
     const int G = s->tune_gather > 0 ? s->tune_gather : 4;
-    const int64_t nwg = (ntiles + tpw - 1) / tpw;
-    if (precision == 0)
-        launch_g<double2>(G, nwg, lay, st, s->view(), w, x, y, partial, ntiles, tpw);
-    else
-        launch_g<float2>(G, nwg, lay, st, s->view(), w, x, y, partial, ntiles, tpw);
+    const int64_t nwg = precision == 0
+                            ? launch_g<double2>(G, s->tune_tiles, lay, st, s->view(), w, x, y, partial, ntiles)
+                            : launch_g<float2>(G, s->tune_tiles, lay, st, s->view(), w, x, y, partial, ntiles);
     ED_HIP(hipGetLastError());
     return nwg;
 }

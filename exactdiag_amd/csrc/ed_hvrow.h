@@ -19,8 +19,14 @@
 
 namespace ed {
 
+// Byte offsets of the staged tables; rt >= 0 adds the chunked spin-rank
+// table (RankGeom) for the wavefront-uniform row walk.
 struct LdsLayout {
     int comb, amp, sq, gm, cb, bb, bt, bytes;
+    int rt = -1;
+    int rt_n = 0, rt_entries = 0;
+    int rt_base[4] = {0, 0, 0, 0};
+    int rt_width[4] = {0, 0, 0, 0};
 };
 
 inline LdsLayout lds_layout(const ed_sector* s) {
@@ -45,6 +51,27 @@ inline LdsLayout lds_layout(const ed_sector* s) {
 
 constexpr int LDS_CAP = 60 * 1024;
 
+// lds_layout with the chunked spin-rank table appended when the sector can
+// take the wavefront-uniform walk (no spin flip, one momentum, vectors under
+// 4 GiB) and all of it fits in LDS_CAP; else rt = -1.
+inline LdsLayout lds_layout_ranks(const ed_sector* s) {
+    LdsLayout l = lds_layout(s);
+    const RankGeom g = rank_geom(s->ne, s->nd);
+    const int bytes = (g.entries * 4 + 15) & ~15;
+    // 32-bit byte offsets of x_j for either vector type
+    const bool small = s->D * 16 < (int64_t(1) << 32);
+    if (s->flip || s->any_k || !small || s->rank_tab.empty() || l.bytes + bytes > LDS_CAP) return l;
+    l.rt = l.bytes;
+    l.bytes += bytes;
+    l.rt_n = g.n;
+    l.rt_entries = g.entries;
+    for (int c = 0; c < 4; ++c) {
+        l.rt_base[c] = g.base[c];
+        l.rt_width[c] = g.width[c];
+    }
+    return l;
+}
+
 struct Tabs {
     const uint32_t* comb;
     const double* amp;
@@ -53,10 +80,11 @@ struct Tabs {
     const int32_t* cb;
     const int64_t* bb;
     const int64_t* bt;
+    const uint32_t* rt;  // chunked spin ranks (LdsLayout::rt >= 0 only)
 };
 
 __device__ inline Tabs global_tabs(const SectorView& s) {
-    return Tabs{s.comb, s.amp, s.sqrt_tab, s.canon_gmask, s.canon_block, s.block_base, s.block_table};
+    return Tabs{s.comb, s.amp, s.sqrt_tab, s.canon_gmask, s.canon_block, s.block_base, s.block_table, s.rank_tab};
 }
 
 // Copy the tables into LDS (all threads of the workgroup take part; the
@@ -83,7 +111,12 @@ __device__ inline Tabs stage_tabs(const SectorView& s, const LdsLayout& lay, uns
         s_bb[i] = s.block_base[i];
         s_bt[i] = s.block_table[i];
     }
-    return Tabs{s_comb, s_amp, s_sq, s_gm, s_cb, s_bb, s_bt};
+    uint32_t* s_rt = nullptr;
+    if (lay.rt >= 0) {
+        s_rt = reinterpret_cast<uint32_t*>(lds + lay.rt);
+        for (int i = t; i < lay.rt_entries; i += BS) s_rt[i] = s.rank_tab[i];
+    }
+    return Tabs{s_comb, s_amp, s_sq, s_gm, s_cb, s_bb, s_bt, s_rt};
 }
 
 __device__ inline int32_t rank_holes32(uint32_t H, const uint32_t* comb) {

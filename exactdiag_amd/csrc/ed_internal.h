@@ -58,6 +58,31 @@ __device__ inline void sum_partials(const double* partial, int64_t n, double& re
     im = (i[0] + i[1]) + (i[2] + i[3]);
 }
 
+// Colex rank of an ne-bit pattern with nd ones (the spin rank of a down
+// pattern in electron order, holes removed) by 8-bit chunks: chunk c (bits
+// 8c .. 8c + width[c] - 1) with kb ones below it adds
+// tab[base[c] + (kb << width[c]) + chunk bits], the sum over its set bits
+// (the j-th of the chunk at bit i) of C(8c + i, kb + j + 1); kb runs to
+// min(8c, nd).  entries = size of the table.
+struct RankGeom {
+    int n = 0;
+    int base[4] = {0, 0, 0, 0};
+    int width[4] = {0, 0, 0, 0};
+    int entries = 0;
+};
+
+inline RankGeom rank_geom(int ne, int nd) {
+    RankGeom g;
+    g.n = (ne + 7) / 8;
+    for (int c = 0; c < g.n; ++c) {
+        g.width[c] = ne - 8 * c < 8 ? ne - 8 * c : 8;
+        g.base[c] = g.entries;
+        const int kmax = 8 * c < nd ? 8 * c : nd;
+        g.entries += (kmax + 1) << g.width[c];
+    }
+    return g;
+}
+
 void set_error(const std::string& msg);
 
 struct Fail {
@@ -175,6 +200,7 @@ struct ed_sector {
     std::vector<uint32_t> comb;        // COMB_W * COMB_W
     std::vector<double> amp;           // see SectorView::amp
     std::vector<double> sqrt_tab;      // NO + 1 (SectorView::NO)
+    std::vector<uint32_t> rank_tab;    // spin ranks by chunks (ed::RankGeom of ne, nd)
     std::vector<uint32_t> hole_by_rank;
     std::vector<uint32_t> canon_gmask;
     std::vector<int32_t> canon_block, block_of_rank;
@@ -189,6 +215,7 @@ struct ed_sector {
     uint32_t* d_comb = nullptr;
     double* d_amp = nullptr;
     double* d_sqrt = nullptr;
+    uint32_t* d_rank_tab = nullptr;
     uint32_t* d_canon_gmask = nullptr;
     int32_t* d_canon_block = nullptr;
     int32_t* d_block_of_rank = nullptr;

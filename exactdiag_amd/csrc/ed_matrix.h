@@ -18,7 +18,6 @@ int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y
 int64_t launch_hv_lds(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
                       hipStream_t st);
 int64_t hv_partials(const ed_sector* s);
-int hv_default_tiles(const ed_sector* s);
 int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st);
 const ed_sector* hmat_sector(const ed_hmat* m);
 bool hmat_is_real(const ed_hmat* m);

@@ -381,6 +381,7 @@ ed::SectorView ed_sector::host_view() const {
     v.comb = comb.data();
     v.amp = amp.data();
     v.sqrt_tab = sqrt_tab.data();
+    v.rank_tab = rank_tab.data();
     return v;
 }
 
@@ -398,16 +399,18 @@ ed::SectorView ed_sector::view() const {
     v.orbit = d_orbit;
     v.amp = d_amp;
     v.sqrt_tab = d_sqrt;
+    v.rank_tab = d_rank_tab;
     return v;
 }
 
 void ed_sector::release() {
     for (void* p : {(void*)d_comb, (void*)d_amp, (void*)d_sqrt, (void*)d_canon_gmask, (void*)d_canon_block,
                     (void*)d_block_of_rank, (void*)d_block_base, (void*)d_block_table, (void*)d_block_hole,
-                    (void*)d_table, (void*)d_key, (void*)d_orbit, (void*)d_partial})
+                    (void*)d_table, (void*)d_key, (void*)d_orbit, (void*)d_partial, (void*)d_rank_tab})
         dev_free(p);
     d_comb = nullptr;
     d_amp = d_sqrt = nullptr;
+    d_rank_tab = nullptr;
     d_canon_gmask = nullptr;
     d_canon_block = d_block_of_rank = nullptr;
     d_block_base = d_block_table = nullptr;
@@ -475,6 +478,24 @@ int sector_create(int L, int h, int nd, int kx, int ky, int any_k, int flip, ed_
         s->amp.assign(size_t(G) * (NO + 1) * 2, 0.0);
         s->sqrt_tab.assign(NO + 1, 0.0);
         for (int n = 0; n <= NO; ++n) s->sqrt_tab[n] = std::sqrt(double(n));
+        {
+            const RankGeom g = rank_geom(s->ne, nd);
+            s->rank_tab.assign(size_t(g.entries), 0u);
+            for (int c = 0; c < g.n; ++c) {
+                const int kmax = std::min(8 * c, nd);
+                for (int kb = 0; kb <= kmax; ++kb)
+                    for (uint32_t bits = 0; bits < (1u << g.width[c]); ++bits) {
+                        uint32_t r = 0;
+                        int j = 0;
+                        for (int i = 0; i < g.width[c]; ++i)
+                            if ((bits >> i) & 1u) {
+                                const int k = kb + ++j;
+                                r += k <= COMB_W - 1 ? s->comb[k * COMB_W + 8 * c + i] : 0u;
+                            }
+                        s->rank_tab[size_t(g.base[c] + (kb << g.width[c]) + int(bits))] = r;
+                    }
+            }
+        }
         for (int f = 0; f < (flip ? 2 : 1); ++f)
             for (int m = 0; m < L; ++m)
                 for (int b = 0; b < 2; ++b) {
@@ -578,6 +599,7 @@ int ed_basis_build(ed_sector* s, void* stream) {
         s->d_comb = dev_upload(s->comb, st);
         s->d_amp = dev_upload(s->amp, st);
         s->d_sqrt = dev_upload(s->sqrt_tab, st);
+        s->d_rank_tab = dev_upload(s->rank_tab, st);
         s->d_canon_gmask = dev_upload(s->canon_gmask, st);
         s->d_canon_block = dev_upload(s->canon_block, st);
         s->d_block_of_rank = dev_upload(s->block_of_rank, st);
