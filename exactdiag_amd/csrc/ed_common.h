@@ -48,6 +48,7 @@ struct SectorView {
     const double* amp;            // A[(p * (NO + 1) + n) * 2 + {re, im}] = F[p] / sqrt(n), p = 2m + b (+ N)
     const double* sqrt_tab;       // sqrt(n) for n = 0..NO, computed on the host
     const uint32_t* rank_tab;     // spin ranks by 8-bit chunks (RankGeom of ne, nd)
+    const uint8_t* table_orbit;   // per table entry: orbit size of its state, 0 where table holds -1
 };
 
 __host__ __device__ inline uint32_t rotl_n(uint32_t x, int s, int N, uint32_t full) {

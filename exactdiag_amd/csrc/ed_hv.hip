@@ -9,11 +9,11 @@
 //    default), so the table copy is shared and the dot-product partials are
 //    one per workgroup; workgroup w is placed so consecutive workgroups on
 //    one XCD (round-robin dispatch over 8 XCDs) take neighbouring ranges;
-//  * a wavefront whose 64 rows share one hole pattern without stabiliser
-//    (nearly all of them: such a block holds every spin configuration) does
-//    the hole-only work once, on the scalar unit, and finds exchange targets
-//    by closed-form rank changes and hop targets by chunked ranks
-//    (hv_row_uniform); other wavefronts take the per-lane walk;
+//  * a wavefront whose 64 rows share one hole pattern (all but those where a
+//    block ends) does the hole-only work once, on the scalar unit, finds
+//    hop targets by chunked ranks and, in blocks without stabiliser,
+//    exchange targets by closed-form rank changes (hv_row_uniform); the
+//    other wavefronts take the per-lane walk;
 //  * the gather group size G and the tiles per workgroup are tunable per
 //    sector (ed_hv_tune times the candidates and keeps the fastest).
 
@@ -117,24 +117,109 @@ __device__ inline C2 exchange_sum(const VT* __restrict__ x, const Tabs& T, uint3
     return sum;
 }
 
+// Exchange elements of one term in a block whose hole pattern H has a
+// stabiliser (gm: its group elements, identity included; the block's spin
+// ranks map to rows through the table at `off`): the target (H, D2) is
+// brought to its lowest image over gm (image_mask(H, p) = H for those p,
+// the first minimum wins as in lowest_downs), ranked, looked up with its
+// orbit size, and weighted per lane.  Table reads of a group are issued
+// together, then its gathers.  SPAN 2 includes the bonds that wrap round.
+template <int G, int SPAN, class VT>
+__device__ inline void exchange_sym(const SectorView& s, const VT* __restrict__ x, const Tabs& T,
+                                    const LdsLayout& lay, uint32_t H, uint32_t Dn, uint32_t anti, uint32_t bonds,
+                                    uint32_t gm, int64_t off, double half_sqf, C2 W, C2& acc) {
+    const int N = s.N;
+    const uint32_t full = s.full;
+    while (bonds) {
+        uint32_t idx[G];
+        int bp[G];
+        bool act[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            idx[k] = 0;
+            bp[k] = 0;
+            act[k] = false;
+            if (bonds) {
+                const int i = ctz32(bonds);
+                bonds &= bonds - 1;
+                const int kk = i + SPAN < N ? i + SPAN : i + SPAN - N;
+                act[k] = (anti >> i) & 1u;
+                const uint32_t D2 = Dn ^ (1u << i) ^ (1u << kk);
+                uint32_t best = 0;
+                uint32_t g = gm;
+                bool first = true;
+                while (g) {
+                    const int p = ctz32(g);
+                    g &= g - 1;
+                    const uint32_t d = image_mask(D2, p, N, full);
+                    if (first || d < best) {
+                        best = d;
+                        bp[k] = p;
+                    }
+                    first = false;
+                }
+                idx[k] = uint32_t(off) + uint32_t(rank_by_chunks(electron_order(best, H), T.rt, lay));
+            }
+        }
+        int32_t j[G];
+        int nj[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            j[k] = -1;
+            nj[k] = 0;
+            if (act[k]) {
+                j[k] = s.table[idx[k]];
+                nj[k] = s.table_orbit[idx[k]];
+            }
+        }
+        C2 v[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            act[k] = act[k] && j[k] >= 0;
+            v[k] = C2{0.0, 0.0};
+            if (act[k]) v[k] = ldx32(x, uint32_t(j[k]));
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (act[k]) acc = cadd(acc, cmul(weighted_conj<false>(s, T, H, bp[k], nj[k], half_sqf, W), v[k]));
+    }
+}
+
 // Row f of the gather form when every lane of the wavefront has the same hole
-// pattern H (a scalar) without stabiliser (block blk_self, orbit size N), in
-// a sector without spin flip: the elements and coefficients of walk_row_t,
-// with the hole-only work (bond sites, hop targets' hole patterns, group
-// elements, signs and coefficients) done once per wavefront, exchange
-// targets by exchange_delta, hop targets' ranks from the chunked table, and
-// each exchange term's x_j summed before its one coefficient is applied.
-template <class VT, int G>
+// pattern H (a scalar), in a sector without spin flip: the elements and
+// coefficients of walk_row_t, with the hole-only work (bond sites, hop
+// targets' hole patterns, group elements, signs and coefficients) done once
+// per wavefront and hop targets' ranks from the chunked table.  Without
+// stabiliser (SYM false: block blk_self holds every spin configuration,
+// orbit size N) exchange targets come from exchange_delta and each exchange
+// term's x_j are summed before its one coefficient is applied; with one
+// (SYM: stabiliser elements gm_self, rows through the block's table)
+// exchange_sym finds and weights them per lane.
+template <class VT, int G, bool SYM>
 __device__ inline C2 hv_row_uniform(const SectorView& s, const Tabs& T, const LdsLayout& lay, const HvWeights& w,
                                     const VT* __restrict__ x, uint32_t f, uint32_t H, uint32_t Dn, int blk_self,
-                                    C2& xf) {
+                                    uint32_t gm_self, C2& xf) {
     const int N = s.N, L = s.L;
     const uint32_t full = s.full;
     const uint32_t E = ~H & full, Up = E & ~Dn;
     GatherSink<VT> sink{x, ldx32(x, f), w, C2{0.0, 0.0}};
-    const double sqf = T.sq[N];
+    const double sqf = SYM ? T.sq[s.orbit[f]] : T.sq[N];
     const uint32_t base_self = uint32_t(T.bb[blk_self]);
-    if (w.active & 4) {  // jl: bonds (i, i + 2)
+    const int64_t off_self = T.bt[blk_self];
+    if (SYM && (w.active & 4)) {  // jl, every bond by the stabiliser search
+        const uint32_t d2 = rotl_n(Dn, N - 2, N, full), u2 = rotl_n(Up, N - 2, N, full);
+        const uint32_t anti = N == 2 ? 0u : (((Dn & u2) | (Up & d2)) & full);
+        const uint32_t both = N == 2 ? 0u : (E & rotl_n(E, N - 2, N, full));
+        sink.diag(2, -0.5 * double(popc32(anti)));
+        exchange_sym<G, 2>(s, x, T, lay, H, Dn, anti, both, gm_self, off_self, 0.5 * sqf, w.w[2], sink.acc);
+    }
+    if (SYM && (w.active & 8)) {  // jr
+        const uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
+        const uint32_t both = E & (E >> 1) & EVEN_BITS & full;
+        sink.diag(3, -0.5 * double(popc32(anti)));
+        exchange_sym<G, 1>(s, x, T, lay, H, Dn, anti, both, gm_self, off_self, 0.5 * sqf, w.w[3], sink.acc);
+    }
+    if (!SYM && (w.active & 4)) {  // jl: bonds (i, i + 2)
         const uint32_t d2 = rotl_n(Dn, N - 2, N, full), u2 = rotl_n(Up, N - 2, N, full);
         const uint32_t anti = N == 2 ? 0u : (((Dn & u2) | (Up & d2)) & full);
         const uint32_t both = N == 2 ? 0u : (E & rotl_n(E, N - 2, N, full));
@@ -159,7 +244,7 @@ __device__ inline C2 hv_row_uniform(const SectorView& s, const Tabs& T, const Ld
         gather_sum<2>(x, j, act, sum);
         sink.acc = cadd(sink.acc, cmul(cg, sum));
     }
-    if (w.active & 8) {  // jr: bonds (2x, 2x + 1)
+    if (!SYM && (w.active & 8)) {  // jr: bonds (2x, 2x + 1)
         const uint32_t anti = (Dn ^ (Dn >> 1)) & E & (E >> 1) & EVEN_BITS & full;
         const uint32_t both = E & (E >> 1) & EVEN_BITS & full;
         const C2 cg = weighted_conj<false>(s, T, H, 0, N, 0.5 * sqf, w.w[3]);
@@ -256,10 +341,14 @@ __global__ void __launch_bounds__(BS) hv_lds_kernel(SectorView s, HvWeights w, c
                 const int32_t hr = rank_holes32(H, T.comb);
                 const uint32_t gm = T.gm[hr];
                 const int blk = T.cb[hr];
-                if ((gm & (gm - 1u)) == 0u && T.bt[blk] < 0) {
+                const bool generic = (gm & (gm - 1u)) == 0u && T.bt[blk] < 0;
+                if (generic || T.bt[blk] >= 0) {
                     C2 xf;
-                    const C2 acc =
-                        hv_row_uniform<VT, G>(s, T, lay, w, x, uint32_t(live ? f : f0), H, uint32_t(key), blk, xf);
+                    const uint32_t fe = uint32_t(live ? f : f0);
+                    const C2 acc = generic ? hv_row_uniform<VT, G, false>(s, T, lay, w, x, fe, H, uint32_t(key), blk,
+                                                                          gm, xf)
+                                           : hv_row_uniform<VT, G, true>(s, T, lay, w, x, fe, H, uint32_t(key), blk,
+                                                                         gm, xf);
                     if (live) {
                         VT o;
                         o.x = decltype(o.x)(acc.re);

@@ -223,6 +223,7 @@ struct ed_sector {
     int64_t* d_block_table = nullptr;
     uint32_t* d_block_hole = nullptr;
     int32_t* d_table = nullptr;
+    uint8_t* d_table_orbit = nullptr;  // orbit size of each table entry's state (0: none)
     uint64_t* d_key = nullptr;
     uint8_t* d_orbit = nullptr;
     double* d_partial = nullptr;  // per-workgroup partials of the fused H·v dot (calls on one sector serialise on it)

@@ -107,6 +107,15 @@ __global__ void __launch_bounds__(SCAN_THREADS) basis_fill_kernel(SectorView s, 
     }
 }
 
+// Orbit size beside each table entry (0 where the spin rank is no basis
+// state), so that a lookup through the table needs no second, dependent load.
+__global__ void table_orbit_kernel(const int32_t* table, const uint8_t* orbit, uint8_t* out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = table[i];
+    out[i] = j >= 0 ? orbit[j] : uint8_t(0);
+}
+
 __global__ void basis_export_kernel(SectorView s, int64_t* sparse, int32_t* orb) {
     const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (f >= s.D) return;
@@ -400,17 +409,19 @@ ed::SectorView ed_sector::view() const {
     v.amp = d_amp;
     v.sqrt_tab = d_sqrt;
     v.rank_tab = d_rank_tab;
+    v.table_orbit = d_table_orbit;
     return v;
 }
 
 void ed_sector::release() {
     for (void* p : {(void*)d_comb, (void*)d_amp, (void*)d_sqrt, (void*)d_canon_gmask, (void*)d_canon_block,
                     (void*)d_block_of_rank, (void*)d_block_base, (void*)d_block_table, (void*)d_block_hole,
-                    (void*)d_table, (void*)d_key, (void*)d_orbit, (void*)d_partial, (void*)d_rank_tab})
+                    (void*)d_table, (void*)d_key, (void*)d_orbit, (void*)d_partial, (void*)d_rank_tab, (void*)d_table_orbit})
         dev_free(p);
     d_comb = nullptr;
     d_amp = d_sqrt = nullptr;
     d_rank_tab = nullptr;
+    d_table_orbit = nullptr;
     d_canon_gmask = nullptr;
     d_canon_block = d_block_of_rank = nullptr;
     d_block_base = d_block_table = nullptr;
@@ -613,6 +624,7 @@ int ed_basis_build(ed_sector* s, void* stream) {
             s->d_key = dev_alloc<uint64_t>(1);
             s->d_orbit = dev_alloc<uint8_t>(1);
             s->d_table = dev_alloc<int32_t>(1);
+            s->d_table_orbit = dev_alloc<uint8_t>(1);
             ED_HIP(hipStreamSynchronize(st));
             return;
         }
@@ -657,6 +669,12 @@ int ed_basis_build(ed_sector* s, void* stream) {
         hipLaunchKernelGGL(basis_fill_kernel, grid, dim3(SCAN_THREADS), 0, st, v, flags, d_off, cpb, s->d_key,
                            s->d_orbit, s->d_table);
         ED_HIP(hipGetLastError());
+        s->d_table_orbit = dev_alloc<uint8_t>(size_t(std::max<int64_t>(s->table_size, 1)));
+        if (s->table_size > 0) {
+            hipLaunchKernelGGL(table_orbit_kernel, dim3(unsigned((s->table_size + 255) / 256)), dim3(256), 0, st,
+                               s->d_table, s->d_orbit, s->d_table_orbit, s->table_size);
+            ED_HIP(hipGetLastError());
+        }
         ED_HIP(hipStreamSynchronize(st));
         dev_free(flags);
         dev_free(counts);
