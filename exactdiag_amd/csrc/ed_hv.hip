@@ -395,18 +395,26 @@ HvKernel<VT> pick_kernel(int G, const LdsLayout& lay) {
     return hv_lds_kernel<VT, BS, 4, true, false>;
 }
 
-// tiles_per_wg = 0: two row tiles per workgroup.  Many short workgroups let
-// the dispatcher balance rows of unequal cost (on the 2x12 ladder 0.78 ms
-// per complex128 product, against 1.35 ms with one round of resident
-// workgroups walking 25 tiles each).  Returns the number of workgroups
-// (partials).
+// tiles_per_wg = 0: one to four row tiles per workgroup, so that there are
+// at least about 24 workgroups per CU.  Many short workgroups let the
+// dispatcher balance rows of unequal cost (on the 2x12 ladder 0.60 ms per
+// complex128 product with four tiles, against 1.35 ms with one round of
+// resident workgroups walking 25 tiles each).  Returns the number of
+// workgroups (partials).
 template <class VT>
 int64_t launch_g(int G, int64_t tiles_per_wg, const LdsLayout& lay, hipStream_t st, const SectorView& v,
                  const HvWeights& w, const void* x, void* y, double* partial, int64_t ntiles) {
     constexpr int BS = 256;
     const HvKernel<VT> k = pick_kernel<VT>(G, lay);
     const size_t bytes = lay.bytes > LDS_CAP ? 0 : size_t(lay.bytes);
-    const int64_t tpw = tiles_per_wg > 0 ? tiles_per_wg : 2;
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 256;
+        return n;
+    }();
+    const int64_t tpw = tiles_per_wg > 0 ? tiles_per_wg : std::clamp<int64_t>(ntiles / (24 * int64_t(cus)), 1, 4);
     const int64_t nwg = (ntiles + tpw - 1) / tpw;
     // XCD chunks of about 131,072 rows (ED_HV_XCD_CHUNK_ROWS overrides, 0 = one eighth per XCD)
     static const int64_t chunk_rows = [] {
