@@ -629,8 +629,7 @@ int ed_hv_tune(ed_sector* s, const double* w, int precision, const double* d_x, 
         double best = 1e300;
         int bt = 0, bg = 0;
         if (s->D > 0 && lds_layout(s).bytes <= LDS_CAP) {
-            // 0: one round of resident workgroups (the default)
-            for (int t : {0, 1, 2, 4, 8, 16, 32}) {
+            for (int t : TUNE_TILES) {
                 for (int g : TUNE_GATHER) {
                     s->tune_tiles = t;
                     s->tune_gather = g;

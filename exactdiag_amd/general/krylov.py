@@ -135,8 +135,10 @@ def lowest_eigenpairs(apply, D, nev, tol=1e-10, ncv=None, v0=None, max_restarts=
     m = min(m, D)
     if isinstance(v0, torch.Tensor):
         v = v0.to("cuda", torch.complex128)
+    elif v0 is None:
+        v = device_start_vector(D)
     else:
-        v = torch.as_tensor(start_vector(D) if v0 is None else np.asarray(v0, np.complex128), device="cuda")
+        v = torch.as_tensor(np.asarray(v0, np.complex128), device="cuda")
     if real:
         # a complex guess is rotated by its global phase first (its real part alone could vanish); a guess that
         # stays complex gives way to a seeded real start vector

@@ -11,6 +11,7 @@ the GPU thick-restart Lanczos of ``krylov.py`` in place of ARPACK.
 from __future__ import annotations
 
 import logging
+import os
 import pathlib
 from os import makedirs
 
@@ -35,6 +36,45 @@ def get_lowest_eigenpairs(config):
     )
     _logger.info(f"Got lowest eigenpairs with k={config.hamiltonian.symmetry_qs.to_npint32()!s}, {eigvals=}")
     return eigvals, eigvecs
+
+
+# Device copies of the ground-state vectors last loaded or computed in this process, keyed by their files
+# (path, size, modification time): every q of an all-q run starts from the same ground state, which the
+# reference reloads from disk for each q.  One entry; a rewritten file misses.
+_GROUND_STATES: dict = {}
+
+
+def _file_key(paths):
+    key = []
+    for p in paths:
+        st = os.stat(p)
+        key.append((str(p), st.st_size, st.st_mtime_ns))
+    return tuple(key)
+
+
+def ground_states_on_device(config):
+    """(eigenvalues, (n_degenerate, D) device tensor of the ground states) for calculate_spectrum: the same
+    eigenpairs get_lowest_eigenpairs returns (loaded or calculated and saved), kept on the device between
+    calls for the same files."""
+    import torch
+
+    paths = config.get_eigenpair_paths()
+    if all(os.path.isfile(p) for p in paths):
+        hit = _GROUND_STATES.get(_file_key(paths))
+        if hit is not None:
+            return hit
+    eigvals, eigvecs = get_lowest_eigenpairs(config)
+    eigvals = np.asarray(eigvals)
+    if len(eigvals) == 0:
+        return eigvals, eigvecs
+    n = 0
+    while n < len(eigvals) and abs(eigvals[0] - eigvals[n]) <= 1e-6:
+        n += 1
+    vecs = torch.as_tensor(np.ascontiguousarray(np.asarray(eigvecs)[:, :n].T), device="cuda")
+    _GROUND_STATES.clear()
+    if all(os.path.isfile(p) for p in paths):
+        _GROUND_STATES[_file_key(paths)] = (eigvals[:n], vecs)
+    return eigvals[:n], vecs
 
 
 def load_eigenpairs(vals_path, vecs_path):
@@ -106,7 +146,7 @@ def get_spectrum(config, suffix: str = ""):
     """Load or calculate and save one spectrum (lanczos_diagonalization.py:124-143)."""
 
     def calculate(**kwargs):
-        eigvals, eigvecs = get_lowest_eigenpairs(config)
+        eigvals, eigvecs = ground_states_on_device(config)
         get_operator, final_kwargs, get_excited_H = config.setup_excitation_operator()
         return calculate_spectrum(
             ws=config.spectrum.get_omegas(),
