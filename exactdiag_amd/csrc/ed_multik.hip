@@ -46,6 +46,12 @@ ed_hmat* hmat_build(const ed_sector* s, const double* w, int coded, hipStream_t 
 
 namespace {
 
+// acc += (vr + i vi) x as fused multiply-adds (the stored products' accumulation)
+__device__ inline void mk_fma(C2& acc, double vr, double vi, const C2& x) {
+    acc.re = __builtin_fma(vr, x.re, __builtin_fma(-vi, x.im, acc.re));
+    acc.im = __builtin_fma(vr, x.im, __builtin_fma(vi, x.re, acc.im));
+}
+
 constexpr int MK = 16;          // slots per block
 constexpr int MBS = 256;        // threads per workgroup: 16 rows x 16 slots
 constexpr int MK_CODES = 128;   // term * 32 + p
@@ -178,7 +184,7 @@ __global__ void __launch_bounds__(64 * NW) mk_spmv_kernel(StoredView m, int64_t 
             for (int u = 0; u < U; ++u) {
                 const double2 q = Q[c[u] * MK + k];
                 const double sg = ng[u] ? -1.0 : 1.0;
-                acc = cadd(acc, cmul(C2{sg * q.x, sg * q.y}, xv[u]));
+                mk_fma(acc, sg * q.x, sg * q.y, xv[u]);
             }
         }
         for (; e < n; ++e) {
@@ -189,7 +195,7 @@ __global__ void __launch_bounds__(64 * NW) mk_spmv_kernel(StoredView m, int64_t 
             const C2 xv = ldv(x, int64_t(j) * MK + k);
             const double2 q = Q[c * MK + k];
             const double sg = ng ? -1.0 : 1.0;
-            acc = cadd(acc, cmul(C2{sg * q.x, sg * q.y}, xv));
+            mk_fma(acc, sg * q.x, sg * q.y, xv);
         }
         const C2 xf = ldv(x, f * MK + k);
         // diagonal: -1/2 per antiparallel bond times the slot's jl / jr weight, as the single-sector table has it

@@ -146,7 +146,9 @@ def test_flip_blocks_hold_the_sector_spectrum_and_its_spectra(torch):
     assert abs(n2a - n2b) < 1e-9
     # the same recurrence in another basis: the coefficients agree until orthogonality is lost, the spectra
     # to the truncation level of the fraction
-    assert np.max(np.abs(aa[:40] - ab[:40])) < 1e-9 and np.max(np.abs(ba[:40] - bb[:40])) < 1e-9
+    # (the two ground states agree to their tolerance, 1e-11, and the recurrences amplify the difference by
+    # orders of magnitude within a few tens of steps: about 1e-13 at step 5, 1e-9 near step 30)
+    assert np.max(np.abs(aa[:20] - ab[:20])) < 1e-9 and np.max(np.abs(ba[:20] - bb[:20])) < 1e-9
     dev = np.max(np.abs(sa - sb)) / np.max(sa)
     print("max |S - S_flip| / max S", dev, flush=True)
     assert dev < 1e-3
