@@ -147,7 +147,8 @@ __global__ void __launch_bounds__(BS) hm_orbit_sizes_kernel(const uint8_t* __res
 // rows hold is ranked by its place in row 0, and each row puts its elements
 // with those pairs first, in that order (exchange on bonds above the spins that
 // vary within the slice has the same rank change in every row), then the rest
-// by column; hm_uniform_kernel then finds the shared leading slots.  Otherwise
+// by column (a pair that row 0 holds twice counts once); hm_uniform_kernel
+// then finds the shared leading slots.  Otherwise
 // the rows are sorted by column alone.  A row of up to 32 elements is sorted
 // by a bitonic network in registers; a longer one (0.07 % of the rows of the
 // 2x12, 2-hole sector) is insertion-sorted in place in HBM.  The key is
@@ -196,34 +197,64 @@ __global__ void __launch_bounds__(64) hm_sort_kernel(int64_t D, const int64_t* _
 #pragma unroll
         for (int k = 0; k < 32; ++k) r[k] = k < n ? ((uint64_t(255) << HB) | word_at(k)) : ~uint64_t(0);
         if (align) {
-            // row 0's elements (the candidates) through LDS, so that the register array is only indexed by
-            // unrolled constants
-            __shared__ uint64_t row0[32];
+            // row 0's (offset, code) pairs, the candidates, in a 64-slot LDS hash table (key -> place in row 0;
+            // a pair row 0 holds twice keeps its first place); every lane then looks its own pairs up, and the
+            // candidates that all 64 rows hold (the AND of the lanes' masks) get their place as rank
+            __shared__ uint64_t hkey[64];
+            __shared__ int hidx[64];
+            auto pair_key = [&](uint64_t w, int64_t row) -> uint64_t {
+                return (uint64_t(uint32_t(offset_of(w, row))) << 32) | code_of(w);
+            };
+            auto slot_of = [](uint64_t key) -> int {
+                const uint32_t hsh = uint32_t(key >> 32) * 0x9E3779B1u ^ uint32_t(key) * 0x85EBCA6Bu;
+                return int(hsh >> 26);
+            };
+            hidx[lane] = -1;
+            hkey[lane] = 0;
+            __syncthreads();
+            const int64_t f0 = int64_t(blockIdx.x) * 64;
             if (lane == 0) {
 #pragma unroll
-                for (int k = 0; k < 32; ++k) row0[k] = r[k];
-            }
-            __syncthreads();
-            const int n0 = __shfl(n, 0, 64);
-            const int64_t f0 = int64_t(blockIdx.x) * 64;
-            for (int c = 0; c < 32; ++c) {
-                if (c < n0) {  // uniform: n0 is lane 0's count
-                    const uint64_t w0 = row0[c];
-                    const int32_t d0 = offset_of(w0 & LOW, f0);
-                    const uint32_t c0 = code_of(w0 & LOW);
-                    int hit = -1;
-#pragma unroll
-                    for (int k = 0; k < 32; ++k)
-                        if (hit < 0 && k < n && (r[k] >> HB) == 255u && offset_of(r[k] & LOW, f) == d0 &&
-                            code_of(r[k] & LOW) == c0)
-                            hit = k;
-                    if (__all(hit >= 0)) {
-#pragma unroll
-                        for (int k = 0; k < 32; ++k)
-                            if (k == hit) r[k] = (uint64_t(c) << HB) | (r[k] & LOW);
+                for (int k = 0; k < 32; ++k) {
+                    if (k < n) {
+                        const uint64_t key = pair_key(r[k] & LOW, f0);
+                        int h = slot_of(key);
+                        while (hidx[h] >= 0 && hkey[h] != key) h = (h + 1) & 63;
+                        if (hidx[h] < 0) {
+                            hkey[h] = key;
+                            hidx[h] = k;
+                        }
                     }
                 }
             }
+            __syncthreads();
+            uint32_t have = 0;
+            int cand[32];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                cand[k] = -1;
+                if (k < n) {
+                    const uint64_t key = pair_key(r[k] & LOW, f);
+                    int h = slot_of(key);
+                    int c = -1;
+                    while (hidx[h] >= 0) {  // at most 32 of the 64 slots are taken, so an empty one ends the probe
+                        if (hkey[h] == key) {
+                            c = hidx[h];
+                            break;
+                        }
+                        h = (h + 1) & 63;
+                    }
+                    if (c >= 0 && !((have >> c) & 1u)) {
+                        have |= 1u << c;
+                        cand[k] = c;
+                    }
+                }
+            }
+            uint32_t all = have;
+            for (int o = 32; o > 0; o >>= 1) all &= __shfl_xor(all, o, 64);
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                if (cand[k] >= 0 && ((all >> cand[k]) & 1u)) r[k] = (uint64_t(cand[k]) << HB) | (r[k] & LOW);
         }
 #pragma unroll
         for (int size = 2; size <= 32; size <<= 1) {
