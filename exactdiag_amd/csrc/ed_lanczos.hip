@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "ed_internal.h"
@@ -40,6 +41,15 @@ __device__ inline void ldc(const VT* p, int64_t i, double& re, double& im) {
         re = double(v.x);
         im = double(v.y);
     }
+}
+
+// x rounded to the component type of VT, as stc stores it
+template <class VT>
+__device__ inline double rounded(double x) {
+    if constexpr (std::is_same_v<VT, float2> || std::is_same_v<VT, float>)
+        return double(float(x));
+    else
+        return x;
 }
 
 template <class VT>
@@ -134,8 +144,8 @@ __global__ void __launch_bounds__(BS) lz_update_kernel(LzDev lz, int step, int64
         const double wi = hi - an * vi - bn * qi;
         stc(up, i, wr, wi);
         if (basis) stc(basis, i, vr, vi);
-        double r2, i2;
-        ldc(up, i, r2, i2);
+        // the norm of w as stored (rounded to the vector type), from registers
+        const double r2 = rounded<VT>(wr), i2 = is_real_vec<VT>::value ? 0.0 : rounded<VT>(wi);
         n2 += r2 * r2 + i2 * i2;
     }
     double z = 0.0;
