@@ -33,7 +33,7 @@
 #include <vector>
 
 #include "ed_hmatrow.h"
-#include "ed_hvrow.h"
+#include "ed_hvuni.h"
 
 namespace ed {
 namespace {
@@ -118,7 +118,25 @@ __global__ void __launch_bounds__(BS) bhv_kernel(const BInst* __restrict__ insts
         const Tabs T = I.lds ? stage_tabs<BS>(s, I.lay, lds) : global_tabs(s);
         __syncthreads();
         const HvWeights w = I.w;
-        for (int64_t f = job.r0 + threadIdx.x; f < job.r1; f += BS) {
+        // the wavefront-uniform walk where the instance's tables (with the chunked ranks) are staged; a job's
+        // rows start at a multiple of 256, so each wavefront covers 64 consecutive rows
+        const bool uni = I.lds && I.lay.rt >= 0;
+        for (int64_t r = job.r0; r < job.r1; r += BS) {
+            const int64_t f = r + threadIdx.x;
+            if (uni) {
+                const int64_t f0 = r + int(threadIdx.x & ~63u);
+                if (f0 >= job.r1) continue;
+                const bool live = f < job.r1;
+                C2 acc, xf;
+                if (uniform_row<double2, G>(s, T, I.lay, w, xi, f, f0, live, acc, xf)) {
+                    if (live) {
+                        yi[f] = double2{acc.re, acc.im};
+                        dot = cadd(dot, cmul(C2{xf.re, -xf.im}, acc));
+                    }
+                    continue;
+                }
+            }
+            if (f >= job.r1) continue;
             C2 xf;
             const C2 acc = hv_row<double2, G>(s, T, w, xi, f, xf);
             yi[f] = double2{acc.re, acc.im};
@@ -370,7 +388,7 @@ int ed_batch_create(ed_sector* const* sectors, const double* weights, int n, ed_
                 b->sectors.push_back(s);
                 BInst I{};
                 I.s = s->view();
-                I.lay = lds_layout(s);
+                I.lay = hv_uniform_enabled() ? lds_layout_ranks(s) : lds_layout(s);
                 I.lds = I.lay.bytes <= LDS_CAP;
                 if (I.lds) b->lds_bytes = std::max(b->lds_bytes, I.lay.bytes);
                 I.w = make_weights(weights + 8 * i);
