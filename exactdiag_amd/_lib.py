@@ -142,6 +142,11 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise LibraryMissing(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        # one HIP runtime per process: torch's own libamdhip64 must be loaded before the library, whose
+        # dependency then resolves to it; loaded first, the library would pull in the system runtime and torch
+        # would see no device
+        import torch  # noqa: F401
+
         handle = ct.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(handle, name)

@@ -48,4 +48,7 @@ def test_learned_closure_beats_terminator(model):
                   flush=True)
     gains = np.array(gains)
     assert len(gains) >= 8
-    assert np.exp(np.mean(np.log(gains))) > 1.0  # geometric-mean error ratio in favour of the network
+    # geometric-mean error ratio in favour of the network (1.20 on these systems when measured), and the
+    # network ahead on most of the spectra
+    assert np.exp(np.mean(np.log(gains))) > 1.1
+    assert np.mean(gains > 1.0) > 0.6

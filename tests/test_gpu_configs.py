@@ -250,5 +250,5 @@ def test_c5_learned_closure_on_2x12_all_q(torch):
     gain = np.exp(np.mean(np.log(errs[:, 0] / errs[:, 1])))
     print(f"C5: mean L1 terminator {errs[:, 0].mean():.2e}, learned {errs[:, 1].mean():.2e}, geometric gain "
           f"{gain:.2f}", flush=True)
-    assert gain > 1.0
-    assert errs[:, 1].max() < 2e-2
+    assert gain > 1.1  # 1.18 when measured
+    assert errs[:, 1].mean() < errs[:, 0].mean() and errs[:, 1].max() < 2e-2
