@@ -202,8 +202,17 @@ ALLQ_WS = (0.0, 4.0, 2001)
 ALLQ_ETA = 0.05
 ALLQ_STEPS_REF = 1000
 ALLQ_STEPS_PLAIN = 256
-ALLQ_STEPS_FAST = 170  # 150 meets 1e-3 only by a few per cent (its FP64 max is 9.0e-4); 170: 5.8e-4
+ALLQ_STEPS_FAST = 170  # fixed count: 150 meets 1e-3 only by a few per cent (its FP64 max is 9.0e-4); 170: 5.8e-4
+ALLQ_STEPS_CAP = 300  # the fast form's cap when each q stops by the spectrum-change rule
 ALLQ_TARGET = 1e-3
+
+
+def steps_summary(steps):
+    v = list(steps.values())
+    return {"rule": "per q: stop when the terminated spectrum changes by < 1e-3 (relative L1) over 20 steps, "
+                    f"checked every 5 steps from step 60, at most {ALLQ_STEPS_CAP}",
+            "mean": float(sum(v) / max(len(v), 1)), "max": max(v, default=0), "total": sum(v),
+            "per_q": {f"{q[0]},{q[1]}": n for q, n in steps.items()}}
 
 
 def allq_larger(rungs):
@@ -222,16 +231,16 @@ def allq_larger(rungs):
     e0, ref = allq.allq_reference(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_REF, log=log)
     t_ref = time.perf_counter() - t0
     log(f"2x{rungs}: all-q reference ({ALLQ_STEPS_REF} steps) in {t_ref:.1f}s, E0 = {e0:.12f}")
-    t_fast, s_fast = allq.allq_fast(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
-    fast_breakdown = dict(allq.LAST_FAST_TIMINGS)
-    log(f"2x{rungs}: all-q fast in {t_fast:.2f}s: {fast_breakdown}")
+    t_fast, s_fast = allq.allq_fast(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_CAP, spectrum_tol=ALLQ_TARGET)
+    fast_breakdown, fast_steps = dict(allq.LAST_FAST_TIMINGS), dict(allq.LAST_FAST_STEPS)
+    log(f"2x{rungs}: all-q fast in {t_fast:.2f}s: {fast_breakdown}, steps {fast_steps}")
     t_m, s_m, tm_m = allq.allq_multik(rungs, HOLES, nd, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
     log(f"2x{rungs}: all-q over one multi-momentum H in {t_m:.2f}s: {tm_m}")
     return {
         "workload": f"S(q,w) for the {len(ref)} distinct q of the 2x{rungs} ladder, 2 holes, Sz=0, k=(0,0), t=1, "
                     "J=0.4; eta=0.05, w in [0,4] on 2001 points; L1 vs 1000-step FP64",
         "reference_seconds": t_ref,
-        "fast": {"seconds": t_fast, "steps": ALLQ_STEPS_FAST, "breakdown_s": fast_breakdown,
+        "fast": {"seconds": t_fast, "steps": steps_summary(fast_steps), "breakdown_s": fast_breakdown,
                  "max_l1": max(allq.spectral_l1(s_fast[q], ref[q], ws) for q in ref)},
         "multik": {"seconds": t_m, "steps": ALLQ_STEPS_FAST, "breakdown_s": tm_m,
                    "max_l1": max(allq.spectral_l1(s_m[q], ref[q], ws) for q in ref)},
@@ -257,9 +266,11 @@ def allq_section():
     log(f"all-q reference ({ALLQ_STEPS_REF} steps) in {time.perf_counter() - t0:.1f}s, E0 = {e0:.12f}")
     t_plain, s_plain = allq.allq_plain(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_PLAIN)
     log(f"all-q plain in {t_plain:.2f}s")
-    t_fast, s_fast = allq.allq_fast(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
-    fast_breakdown = dict(allq.LAST_FAST_TIMINGS)
-    log(f"all-q fast in {t_fast:.2f}s: {fast_breakdown}")
+    t_fast, s_fast = allq.allq_fast(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_CAP, spectrum_tol=ALLQ_TARGET)
+    fast_breakdown, fast_steps = dict(allq.LAST_FAST_TIMINGS), dict(allq.LAST_FAST_STEPS)
+    log(f"all-q fast in {t_fast:.2f}s: {fast_breakdown}, steps {fast_steps}")
+    t_ff, s_ff = allq.allq_fast(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_FAST)
+    log(f"all-q fast with a fixed {ALLQ_STEPS_FAST} steps in {t_ff:.2f}s")
     t_ps, s_ps = allq.allq_fast(L, HOLES, NDOWN, WEIGHTS, ws, ALLQ_ETA, ALLQ_STEPS_PLAIN, precision=0,
                                 terminator=False, two_pass_gs=False, real=False)
     log(f"all-q plain method on the stored H in {t_ps:.2f}s")
@@ -276,12 +287,17 @@ def allq_section():
         "plain": {"seconds": t_plain, "steps": ALLQ_STEPS_PLAIN, "max_l1": max(err_plain.values()),
                   "method": "complex128, matrix-free H·v, ground state by thick-restart Lanczos at tol 2e-7, "
                             "truncated continued fraction, q one after the other"},
-        "fast": {"seconds": t_fast, "steps": ALLQ_STEPS_FAST, "max_l1": max(err_fast.values()),
+        "fast": {"seconds": t_fast, "steps": steps_summary(fast_steps), "max_l1": max(err_fast.values()),
                  "breakdown_s": fast_breakdown,
                  "method": "stored sliced-ELL H per sector, complex64 vectors with FP64 coefficient reductions "
                            "(float32 in the sectors whose H is real, kx = 0 and L/2), square-root terminator; ground "
                            "state by restarted two-pass Lanczos (FP64: float64 vectors in the real k = (0,0) "
-                           "sector; no re-orthogonalisation) to the same tolerance"},
+                           "sector; no re-orthogonalisation) to the same tolerance; each q's run stops when its "
+                           "terminated spectrum has converged (the steps rule), checked on the host while the GPU "
+                           "runs the next 5 steps"},
+        "fast_fixed_steps": {"seconds": t_ff, "steps": ALLQ_STEPS_FAST,
+                             "max_l1": max(allq.spectral_l1(s_ff[q], ref[q], ws) for q in ref),
+                             "method": f"the fast form with a fixed {ALLQ_STEPS_FAST} steps for every q"},
         "plain_stored": {"seconds": t_ps, "steps": ALLQ_STEPS_PLAIN,
                          "max_l1": max(allq.spectral_l1(s_ps[q], ref[q], ws) for q in ref),
                          "method": "the plain method (complex128, truncated fraction) on the stored H: the H·v "

@@ -31,6 +31,7 @@ EXPORTED = [
     "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune", "ed_lanczos_ritz",
     "ed_hmat_create", "ed_hmat_destroy", "ed_hmat_info", "ed_hmat_hv", "ed_hmat_lanczos_run", "ed_hmat_lanczos_ritz",
     "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_hmat_is_real", "ed_batch_create_stored",
+    "ed_hmat_lanczos_run_tol", "ed_spectrum_change",
     "ed_csr_from_sector", "ed_csr_from_operator", "ed_csr_create", "ed_csr_info", "ed_csr_copy_out", "ed_csr_destroy",
     "ed_csr_sum_create", "ed_csr_sum_info", "ed_csr_sum_hv", "ed_csr_sum_destroy",
     "ed_position_op_columns", "ed_position_op_apply", "ed_position_expectation", "ed_position_correlations",
@@ -128,6 +129,19 @@ _SIGNATURES = {
     "ed_mk_residual_norms": (ct.c_int, [_vp, _vp, _vp, _dp, _dp, _vp]),
     "ed_mk_scale": (ct.c_int, [_vp, _vp, _dp, _vp]),
 }
+
+
+
+class LanczosStop(ct.Structure):
+    """struct ed_lanczos_stop (include/exactdiag_hip.h)."""
+
+    _fields_ = [("ws", _dp), ("n_w", ct.c_int), ("stride", ct.c_int), ("e0", ct.c_double), ("eta", ct.c_double),
+                ("tol", ct.c_double), ("min_steps", ct.c_int), ("check_every", ct.c_int), ("delta", ct.c_int)]
+
+
+_SIGNATURES["ed_hmat_lanczos_run_tol"] = (ct.c_int, [_vp, _vp, ct.c_int, ct.POINTER(LanczosStop), _dp, _dp, _ip, _ip,
+                                                     ct.c_int, _vp])
+_SIGNATURES["ed_spectrum_change"] = (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, ct.POINTER(LanczosStop), _dp])
 
 _lib = None
 

@@ -137,6 +137,7 @@ def run_allq(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), precision=0, limited
 REFERENCE_EIG_TOL = 2e-7  # the reference's ARPACK tolerance (general/lanczos_diagonalization.py:89)
 GS_STEPS = 100  # first-pass length of the fast forms' two-pass ground state (restarts cover slower systems)
 LAST_FAST_TIMINGS = {}  # seconds per stage of the last allq_fast run
+LAST_FAST_STEPS = {}  # Lanczos steps per q of the last allq_fast run
 
 
 def _sync():
@@ -208,16 +209,18 @@ def allq_plain(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=
 
 
 def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=REFERENCE_EIG_TOL, precision=1,
-              terminator=True, two_pass_gs=True, real=True):
+              terminator=True, two_pass_gs=True, real=True, spectrum_tol=None):
     """The fast form: stored H (built once per sector on the GPU), complex64 Lanczos vectors with FP64
     coefficient reductions, fewer steps closed by the square-root terminator; the ground state by restarted
     two-pass Lanczos in complex128 (no re-orthogonalisation, the same residual criterion as the plain run).
     With real=True, sectors whose stored H is real (kx = 0, L/2) run in real vectors of the same precision
     (float64 for the ground state, float32 for the spectra) when the start vector is real up to a global
-    phase.  With precision=0, terminator=False, two_pass_gs=False and real=False it is the plain method on the
-    stored H.  Returns (seconds, spectra)."""
+    phase.  With spectrum_tol, each q's run stops by the rule of :class:`StopRule` (its terminated spectrum
+    changes by less than spectrum_tol, relative L1, over 20 steps; checked every 5 steps from step 60) and
+    nsteps is only the cap; the steps each q took are in LAST_FAST_STEPS.  With precision=0, terminator=False,
+    two_pass_gs=False and real=False it is the plain method on the stored H.  Returns (seconds, spectra)."""
     from exactdiag_amd.closure import sqrt_terminator
-    from exactdiag_amd.sector import real_phase
+    from exactdiag_amd.sector import StopRule, real_phase
 
     tm = dict.fromkeys(("ground_state", "bases", "stored_h", "operator", "lanczos", "continued_fraction"), 0.0)
 
@@ -241,7 +244,7 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
         e0, psi = _gs(hm0.hv, s0.dim, tol)
     hm0.close()
     t = lap("ground_state", t)
-    out = {}
+    out, steps = {}, {}
     for qx, qy in distinct_qs(L, limited):
         kx, ky = final_momentum(L, k0[0], k0[1], qx, qy)
         sec = Sector(L, h, nd, kx, ky)
@@ -255,13 +258,17 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
             hm = sec.store(weights)
             t = lap("stored_h", t)
             phr = real_phase(phi) if (real and hm.is_real) else None
-            if phr is not None:
-                a, b = hm.lanczos(phr, nsteps, precision=_lib.PREC_R64 if precision == 0 else _lib.PREC_R32)
+            v0, prec = (phr, _lib.PREC_R64 if precision == 0 else _lib.PREC_R32) if phr is not None else (phi, precision)
+            if spectrum_tol is not None:
+                a, b, why = hm.lanczos_to_tol(v0, nsteps, StopRule(ws, e0, eta, spectrum_tol), precision=prec)
+                exhausted = why == 2
             else:
-                a, b = hm.lanczos(phi, nsteps, precision=precision)
+                a, b = hm.lanczos(v0, nsteps, precision=prec)
+                exhausted = len(a) < nsteps
+            steps[(qx, qy)] = len(a)
             hm.close()
             t = lap("lanczos", t)
-            at, bt, tail = sqrt_terminator(a, b, len(a)) if (terminator and len(a) == nsteps) else (a, b, None)
+            at, bt, tail = sqrt_terminator(a, b, len(a)) if (terminator and not exhausted) else (a, b, None)
             out[(qx, qy)] = cf_eval(at, bt, ws, e0, eta, -n2 / np.pi, tail=tail)
             t = lap("continued_fraction", t)
         sec.close()
@@ -269,6 +276,8 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
     _sync()
     LAST_FAST_TIMINGS.clear()
     LAST_FAST_TIMINGS.update(tm)
+    LAST_FAST_STEPS.clear()
+    LAST_FAST_STEPS.update(steps)
     return time.perf_counter() - t0, out
 
 

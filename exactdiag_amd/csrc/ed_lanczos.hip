@@ -7,9 +7,13 @@
 // s_{n+1} = 1 / b_{n+1}.  Vectors are never rescaled in place: u_n is the
 // unnormalised Lanczos vector and s_n its scale, which saves a full pass per
 // step.  No host synchronisation inside the loop except a stop check every
-// 64 steps.  All reductions are fixed-order, so runs are reproducible.
+// 64 steps; with a stop rule (ed_hmat_lanczos_run_tol) the host reads the
+// coefficients of each chunk while the GPU runs the next one.  All
+// reductions are fixed-order, so runs are reproducible.
 
+#include <algorithm>
 #include <cmath>
+#include <complex>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -269,13 +273,121 @@ __global__ void zero2_kernel(double2* p, int64_t n) {
     if (i < n) p[i] = double2{0.0, 0.0};
 }
 
+// Eigenvalues below x of the symmetric tridiagonal with diagonal d[0..n) and
+// off-diagonal e[1..n) (e[i] couples i - 1 and i): the negative pivots of
+// its LDL^T factorisation (Sturm count); a pivot that vanishes is replaced
+// by -pivmin.
+int sturm_below(const double* d, const double* e, int n, double x, double pivmin) {
+    int c = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {
+        q = (d[i] - x) - (i ? e[i] * e[i] / q : 0.0);
+        if (std::fabs(q) < pivmin) q = -pivmin;
+        if (q < 0.0) ++c;
+    }
+    return c;
+}
+
+// Smallest and largest eigenvalue of that tridiagonal, by bisection inside
+// its Gershgorin interval to a relative width of 1e-14.
+void extreme_eigenvalues(const double* d, const double* e, int n, double& lo, double& hi) {
+    double gl = d[0], gu = d[0], emax = 1.0;
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? std::fabs(e[i]) : 0.0) + (i + 1 < n ? std::fabs(e[i + 1]) : 0.0);
+        gl = std::min(gl, d[i] - r);
+        gu = std::max(gu, d[i] + r);
+        if (i > 0) emax = std::max(emax, e[i] * e[i]);
+    }
+    const double pivmin = 1e-300 * emax;
+    auto kth = [&](int k) {  // the (k + 1)-th smallest eigenvalue
+        double l = gl, u = gu;
+        for (int it = 0; it < 200; ++it) {
+            const double m = 0.5 * (l + u);
+            if (u - l <= 1e-14 * std::max(std::fabs(l), std::fabs(u)) || m <= l || m >= u) break;
+            if (sturm_below(d, e, n, m, pivmin) > k)
+                u = m;
+            else
+                l = m;
+        }
+        return 0.5 * (l + u);
+    };
+    lo = kth(0);
+    hi = kth(n - 1);
+}
+
+// Im(1 / det(z)) of the first n levels, closed by the square-root terminator
+// at the bandwidth of their tridiagonal (as cf_kernel with a_inf = centre,
+// b_inf = half-width / 2), at z = w + i eta for the grid points of `w`.
+void terminated_spectrum(const double* a, const double* b, int n, const std::vector<double>& w, double eta,
+                         std::vector<double>& out) {
+    double lo, hi;
+    extreme_eigenvalues(a, b, n, lo, hi);
+    const double a_inf = 0.5 * (hi + lo), b_inf = 0.5 * std::max(0.5 * (hi - lo), 1e-12);
+    const double bb = b_inf * b_inf;
+    const size_t m = w.size();
+    std::vector<double> dr(m), di(m);
+    for (size_t k = 0; k < m; ++k) {
+        const std::complex<double> z(w[k], eta), u = z - a_inf;
+        const std::complex<double> s = std::sqrt(u * u - 4.0 * bb);
+        std::complex<double> t = (u - s) / (2.0 * bb);
+        if (t.imag() > 0.0) t = (u + s) / (2.0 * bb);
+        const std::complex<double> det0 = z - a[n - 1] - bb * t;
+        dr[k] = det0.real();
+        di[k] = det0.imag();
+    }
+    // the levels in real arithmetic, all grid points per level: det <- z - a_i - b_{i+1}^2 / det
+    for (int i = 1; i < n; ++i) {
+        const double ai = a[n - 1 - i], b2 = b[n - i] * b[n - i];
+        for (size_t k = 0; k < m; ++k) {
+            const double q = b2 / (dr[k] * dr[k] + di[k] * di[k]);
+            dr[k] = w[k] - ai - q * dr[k];
+            di[k] = eta + q * di[k];
+        }
+    }
+    out.resize(m);
+    for (size_t k = 0; k < m; ++k) out[k] = -di[k] / (dr[k] * dr[k] + di[k] * di[k]);
+}
+
+// Relative L1 distance between the terminated spectra of n and n - delta
+// levels on the rule's strided grid (weights: central differences of the
+// grid, one-sided at its ends).
+double spectrum_change(const double* a, const double* b, int n, int delta, const ed_lanczos_stop& r) {
+    std::vector<double> w;
+    for (int k = 0; k < r.n_w; k += r.stride) w.push_back(r.ws[k] + r.e0);
+    const size_t m = w.size();
+    std::vector<double> dw(m, 1.0);
+    if (m > 1) {
+        dw[0] = w[1] - w[0];
+        dw[m - 1] = w[m - 1] - w[m - 2];
+        for (size_t k = 1; k + 1 < m; ++k) dw[k] = 0.5 * (w[k + 1] - w[k - 1]);
+    }
+    std::vector<double> s1, s2;
+    terminated_spectrum(a, b, n, w, r.eta, s1);
+    terminated_spectrum(a, b, n - delta, w, r.eta, s2);
+    double num = 0.0, den = 0.0;
+    for (size_t k = 0; k < m; ++k) {
+        num += std::fabs(s1[k] - s2[k]) * dw[k];
+        den += std::fabs(s2[k]) * dw[k];
+    }
+    if (den == 0.0) return num == 0.0 ? 0.0 : HUGE_VAL;
+    return num / den;
+}
+
+void check_stop_rule(const ed_lanczos_stop* r) {
+    ED_REQUIRE(r && r->ws && r->n_w >= 2 && r->stride >= 1 && r->eta > 0.0 && r->tol > 0.0 && r->check_every >= 1 &&
+                   r->delta >= 1 && r->min_steps >= 1,
+               ED_ERR_ARG, "bad stop rule: need a grid of >= 2 points, stride >= 1, eta > 0, tol > 0, "
+                           "check_every >= 1, delta >= 1, min_steps >= 1");
+}
+
 // acc (optional, c128) receives sum_n ritz[n] v_n: the second pass of
 // two-pass Lanczos (the recurrence is deterministic, so it sees the vectors
 // of the first pass).
 template <class VT>
 void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
                    int* n_out, VT* d_basis, int precision, hipStream_t st, double2* acc = nullptr,
-                   const double* ritz = nullptr, const ed_hmat* m = nullptr) {
+                   const double* ritz = nullptr, const ed_hmat* m = nullptr, const ed_lanczos_stop* stop = nullptr,
+                   int* reason = nullptr) {
     if constexpr (is_real_vec<VT>::value) ED_REQUIRE(m, ED_ERR_ARG, "real vectors need a stored matrix");
     const int64_t D = s->D;
     const int64_t nwg = (D + BS - 1) / BS;
@@ -286,7 +398,15 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
     LzDev lz{dev_alloc<double>(size_t(nsteps)), dev_alloc<double>(size_t(nsteps + 1)),
              dev_alloc<double>(size_t(nsteps + 1)), dev_alloc<int>(2)};
     double* d_ritz = nullptr;
+    // the stop rule's two pinned host copies of the coefficients and the flag, one per chunk in flight
+    struct Slot {
+        double* a = nullptr;
+        double* b = nullptr;
+        int* flag = nullptr;
+        hipEvent_t ev = nullptr;
+    } slot[2];
     auto cleanup = [&] {
+        if (stop) (void)hipStreamSynchronize(st);
         for (auto* p : buf) dev_free(p);
         dev_free(partial);
         dev_free(lz.a);
@@ -294,6 +414,11 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         dev_free(lz.s);
         dev_free(lz.flag);
         dev_free(d_ritz);
+        for (Slot& sl : slot) {
+            for (void* p : {(void*)sl.a, (void*)sl.b, (void*)sl.flag})
+                if (p) (void)hipHostFree(p);
+            if (sl.ev) (void)hipEventDestroy(sl.ev);
+        }
     };
     try {
         VT* u = buf[0];
@@ -315,7 +440,7 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
             ED_REQUIRE(flag[1] >= 0, ED_ERR_ARG,
                        "start vector not real (imaginary part above 1e-10 of its norm): rotate its phase first");
         }
-        for (int n = 0; n < nsteps; ++n) {
+        auto launch_step = [&](int n) {
             const int64_t np =
                 m ? launch_hmat(m, u, y, partial, precision, st) : launch_hv(s, hw, u, y, partial, precision, 1, st);
             hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(RBS), 0, st, lz, partial, np, n);
@@ -326,6 +451,61 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
             VT* t = up;  // w (u_{n+1}) now lives in up
             up = u;
             u = t;
+        };
+        if (stop) {
+            // chunks of check_every steps; the host checks the coefficients of chunk c (copied to pinned memory)
+            // while the GPU runs chunk c + 1
+            for (Slot& sl : slot) {
+                ED_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.a), size_t(nsteps) * sizeof(double)));
+                ED_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.b), size_t(nsteps + 1) * sizeof(double)));
+                ED_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.flag), 2 * sizeof(int)));
+                ED_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+            }
+            auto post = [&](Slot& sl, int upto) {
+                ED_HIP(hipMemcpyAsync(sl.a, lz.a, size_t(upto) * sizeof(double), hipMemcpyDeviceToHost, st));
+                ED_HIP(hipMemcpyAsync(sl.b, lz.b, size_t(upto + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+                ED_HIP(hipMemcpyAsync(sl.flag, lz.flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+                ED_HIP(hipEventRecord(sl.ev, st));
+            };
+            int n = 0;
+            const int first = std::min(std::max(stop->min_steps, stop->delta + 1), nsteps);
+            while (n < first) launch_step(n++);
+            post(slot[0], n);
+            int cur = 0, result = 0, why = 0;
+            for (;;) {
+                const int pend = n;  // steps that slot[cur] covers
+                if (n < nsteps) {
+                    const int next = std::min(n + stop->check_every, nsteps);
+                    while (n < next) launch_step(n++);
+                    post(slot[1 - cur], n);
+                }
+                ED_HIP(hipEventSynchronize(slot[cur].ev));
+                const Slot& sl = slot[cur];
+                if (sl.flag[0]) {  // breakdown (or a zero start vector): the fraction is exact, no terminator
+                    result = std::max(sl.flag[1], 0);
+                    why = 2;
+                    break;
+                }
+                const bool met = pend > stop->delta && spectrum_change(sl.a, sl.b, pend, stop->delta, *stop) < stop->tol;
+                if (met || pend >= nsteps) {
+                    result = pend;
+                    why = met ? 1 : 0;
+                    break;
+                }
+                cur = 1 - cur;
+            }
+            ED_HIP(hipStreamSynchronize(st));
+            if (result > 0) {
+                std::copy(slot[cur].a, slot[cur].a + result, a);
+                std::copy(slot[cur].b, slot[cur].b + result, b);
+            }
+            *n_out = result;
+            if (reason) *reason = why;
+            cleanup();
+            return;
+        }
+        for (int n = 0; n < nsteps; ++n) {
+            launch_step(n);
             if ((n & 63) == 63) {
                 ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
                 ED_HIP(hipStreamSynchronize(st));
@@ -414,6 +594,36 @@ int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double
             using VT = decltype(vt);
             lanczos_run_t<VT>(s, w0, d_v0, nsteps, a, b, n_out, nullptr, precision, st, nullptr, nullptr, m);
         });
+    });
+}
+
+int ed_hmat_lanczos_run_tol(const ed_hmat* m, const double* d_v0, int max_steps, const ed_lanczos_stop* stop,
+                            double* a, double* b, int* n_out, int* reason, int precision, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
+        const ed_sector* s = hmat_sector(m);
+        ED_REQUIRE(max_steps >= 1 && a && b && n_out && reason, ED_ERR_ARG, "bad Lanczos arguments");
+        check_stop_rule(stop);
+        hmat_require_precision(m, precision);
+        *n_out = 0;
+        *reason = 2;
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const double w0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        hmat_lanczos(precision, [&](auto vt) {
+            using VT = decltype(vt);
+            lanczos_run_t<VT>(s, w0, d_v0, max_steps, a, b, n_out, nullptr, precision, st, nullptr, nullptr, m, stop,
+                              reason);
+        });
+    });
+}
+
+int ed_spectrum_change(const double* a, const double* b, int n, int delta, const ed_lanczos_stop* stop,
+                       double* out) {
+    return guarded([&] {
+        check_stop_rule(stop);
+        ED_REQUIRE(a && b && out && delta >= 1 && n > delta, ED_ERR_ARG, "need n > delta >= 1 coefficients");
+        *out = spectrum_change(a, b, n, delta, *stop);
     });
 }
 

@@ -376,6 +376,48 @@ class StoredH:
         check(lib().ed_hmat_lanczos_ritz(self._h, _ptr(v0), len(r), rp, _ptr(x), int(precision), _stream()))
         return x
 
+    def lanczos_to_tol(self, v0, max_steps: int, rule: "StopRule", precision: int = 0):
+        """Lanczos coefficients from v0 until the terminated spectrum changes by less than rule.tol over
+        rule.delta steps (ed_hmat_lanczos_run_tol).  Returns (a, b, reason): reason 0 = max_steps reached,
+        1 = the rule was met, 2 = the Krylov space was exhausted (the fraction is exact, no terminator)."""
+        torch = _torch()
+        v0 = v0.to(torch.complex128).contiguous()
+        a = np.zeros(max_steps, np.float64)
+        b = np.zeros(max_steps + 1, np.float64)
+        n, why = ct.c_int(0), ct.c_int(0)
+        check(lib().ed_hmat_lanczos_run_tol(self._h, _ptr(v0), int(max_steps), ct.byref(rule.c),
+                                            a.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                            b.ctypes.data_as(ct.POINTER(ct.c_double)), ct.byref(n), ct.byref(why),
+                                            int(precision), _stream()))
+        k = n.value
+        return a[:k].copy(), b[:k].copy(), why.value
+
+
+class StopRule:
+    """The stop rule of :meth:`StoredH.lanczos_to_tol` (struct ed_lanczos_stop): from min_steps on, every
+    check_every steps, the spectrum of the first n levels closed by the square-root terminator is compared with
+    that of the first n - delta on every stride-th point of ws (z = w + e0 + i eta); the run stops when their
+    relative L1 distance falls below tol."""
+
+    def __init__(self, ws, e0, eta, tol, min_steps=60, check_every=5, delta=20, max_points=1024):
+        from exactdiag_amd._lib import LanczosStop
+
+        self.ws = np.ascontiguousarray(ws, np.float64)
+        stride = max(1, -(-len(self.ws) // max_points))
+        self.c = LanczosStop(self.ws.ctypes.data_as(ct.POINTER(ct.c_double)), len(self.ws), stride, float(e0),
+                             float(eta), float(tol), int(min_steps), int(check_every), int(delta))
+
+    def change(self, a, b, n=None):
+        """The rule's estimate for the first n coefficients (host only, ed_spectrum_change)."""
+        n = len(a) if n is None else n
+        a = np.ascontiguousarray(a[:n], np.float64)
+        b = np.ascontiguousarray(b[:n], np.float64)
+        out = ct.c_double(0.0)
+        check(lib().ed_spectrum_change(a.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                       b.ctypes.data_as(ct.POINTER(ct.c_double)), int(n), int(self.c.delta),
+                                       ct.byref(self.c), ct.byref(out)))
+        return out.value
+
 
 def cf_eval(a, b, ws, e0, eta, factor, tail=None):
     """factor * Im(1/det(z)) for z = ws + e0 + i*eta on the GPU.

@@ -225,6 +225,38 @@ int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double
 int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const double* ritz, double* d_x,
                          int precision, void* stream);
 
+/* Stop rule of a time-to-accuracy Lanczos run (no counterpart in the
+ * reference, whose lanczos_dynamical_response runs a fixed number of steps,
+ * general/cython_lanczos_diagonalization.pyx:18-77).  From min_steps on,
+ * after every check_every steps, the spectrum of the first n coefficients
+ * closed by the square-root terminator (ed_spectrum_change) is compared
+ * with that of the first n - delta; the run stops at the first n where
+ * their relative L1 distance on the grid ws[0], ws[stride], ... (z = w + e0
+ * + i eta) is below tol. */
+typedef struct ed_lanczos_stop {
+    const double* ws; /* host grid of n_w frequencies */
+    int n_w;
+    int stride;       /* >= 1: every stride-th frequency enters the estimate */
+    double e0, eta, tol;
+    int min_steps, check_every, delta;
+} ed_lanczos_stop;
+/* ed_hmat_lanczos_run that stops by the rule (at most max_steps).  The check
+ * of each chunk runs on the host while the GPU computes the next one, so a
+ * run that stops at n has also computed up to check_every further steps;
+ * only the first n coefficients are returned (*n_out = n).  *reason: 0 =
+ * max_steps reached, 1 = the rule was met, 2 = the Krylov space was
+ * exhausted (b below 1e-8; then no terminator applies). */
+int ed_hmat_lanczos_run_tol(const ed_hmat* m, const double* d_v0, int max_steps, const ed_lanczos_stop* stop,
+                            double* a, double* b, int* n_out, int* reason, int precision, void* stream);
+/* The rule's estimate on the host (no GPU): relative L1 distance between the
+ * terminated spectra of the first n and the first n - delta coefficients
+ * (a[0..n), b[0..n), b[0] unused).  Each terminator takes the centre and
+ * half-width w of the extreme eigenvalues of its tridiagonal (bisection),
+ * a_inf = centre, b_inf = w / 2; the L1 weights are the grid's central
+ * differences. */
+int ed_spectrum_change(const double* a, const double* b, int n, int delta, const ed_lanczos_stop* stop,
+                       double* out);
+
 /* The reference's on-disk matrices, held in HBM.  An ed_csr is one stored
  * matrix in the reference's layout: values complex128[n_values] (re, im
  * pairs), value_inds int16[nnz], minor int64[nnz], major int64[n_major + 1],
