@@ -13,6 +13,8 @@
 // Gathers of x_j are issued G at a time before any is consumed.
 #pragma once
 
+#include <algorithm>
+
 #include "ed_internal.h"
 #include "ed_matrix.h"
 #include "ed_orbit.h"
@@ -20,16 +22,21 @@
 namespace ed {
 
 // Byte offsets of the staged tables; rt >= 0 adds the chunked spin-rank
-// table (RankGeom) for the wavefront-uniform row walk.
+// table (RankGeom) for the wavefront-uniform row walk.  comb_bytes: the
+// leading rows of the binomial table that are staged; amp < 0: the
+// amplitude table is read from global memory, not staged.
 struct LdsLayout {
     int comb, amp, sq, gm, cb, bb, bt, bytes;
+    int comb_bytes = 0;
     int rt = -1;
     int rt_n = 0, rt_entries = 0;
     int rt_base[4] = {0, 0, 0, 0};
     int rt_width[4] = {0, 0, 0, 0};
 };
 
-inline LdsLayout lds_layout(const ed_sector* s) {
+// lean: only the binomial rows 0 .. max(h, nd) + 1 (every rank, rank
+// change and hole rank reads rows k <= max(h, nd)), and no amplitude table.
+inline LdsLayout lds_layout(const ed_sector* s, bool lean = false) {
     LdsLayout l{};
     int off = 0;
     auto take = [&](int bytes) {
@@ -37,9 +44,10 @@ inline LdsLayout lds_layout(const ed_sector* s) {
         off += (bytes + 15) & ~15;
         return at;
     };
-    l.comb = take(COMB_W * COMB_W * 4);
+    l.comb_bytes = (lean ? std::min(COMB_W, std::max(s->h, s->nd) + 2) : COMB_W) * COMB_W * 4;
+    l.comb = take(l.comb_bytes);
     const int G = s->flip ? 2 * s->N : s->N;
-    l.amp = take(G * (G + 1) * 16);
+    l.amp = lean ? -1 : take(G * (G + 1) * 16);
     l.sq = take((G + 1) * 8);
     l.gm = take(int(s->n_hole) * 4);
     l.cb = take(int(s->n_hole) * 4);
@@ -55,12 +63,13 @@ constexpr int LDS_CAP = 60 * 1024;
 // take the wavefront-uniform walk (no spin flip, one momentum, vectors under
 // 4 GiB) and all of it fits in LDS_CAP; else rt = -1.
 inline LdsLayout lds_layout_ranks(const ed_sector* s) {
-    LdsLayout l = lds_layout(s);
+    const LdsLayout full = lds_layout(s);
+    LdsLayout l = lds_layout(s, true);
     const RankGeom g = rank_geom(s->ne, s->nd);
     const int bytes = (g.entries * 4 + 15) & ~15;
     // 32-bit byte offsets of x_j for either vector type
     const bool small = s->D * 16 < (int64_t(1) << 32);
-    if (s->flip || s->any_k || !small || s->rank_tab.empty() || l.bytes + bytes > LDS_CAP) return l;
+    if (s->flip || s->any_k || !small || s->rank_tab.empty() || l.bytes + bytes > LDS_CAP) return full;
     l.rt = l.bytes;
     l.bytes += bytes;
     l.rt_n = g.n;
@@ -87,36 +96,45 @@ __device__ inline Tabs global_tabs(const SectorView& s) {
     return Tabs{s.comb, s.amp, s.sqrt_tab, s.canon_gmask, s.canon_block, s.block_base, s.block_table, s.rank_tab};
 }
 
+// Copy `bytes` bytes into LDS with 16-byte loads (dst and src 16-byte
+// aligned), the tail with 4-byte ones.
+template <int BS>
+__device__ inline void stage_bytes(unsigned char* dst, const void* src, int bytes) {
+    const int nv = bytes >> 4;
+    const uint4* s4 = static_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (int i = threadIdx.x; i < nv; i += BS) d4[i] = s4[i];
+    const int nt = (bytes - (nv << 4)) >> 2;
+    if (int(threadIdx.x) < nt)
+        reinterpret_cast<uint32_t*>(dst + (nv << 4))[threadIdx.x] =
+            reinterpret_cast<const uint32_t*>(static_cast<const unsigned char*>(src) + (nv << 4))[threadIdx.x];
+}
+
 // Copy the tables into LDS (all threads of the workgroup take part; the
-// caller synchronises before use).
+// caller synchronises before use).  The device tables are separate
+// allocations (256-byte aligned) and the layout's offsets are multiples of 16.
 template <int BS>
 __device__ inline Tabs stage_tabs(const SectorView& s, const LdsLayout& lay, unsigned char* lds) {
-    uint32_t* s_comb = reinterpret_cast<uint32_t*>(lds + lay.comb);
-    double* s_amp = reinterpret_cast<double*>(lds + lay.amp);
-    double* s_sq = reinterpret_cast<double*>(lds + lay.sq);
-    uint32_t* s_gm = reinterpret_cast<uint32_t*>(lds + lay.gm);
-    int32_t* s_cb = reinterpret_cast<int32_t*>(lds + lay.cb);
-    int64_t* s_bb = reinterpret_cast<int64_t*>(lds + lay.bb);
-    int64_t* s_bt = reinterpret_cast<int64_t*>(lds + lay.bt);
-    const int t = threadIdx.x;
-    for (int i = t; i < COMB_W * COMB_W; i += BS) s_comb[i] = s.comb[i];
-    const int namp = s.G * (s.NO + 1) * 2;
-    for (int i = t; i < namp; i += BS) s_amp[i] = s.amp[i];
-    for (int i = t; i <= s.NO; i += BS) s_sq[i] = s.sqrt_tab[i];
-    for (int i = t; i < s.n_hole_ranks; i += BS) {
-        s_gm[i] = s.canon_gmask[i];
-        s_cb[i] = s.canon_block[i];
+    stage_bytes<BS>(lds + lay.comb, s.comb, lay.comb_bytes);
+    const double* amp = s.amp;
+    if (lay.amp >= 0) {
+        stage_bytes<BS>(lds + lay.amp, s.amp, s.G * (s.NO + 1) * 16);
+        amp = reinterpret_cast<const double*>(lds + lay.amp);
     }
-    for (int i = t; i < s.n_blocks; i += BS) {
-        s_bb[i] = s.block_base[i];
-        s_bt[i] = s.block_table[i];
-    }
-    uint32_t* s_rt = nullptr;
+    stage_bytes<BS>(lds + lay.sq, s.sqrt_tab, (s.NO + 1) * 8);
+    stage_bytes<BS>(lds + lay.gm, s.canon_gmask, s.n_hole_ranks * 4);
+    stage_bytes<BS>(lds + lay.cb, s.canon_block, s.n_hole_ranks * 4);
+    stage_bytes<BS>(lds + lay.bb, s.block_base, s.n_blocks * 8);
+    stage_bytes<BS>(lds + lay.bt, s.block_table, s.n_blocks * 8);
+    const uint32_t* rt = nullptr;
     if (lay.rt >= 0) {
-        s_rt = reinterpret_cast<uint32_t*>(lds + lay.rt);
-        for (int i = t; i < lay.rt_entries; i += BS) s_rt[i] = s.rank_tab[i];
+        stage_bytes<BS>(lds + lay.rt, s.rank_tab, lay.rt_entries * 4);
+        rt = reinterpret_cast<const uint32_t*>(lds + lay.rt);
     }
-    return Tabs{s_comb, s_amp, s_sq, s_gm, s_cb, s_bb, s_bt, s_rt};
+    return Tabs{reinterpret_cast<const uint32_t*>(lds + lay.comb), amp,
+                reinterpret_cast<const double*>(lds + lay.sq), reinterpret_cast<const uint32_t*>(lds + lay.gm),
+                reinterpret_cast<const int32_t*>(lds + lay.cb), reinterpret_cast<const int64_t*>(lds + lay.bb),
+                reinterpret_cast<const int64_t*>(lds + lay.bt), rt};
 }
 
 __device__ inline int32_t rank_holes32(uint32_t H, const uint32_t* comb) {

@@ -302,25 +302,40 @@ __device__ inline C2 hv_row_uniform(const SectorView& s, const Tabs& T, const Ld
     return sink.acc;
 }
 
-// Row f through hv_row_uniform when the rows of f's wavefront, which start at
-// f0 (f0 < D), share one hole pattern (every wavefront but those where a
-// block ends); lanes past the last row (live false) walk row f0 and keep the
-// result to themselves.  Returns false, with nothing computed, otherwise.
+// Whether the wavefront of rows f0 .. f0 + 63 (f0 < D) can take
+// hv_row_uniform: every live row (below D) has the same hole pattern H, whose
+// block is without stabiliser (generic) or has a table.  Lanes past the last
+// row (live false) read row f0.  Outputs describe the wavefront's rows when
+// it returns true.  The list of wavefronts that fail it (hv_fix_list) is
+// built with this same function.
+__device__ inline bool wave_uniform(const SectorView& s, const Tabs& T, int64_t f, int64_t f0, bool live, uint32_t& H,
+                                    uint32_t& Dn, int& blk, uint32_t& gm, bool& generic) {
+    const uint64_t key = s.key[live ? f : f0];
+    const uint32_t Hl = uint32_t(key >> 32);
+    H = __builtin_amdgcn_readfirstlane(Hl);
+    Dn = uint32_t(key);
+    if (!__all(Hl == H)) return false;
+    const int32_t hr = rank_holes32(H, T.comb);
+    gm = T.gm[hr];
+    blk = T.cb[hr];
+    generic = (gm & (gm - 1u)) == 0u && T.bt[blk] < 0;
+    return generic || T.bt[blk] >= 0;
+}
+
+// Row f through hv_row_uniform when its wavefront (rows from f0, f0 < D)
+// passes wave_uniform (every wavefront but those where a block ends); lanes
+// past the last row (live false) walk row f0 and keep the result to
+// themselves.  Returns false, with nothing computed, otherwise.
 template <class VT, int G>
 __device__ inline bool uniform_row(const SectorView& s, const Tabs& T, const LdsLayout& lay, const HvWeights& w,
                                    const VT* __restrict__ x, int64_t f, int64_t f0, bool live, C2& acc, C2& xf) {
-    const uint64_t key = s.key[live ? f : f0];
-    const uint32_t Hl = uint32_t(key >> 32);
-    const uint32_t H = __builtin_amdgcn_readfirstlane(Hl);
-    if (!__all(Hl == H)) return false;
-    const int32_t hr = rank_holes32(H, T.comb);
-    const uint32_t gm = T.gm[hr];
-    const int blk = T.cb[hr];
-    const bool generic = (gm & (gm - 1u)) == 0u && T.bt[blk] < 0;
-    if (!generic && T.bt[blk] < 0) return false;
+    uint32_t H, Dn, gm;
+    int blk;
+    bool generic;
+    if (!wave_uniform(s, T, f, f0, live, H, Dn, blk, gm, generic)) return false;
     const uint32_t fe = uint32_t(live ? f : f0);
-    acc = generic ? hv_row_uniform<VT, G, false>(s, T, lay, w, x, fe, H, uint32_t(key), blk, gm, xf)
-                  : hv_row_uniform<VT, G, true>(s, T, lay, w, x, fe, H, uint32_t(key), blk, gm, xf);
+    acc = generic ? hv_row_uniform<VT, G, false>(s, T, lay, w, x, fe, H, Dn, blk, gm, xf)
+                  : hv_row_uniform<VT, G, true>(s, T, lay, w, x, fe, H, Dn, blk, gm, xf);
     return true;
 }
 

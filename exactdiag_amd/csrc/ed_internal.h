@@ -227,6 +227,9 @@ struct ed_sector {
     uint64_t* d_key = nullptr;
     uint8_t* d_orbit = nullptr;
     double* d_partial = nullptr;  // per-workgroup partials of the fused H·v dot (calls on one sector serialise on it)
+    // 64-row wavefronts (row f0 / 64) that the wavefront-uniform H·v leaves to the per-lane walk (hv_fix_list)
+    int32_t* d_fix = nullptr;
+    int64_t n_fix = 0;
 
     ed::SectorView host_view() const;  // host pointers (tables only)
     ed::SectorView view() const;       // device pointers

@@ -17,7 +17,11 @@ int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y
                   int remap, hipStream_t st);
 int64_t launch_hv_lds(const ed_sector* s, const HvWeights& w, const void* x, void* y, double* partial, int precision,
                       hipStream_t st);
-int64_t hv_partials(const ed_sector* s);
+int64_t hv_partials(const ed_sector* s);  // the most partials any H·v launch on s writes
+// After the basis: the wavefronts that the wavefront-uniform H·v leaves to the per-lane walk (s->d_fix, n_fix);
+// none when the sector does not take that walk.
+void hv_fix_list(ed_sector* s, hipStream_t st);
+int64_t hv_fix_partials(const ed_sector* s);  // the workgroups (partials) that walk those wavefronts
 int64_t launch_hmat(const ed_hmat* m, const void* x, void* y, double* partial, int precision, hipStream_t st);
 const ed_sector* hmat_sector(const ed_hmat* m);
 bool hmat_is_real(const ed_hmat* m);

@@ -868,7 +868,7 @@ int64_t launch_hv(const ed_sector* s, const HvWeights& w, const void* x, void* y
     return nwg;
 }
 
-int64_t hv_partials(const ed_sector* s) { return (s->D + 255) / 256; }
+int64_t hv_partials(const ed_sector* s) { return (s->D + 255) / 256 + hv_fix_partials(s); }
 
 void launch_reduce_partials(const double* partial, int64_t n, double* out, hipStream_t st) {
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, st, partial, n, out);

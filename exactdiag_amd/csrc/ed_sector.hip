@@ -12,6 +12,7 @@
 #include <numeric>
 
 #include "ed_internal.h"
+#include "ed_matrix.h"
 #include "ed_orbit.h"
 
 namespace ed {
@@ -416,8 +417,11 @@ ed::SectorView ed_sector::view() const {
 void ed_sector::release() {
     for (void* p : {(void*)d_comb, (void*)d_amp, (void*)d_sqrt, (void*)d_canon_gmask, (void*)d_canon_block,
                     (void*)d_block_of_rank, (void*)d_block_base, (void*)d_block_table, (void*)d_block_hole,
-                    (void*)d_table, (void*)d_key, (void*)d_orbit, (void*)d_partial, (void*)d_rank_tab, (void*)d_table_orbit})
+                    (void*)d_table, (void*)d_key, (void*)d_orbit, (void*)d_partial, (void*)d_rank_tab, (void*)d_table_orbit,
+                    (void*)d_fix})
         dev_free(p);
+    d_fix = nullptr;
+    n_fix = 0;
     d_comb = nullptr;
     d_amp = d_sqrt = nullptr;
     d_rank_tab = nullptr;
@@ -664,7 +668,6 @@ int ed_basis_build(ed_sector* s, void* stream) {
         s->d_key = dev_alloc<uint64_t>(size_t(s->D));
         s->d_orbit = dev_alloc<uint8_t>(size_t(s->D));
         s->d_table = dev_alloc<int32_t>(size_t(std::max<int64_t>(s->table_size, 1)));
-        s->d_partial = dev_alloc<double>(size_t(2 * ((s->D + 255) / 256)));
         v = s->view();
         hipLaunchKernelGGL(basis_fill_kernel, grid, dim3(SCAN_THREADS), 0, st, v, flags, d_off, cpb, s->d_key,
                            s->d_orbit, s->d_table);
@@ -679,6 +682,8 @@ int ed_basis_build(ed_sector* s, void* stream) {
         dev_free(flags);
         dev_free(counts);
         dev_free(d_off);
+        hv_fix_list(s, st);
+        s->d_partial = dev_alloc<double>(size_t(2 * hv_partials(s)));
     });
 }
 
