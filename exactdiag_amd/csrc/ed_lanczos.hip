@@ -2,9 +2,10 @@
 //
 // The recurrence keeps its coefficients on the device: per step one fused
 // H·v (with the workgroup partials of <u, Hu>), a one-workgroup reduction
-// that writes a_n, one fused update w = s_n Hu - a_n s_n u_n - b_n s_{n-1}
-// u_{n-1} (with partials of ||w||^2) and a reduction that writes b_{n+1} and
-// s_{n+1} = 1 / b_{n+1}.  Vectors are never rescaled in place: u_n is the
+// that first writes b_n and s_n = 1 / b_n from the previous update's
+// partials and then a_n, and one fused update w = s_n Hu - a_n s_n u_n - b_n
+// s_{n-1} u_{n-1} (with partials of ||w||^2); b of the last step is reduced
+// after the loop.  Vectors are never rescaled in place: u_n is the
 // unnormalised Lanczos vector and s_n its scale, which saves a full pass per
 // step.  No host synchronisation inside the loop except a stop check every
 // 64 steps; with a stop rule (ed_hmat_lanczos_run_tol) the host reads the
@@ -26,12 +27,20 @@ namespace {
 
 constexpr int BS = 256;
 
+// One device block: flag (two ints in the first double), a, b, then s, so
+// that flag, a and b reach the host in one copy.
 struct LzDev {
     double* a;  // [nsteps]
     double* b;  // [nsteps + 1]
     double* s;  // [nsteps + 1]
     int* flag;  // [0] stopped, [1] count
 };
+
+inline size_t lz_doubles(int nsteps) { return size_t(1) + size_t(nsteps) + size_t(nsteps + 1) + size_t(nsteps + 1); }
+
+inline LzDev lz_view(double* block, int nsteps) {
+    return LzDev{block + 1, block + 1 + nsteps, block + 2 + 2 * size_t(nsteps), reinterpret_cast<int*>(block)};
+}
 
 // Vector elements: complex (double2, float2) or real (double, float; the
 // imaginary part reads as zero and is not stored).
@@ -111,15 +120,37 @@ __global__ void __launch_bounds__(RBS) lz_init_kernel(LzDev lz, const double* pa
     }
 }
 
-__global__ void __launch_bounds__(RBS) lz_reduce_a_kernel(LzDev lz, const double* partial, int64_t n, int step) {
+// b_step and s_step from the update partials of step - 1 (step > 0), as
+// lz_reduce_b_kernel; then, unless the space is exhausted, a_step from the
+// product's partials.
+__global__ void __launch_bounds__(RBS) lz_reduce_ba_kernel(LzDev lz, const double* pupd, int64_t nupd,
+                                                           const double* partial, int64_t n, int step) {
     if (lz.flag[0]) return;
+    __shared__ double sn_shared;
+    __shared__ int stopped;
+    if (step > 0) {
+        double re = 0.0, im = 0.0;
+        for (int64_t k = threadIdx.x; k < nupd; k += RBS) re += pupd[2 * k];
+        block_sum2<RBS>(re, im);
+        if (threadIdx.x == 0) {
+            const double bn = std::sqrt(re);
+            lz.b[step] = bn;
+            lz.s[step] = 1.0 / bn;
+            lz.flag[1] = step;
+            stopped = bn < 1e-8;
+            if (stopped) lz.flag[0] = 1;
+            sn_shared = 1.0 / bn;
+        }
+    } else if (threadIdx.x == 0) {
+        stopped = 0;
+        sn_shared = lz.s[0];
+    }
+    __syncthreads();
+    if (stopped) return;
     double re, im;
     sum_partials<RBS>(partial, n, re, im);
     block_sum2<RBS>(re, im);
-    if (threadIdx.x == 0) {
-        const double sn = lz.s[step];
-        lz.a[step] = sn * sn * re;
-    }
+    if (threadIdx.x == 0) lz.a[step] = sn_shared * sn_shared * re;
 }
 
 template <class VT>
@@ -132,7 +163,38 @@ __global__ void __launch_bounds__(BS) lz_update_kernel(LzDev lz, int step, int64
     const double snm = step > 0 ? lz.s[step - 1] : 0.0;
     const double c = acc ? ritz[step] * sn : 0.0;
     double n2 = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * BS + threadIdx.x; i < D; i += int64_t(gridDim.x) * BS) {
+    int64_t first = 0;  // elements before `first` were done 16 bytes at a time
+    if (!basis && !acc) {
+        // the plain recurrence: E elements per 16-byte load and store (vectors are 256-byte aligned)
+        constexpr int E = int(16 / sizeof(VT));
+        const int64_t nv = D / E;
+        const uint4* y4 = reinterpret_cast<const uint4*>(y);
+        const uint4* u4 = reinterpret_cast<const uint4*>(u);
+        uint4* p4 = reinterpret_cast<uint4*>(up);
+        for (int64_t v = int64_t(blockIdx.x) * BS + threadIdx.x; v < nv; v += int64_t(gridDim.x) * BS) {
+            const uint4 yv = y4[v], uv = u4[v], pv = p4[v];
+            const VT* ye = reinterpret_cast<const VT*>(&yv);
+            const VT* ue = reinterpret_cast<const VT*>(&uv);
+            const VT* pe = reinterpret_cast<const VT*>(&pv);
+            uint4 wv;
+            VT* we = reinterpret_cast<VT*>(&wv);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                double yr, yi, ur, ui, pr, pi;
+                ldc(ye, e, yr, yi);
+                ldc(ue, e, ur, ui);
+                ldc(pe, e, pr, pi);
+                const double wr = sn * yr - an * (sn * ur) - bn * (snm * pr);
+                const double wi = sn * yi - an * (sn * ui) - bn * (snm * pi);
+                stc(we, e, wr, wi);
+                const double r2 = rounded<VT>(wr), i2 = is_real_vec<VT>::value ? 0.0 : rounded<VT>(wi);
+                n2 += r2 * r2 + i2 * i2;
+            }
+            p4[v] = wv;
+        }
+        first = nv * E;
+    }
+    for (int64_t i = first + int64_t(blockIdx.x) * BS + threadIdx.x; i < D; i += int64_t(gridDim.x) * BS) {
         double yr, yi, ur, ui, pr, pi;
         ldc(y, i, yr, yi);
         ldc(u, i, ur, ui);
@@ -348,9 +410,11 @@ void terminated_spectrum(const double* a, const double* b, int n, const std::vec
     for (size_t k = 0; k < m; ++k) out[k] = -di[k] / (dr[k] * dr[k] + di[k] * di[k]);
 }
 
-// Relative L1 distance between the terminated spectra of n and n - delta
-// levels on the rule's strided grid (weights: central differences of the
-// grid, one-sided at its ends).
+// The larger relative L1 distance between the terminated spectrum of n
+// levels and those of n - delta and n - delta / 2 levels, on the rule's
+// strided grid (weights: central differences of the grid, one-sided at its
+// ends).  The shorter comparison catches a spectrum that passes close to
+// its n - delta form while still moving.
 double spectrum_change(const double* a, const double* b, int n, int delta, const ed_lanczos_stop& r) {
     std::vector<double> w;
     for (int k = 0; k < r.n_w; k += r.stride) w.push_back(r.ws[k] + r.e0);
@@ -363,14 +427,18 @@ double spectrum_change(const double* a, const double* b, int n, int delta, const
     }
     std::vector<double> s1, s2;
     terminated_spectrum(a, b, n, w, r.eta, s1);
-    terminated_spectrum(a, b, n - delta, w, r.eta, s2);
-    double num = 0.0, den = 0.0;
-    for (size_t k = 0; k < m; ++k) {
-        num += std::fabs(s1[k] - s2[k]) * dw[k];
-        den += std::fabs(s2[k]) * dw[k];
+    double worst = 0.0;
+    for (const int back : {delta, delta / 2}) {
+        if (back < 1) continue;
+        terminated_spectrum(a, b, n - back, w, r.eta, s2);
+        double num = 0.0, den = 0.0;
+        for (size_t k = 0; k < m; ++k) {
+            num += std::fabs(s1[k] - s2[k]) * dw[k];
+            den += std::fabs(s2[k]) * dw[k];
+        }
+        worst = std::max(worst, den == 0.0 ? (num == 0.0 ? 0.0 : HUGE_VAL) : num / den);
     }
-    if (den == 0.0) return num == 0.0 ? 0.0 : HUGE_VAL;
-    return num / den;
+    return worst;
 }
 
 void check_stop_rule(const ed_lanczos_stop* r) {
@@ -395,28 +463,27 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
     const HvWeights hw = make_weights(w);
     VT* buf[3] = {dev_alloc<VT>(size_t(D)), dev_alloc<VT>(size_t(D)), dev_alloc<VT>(size_t(D))};
     double* partial = dev_alloc<double>(size_t(2 * std::max<int64_t>(nwg, hv_partials(s))));
-    LzDev lz{dev_alloc<double>(size_t(nsteps)), dev_alloc<double>(size_t(nsteps + 1)),
-             dev_alloc<double>(size_t(nsteps + 1)), dev_alloc<int>(2)};
+    double* pupd = dev_alloc<double>(size_t(2 * nupd));  // the update's partials, kept until the next step
+    double* lz_block = dev_alloc<double>(lz_doubles(nsteps));
+    const LzDev lz = lz_view(lz_block, nsteps);
     double* d_ritz = nullptr;
     // the stop rule's two pinned host copies of the coefficients and the flag, one per chunk in flight
     struct Slot {
-        double* a = nullptr;
-        double* b = nullptr;
-        int* flag = nullptr;
+        double* block = nullptr;  // host copy of flag, a and b (the layout of lz_view)
+        const double* a = nullptr;
+        const double* b = nullptr;
+        const int* flag = nullptr;
         hipEvent_t ev = nullptr;
     } slot[2];
     auto cleanup = [&] {
         if (stop) (void)hipStreamSynchronize(st);
         for (auto* p : buf) dev_free(p);
         dev_free(partial);
-        dev_free(lz.a);
-        dev_free(lz.b);
-        dev_free(lz.s);
-        dev_free(lz.flag);
+        dev_free(pupd);
+        dev_free(lz_block);
         dev_free(d_ritz);
         for (Slot& sl : slot) {
-            for (void* p : {(void*)sl.a, (void*)sl.b, (void*)sl.flag})
-                if (p) (void)hipHostFree(p);
+            if (sl.block) (void)hipHostFree(sl.block);
             if (sl.ev) (void)hipEventDestroy(sl.ev);
         }
     };
@@ -443,10 +510,9 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         auto launch_step = [&](int n) {
             const int64_t np =
                 m ? launch_hmat(m, u, y, partial, precision, st) : launch_hv(s, hw, u, y, partial, precision, 1, st);
-            hipLaunchKernelGGL(lz_reduce_a_kernel, dim3(1), dim3(RBS), 0, st, lz, partial, np, n);
+            hipLaunchKernelGGL(lz_reduce_ba_kernel, dim3(1), dim3(RBS), 0, st, lz, pupd, nupd, partial, np, n);
             hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nupd)), dim3(BS), 0, st, lz, n, D, y, u, up,
-                               d_basis ? d_basis + int64_t(n) * D : nullptr, acc, d_ritz, partial);
-            hipLaunchKernelGGL(lz_reduce_b_kernel, dim3(1), dim3(RBS), 0, st, lz, partial, nupd, n);
+                               d_basis ? d_basis + int64_t(n) * D : nullptr, acc, d_ritz, pupd);
             ED_HIP(hipGetLastError());
             VT* t = up;  // w (u_{n+1}) now lives in up
             up = u;
@@ -455,29 +521,30 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
         if (stop) {
             // chunks of check_every steps; the host checks the coefficients of chunk c (copied to pinned memory)
             // while the GPU runs chunk c + 1
+            const size_t head = size_t(2 * nsteps + 2);  // flag, a and b: the first doubles of the block
             for (Slot& sl : slot) {
-                ED_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.a), size_t(nsteps) * sizeof(double)));
-                ED_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.b), size_t(nsteps + 1) * sizeof(double)));
-                ED_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.flag), 2 * sizeof(int)));
+                ED_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.block), head * sizeof(double)));
+                const LzDev h = lz_view(sl.block, nsteps);
+                sl.a = h.a;
+                sl.b = h.b;
+                sl.flag = h.flag;
                 ED_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
             }
-            auto post = [&](Slot& sl, int upto) {
-                ED_HIP(hipMemcpyAsync(sl.a, lz.a, size_t(upto) * sizeof(double), hipMemcpyDeviceToHost, st));
-                ED_HIP(hipMemcpyAsync(sl.b, lz.b, size_t(upto + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
-                ED_HIP(hipMemcpyAsync(sl.flag, lz.flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+            auto post = [&](Slot& sl) {
+                ED_HIP(hipMemcpyAsync(sl.block, lz_block, head * sizeof(double), hipMemcpyDeviceToHost, st));
                 ED_HIP(hipEventRecord(sl.ev, st));
             };
             int n = 0;
             const int first = std::min(std::max(stop->min_steps, stop->delta + 1), nsteps);
             while (n < first) launch_step(n++);
-            post(slot[0], n);
+            post(slot[0]);
             int cur = 0, result = 0, why = 0;
             for (;;) {
                 const int pend = n;  // steps that slot[cur] covers
                 if (n < nsteps) {
                     const int next = std::min(n + stop->check_every, nsteps);
                     while (n < next) launch_step(n++);
-                    post(slot[1 - cur], n);
+                    post(slot[1 - cur]);
                 }
                 ED_HIP(hipEventSynchronize(slot[cur].ev));
                 const Slot& sl = slot[cur];
@@ -504,13 +571,19 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
             cleanup();
             return;
         }
+        int last = -1;
         for (int n = 0; n < nsteps; ++n) {
             launch_step(n);
+            last = n;
             if ((n & 63) == 63) {
                 ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
                 ED_HIP(hipStreamSynchronize(st));
                 if (flag[0]) break;
             }
+        }
+        if (last >= 0) {  // b of the last step
+            hipLaunchKernelGGL(lz_reduce_b_kernel, dim3(1), dim3(RBS), 0, st, lz, pupd, nupd, last);
+            ED_HIP(hipGetLastError());
         }
         ED_HIP(hipMemcpyAsync(flag, lz.flag, sizeof(flag), hipMemcpyDeviceToHost, st));
         ED_HIP(hipStreamSynchronize(st));
