@@ -31,7 +31,7 @@ EXPORTED = [
     "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune", "ed_lanczos_ritz",
     "ed_hmat_create", "ed_hmat_destroy", "ed_hmat_info", "ed_hmat_hv", "ed_hmat_lanczos_run", "ed_hmat_lanczos_ritz",
     "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_hmat_is_real", "ed_batch_create_stored",
-    "ed_hmat_lanczos_run_tol", "ed_spectrum_change",
+    "ed_hmat_lanczos_run_tol", "ed_spectrum_change", "ed_hmat_lanczos_run_basis",
     "ed_csr_from_sector", "ed_csr_from_operator", "ed_csr_create", "ed_csr_info", "ed_csr_copy_out", "ed_csr_destroy",
     "ed_csr_sum_create", "ed_csr_sum_info", "ed_csr_sum_hv", "ed_csr_sum_destroy",
     "ed_position_op_columns", "ed_position_op_apply", "ed_position_expectation", "ed_position_correlations",
@@ -141,6 +141,7 @@ class LanczosStop(ct.Structure):
 
 _SIGNATURES["ed_hmat_lanczos_run_tol"] = (ct.c_int, [_vp, _vp, ct.c_int, ct.POINTER(LanczosStop), _dp, _dp, _ip, _ip,
                                                      ct.c_int, _vp])
+_SIGNATURES["ed_hmat_lanczos_run_basis"] = (ct.c_int, [_vp, _vp, ct.c_int, _dp, _dp, _ip, _vp, _i64, ct.c_int, _vp])
 _SIGNATURES["ed_spectrum_change"] = (ct.c_int, [_dp, _dp, ct.c_int, ct.c_int, ct.POINTER(LanczosStop), _dp])
 
 _lib = None

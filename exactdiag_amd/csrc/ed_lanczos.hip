@@ -455,7 +455,7 @@ template <class VT>
 void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int nsteps, double* a, double* b,
                    int* n_out, VT* d_basis, int precision, hipStream_t st, double2* acc = nullptr,
                    const double* ritz = nullptr, const ed_hmat* m = nullptr, const ed_lanczos_stop* stop = nullptr,
-                   int* reason = nullptr) {
+                   int* reason = nullptr, int64_t basis_ld = 0) {
     if constexpr (is_real_vec<VT>::value) ED_REQUIRE(m, ED_ERR_ARG, "real vectors need a stored matrix");
     const int64_t D = s->D;
     const int64_t nwg = (D + BS - 1) / BS;
@@ -512,7 +512,8 @@ void lanczos_run_t(const ed_sector* s, const double* w, const double* d_v0, int 
                 m ? launch_hmat(m, u, y, partial, precision, st) : launch_hv(s, hw, u, y, partial, precision, 1, st);
             hipLaunchKernelGGL(lz_reduce_ba_kernel, dim3(1), dim3(RBS), 0, st, lz, pupd, nupd, partial, np, n);
             hipLaunchKernelGGL((lz_update_kernel<VT>), dim3(unsigned(nupd)), dim3(BS), 0, st, lz, n, D, y, u, up,
-                               d_basis ? d_basis + int64_t(n) * D : nullptr, acc, d_ritz, pupd);
+                               d_basis ? d_basis + int64_t(n) * (basis_ld > 0 ? basis_ld : D) : nullptr, acc, d_ritz,
+                               pupd);
             ED_HIP(hipGetLastError());
             VT* t = up;  // w (u_{n+1}) now lives in up
             up = u;
@@ -666,6 +667,26 @@ int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double
         hmat_lanczos(precision, [&](auto vt) {
             using VT = decltype(vt);
             lanczos_run_t<VT>(s, w0, d_v0, nsteps, a, b, n_out, nullptr, precision, st, nullptr, nullptr, m);
+        });
+    });
+}
+
+int ed_hmat_lanczos_run_basis(const ed_hmat* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
+                              void* d_basis, int64_t ld, int precision, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(m, ED_ERR_ARG, "null matrix");
+        const ed_sector* s = hmat_sector(m);
+        ED_REQUIRE(nsteps >= 1 && a && b && n_out && d_basis && ld >= s->D, ED_ERR_ARG,
+                   "bad Lanczos arguments (the basis needs nsteps rows of ld >= D elements)");
+        hmat_require_precision(m, precision);
+        *n_out = 0;
+        if (s->D == 0) return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const double w0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        hmat_lanczos(precision, [&](auto vt) {
+            using VT = decltype(vt);
+            lanczos_run_t<VT>(s, w0, d_v0, nsteps, a, b, n_out, static_cast<VT*>(d_basis), precision, st, nullptr,
+                              nullptr, m, nullptr, nullptr, ld);
         });
     });
 }

@@ -63,14 +63,19 @@ def _shortest_converged_prefix(a, b, tol, stride=8):
     return lowest(n)[1][:, 0]
 
 
+BASIS_BYTES = 16 << 30  # HBM for the pass-one Lanczos vectors of two_pass_ground_state (16 GiB of the 288 GB)
+
+
 def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=30, seed=0, real=False):
     """Lowest eigenpair of a stored H by restarted two-pass Lanczos in the given precision.
 
     Pass one gives the coefficients; the shortest leading tridiagonal whose Ritz residual estimate meets the
     tolerance gives the lowest eigenvector y; pass two reruns that many steps of the recurrence and accumulates
     x = sum_n y_n v_n in FP64; restart from x until ||H x - E x|| <= tol * max(1, |E|), E the FP64 Rayleigh
-    quotient.  With real=True and a real stored H (hm.is_real) the vectors are real (float64 for precision 0,
-    float32 for 1) from a real start vector; x is still returned as complex128.
+    quotient.  In FP64 (precision 0), when the pass-one vectors fit in BASIS_BYTES of HBM, pass one keeps them
+    (ed_hmat_lanczos_run_basis) and x is their combination instead of a second pass.  With real=True and a
+    real stored H (hm.is_real) the vectors are real (float64 for precision 0, float32 for 1) from a real start
+    vector; x is still returned as complex128.
     Returns (E, x, restarts, residual).
     """
     import torch
@@ -83,11 +88,28 @@ def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=3
     if real:
         v = v.real.to(torch.complex128)
         precision = PREC_R64 if precision == 0 else PREC_R32
+    n1 = min(nsteps, dim)
+    Dp = dim + (dim & 1)  # real rows padded to an even length, read as complex pairs by vec_combine
+    keep = precision in (0, PREC_R64) and n1 * (Dp * 8 if real else dim * 16) <= BASIS_BYTES
     res = np.inf
     for restart in range(max_restarts):
-        a, b = hm.lanczos(v, min(nsteps, dim), precision=precision)
-        y = _shortest_converged_prefix(a, b, tol)
-        x = hm.lanczos_ritz(v, y, precision=precision)
+        if keep:
+            a, b, V = hm.lanczos_basis(v, n1, precision=precision, ld=Dp if real else dim)
+            y = _shortest_converged_prefix(a, b, tol)
+            k = len(y)
+            if real:
+                Vc = torch.view_as_complex(V[:k].view(k, Dp // 2, 2))
+                xc = torch.empty(Dp // 2, dtype=torch.complex128, device=V.device)
+                krylov.vec_combine(Vc, k, y.astype(np.complex128), xc)  # real weights: the real combination
+                x = torch.view_as_real(xc).reshape(-1)[:dim].to(torch.complex128)
+            else:
+                x = torch.empty(dim, dtype=torch.complex128, device=V.device)
+                krylov.vec_combine(V, k, y, x)
+            del V
+        else:
+            a, b = hm.lanczos(v, n1, precision=precision)
+            y = _shortest_converged_prefix(a, b, tol)
+            x = hm.lanczos_ritz(v, y, precision=precision)
         krylov.vec_scale(x, 1.0 / krylov.vec_norm(x))
         if real:
             xr = x.real.contiguous()
@@ -100,7 +122,7 @@ def two_pass_ground_state(hm, precision=0, nsteps=150, tol=1e-10, max_restarts=3
             r = hx.clone()
             krylov.vec_axpys(x.view(1, -1), 1, [e], r)
             res = krylov.vec_norm(r)
-        if res <= tol * max(1.0, abs(e)) or len(a) < min(nsteps, dim):
+        if res <= tol * max(1.0, abs(e)) or len(a) < n1:
             return e, x, restart, res
         v = x
     raise RuntimeError(f"two-pass ground state: residual {res:.3e} above {tol:.1e} x max(1, |E|) after "

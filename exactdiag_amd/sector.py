@@ -376,6 +376,24 @@ class StoredH:
         check(lib().ed_hmat_lanczos_ritz(self._h, _ptr(v0), len(r), rp, _ptr(x), int(precision), _stream()))
         return x
 
+    def lanczos_basis(self, v0, nsteps: int, precision: int = 0, ld: int | None = None):
+        """(a, b, V): the coefficients of :meth:`lanczos` and the normalised Lanczos vectors as the first rows of V,
+        a device tensor (nsteps, ld) of the precision's vector type (ld >= dim; columns past dim stay zero)."""
+        torch = _torch()
+        dt = {PREC_C128: torch.complex128, PREC_C64: torch.complex64, PREC_R64: torch.float64,
+              PREC_R32: torch.float32}[precision]
+        ld = self.dim if ld is None else int(ld)
+        v0 = v0.to(torch.complex128).contiguous()
+        V = torch.zeros((nsteps, ld), dtype=dt, device=self.device)
+        a = np.zeros(nsteps, np.float64)
+        b = np.zeros(nsteps + 1, np.float64)
+        n = ct.c_int(0)
+        check(lib().ed_hmat_lanczos_run_basis(self._h, _ptr(v0), int(nsteps), a.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                              b.ctypes.data_as(ct.POINTER(ct.c_double)), ct.byref(n), _ptr(V), ld,
+                                              int(precision), _stream()))
+        k = n.value
+        return a[:k].copy(), b[:k].copy(), V
+
     def lanczos_to_tol(self, v0, max_steps: int, rule: "StopRule", precision: int = 0):
         """Lanczos coefficients from v0 until the terminated spectrum changes by less than rule.tol over
         rule.delta steps (ed_hmat_lanczos_run_tol).  Returns (a, b, reason): reason 0 = max_steps reached,
@@ -396,8 +414,8 @@ class StoredH:
 class StopRule:
     """The stop rule of :meth:`StoredH.lanczos_to_tol` (struct ed_lanczos_stop): from min_steps on, every
     check_every steps, the spectrum of the first n levels closed by the square-root terminator is compared with
-    that of the first n - delta on every stride-th point of ws (z = w + e0 + i eta); the run stops when their
-    relative L1 distance falls below tol."""
+    those of the first n - delta and n - delta // 2 on every stride-th point of ws (z = w + e0 + i eta); the
+    run stops when the larger relative L1 distance falls below tol."""
 
     def __init__(self, ws, e0, eta, tol, min_steps=60, check_every=5, delta=20, max_points=1024):
         from exactdiag_amd._lib import LanczosStop

@@ -225,14 +225,24 @@ int ed_hmat_lanczos_run(const ed_hmat* m, const double* d_v0, int nsteps, double
 int ed_hmat_lanczos_ritz(const ed_hmat* m, const double* d_v0, int nsteps, const double* ritz, double* d_x,
                          int precision, void* stream);
 
+/* ed_hmat_lanczos_run that also keeps the normalised Lanczos vectors: row n
+ * of d_basis (device, rows of ld >= D elements of the precision's vector
+ * type) receives v_n, so that a Ritz vector is one combination of the rows
+ * instead of the second pass of ed_hmat_lanczos_ritz.  No counterpart in the
+ * reference (scipy's ARPACK keeps its own basis,
+ * general/lanczos_diagonalization.py:81-121); meant for sectors whose nsteps
+ * vectors fit in HBM beside the matrix. */
+int ed_hmat_lanczos_run_basis(const ed_hmat* m, const double* d_v0, int nsteps, double* a, double* b, int* n_out,
+                              void* d_basis, int64_t ld, int precision, void* stream);
+
 /* Stop rule of a time-to-accuracy Lanczos run (no counterpart in the
  * reference, whose lanczos_dynamical_response runs a fixed number of steps,
  * general/cython_lanczos_diagonalization.pyx:18-77).  From min_steps on,
  * after every check_every steps, the spectrum of the first n coefficients
  * closed by the square-root terminator (ed_spectrum_change) is compared
- * with that of the first n - delta; the run stops at the first n where
- * their relative L1 distance on the grid ws[0], ws[stride], ... (z = w + e0
- * + i eta) is below tol. */
+ * with those of the first n - delta and n - delta / 2; the run stops at the
+ * first n where the larger relative L1 distance on the grid ws[0],
+ * ws[stride], ... (z = w + e0 + i eta) is below tol. */
 typedef struct ed_lanczos_stop {
     const double* ws; /* host grid of n_w frequencies */
     int n_w;
@@ -248,9 +258,10 @@ typedef struct ed_lanczos_stop {
  * exhausted (b below 1e-8; then no terminator applies). */
 int ed_hmat_lanczos_run_tol(const ed_hmat* m, const double* d_v0, int max_steps, const ed_lanczos_stop* stop,
                             double* a, double* b, int* n_out, int* reason, int precision, void* stream);
-/* The rule's estimate on the host (no GPU): relative L1 distance between the
- * terminated spectra of the first n and the first n - delta coefficients
- * (a[0..n), b[0..n), b[0] unused).  Each terminator takes the centre and
+/* The rule's estimate on the host (no GPU): the larger relative L1 distance
+ * between the terminated spectrum of the first n coefficients and those of
+ * the first n - delta and n - delta / 2 (a[0..n), b[0..n), b[0] unused).
+ * Each terminator takes the centre and
  * half-width w of the extreme eigenvalues of its tridiagonal (bisection),
  * a_inf = centre, b_inf = w / 2; the L1 weights are the grid's central
  * differences. */
