@@ -21,7 +21,7 @@ POS_HOLES, POS_SZ, POS_SINGLET = range(3)
 PREC_C128, PREC_C64, PREC_R64, PREC_R32 = range(4)
 
 EXPORTED = [
-    "ed_last_error", "ed_version", "ed_sector_create", "ed_sector_create_flip", "ed_sector_flip", "ed_sector_destroy",
+    "ed_last_error", "ed_version", "ed_cache_release", "ed_sector_create", "ed_sector_create_flip", "ed_sector_flip", "ed_sector_destroy",
     "ed_basis_build", "ed_basis_size",
     "ed_basis_export", "ed_lookup", "ed_lowest", "ed_term_columns", "ed_op_columns", "ed_hv", "ed_op_apply",
     "ed_lanczos_run", "ed_cf_eval", "ed_cf_eval_tail", "ed_vec_dots", "ed_vec_dots_noalloc", "ed_vec_axpys", "ed_vec_combine", "ed_vec_norm2", "ed_vec_scale",
@@ -48,6 +48,7 @@ _ip = ct.POINTER(ct.c_int)
 _SIGNATURES = {
     "ed_last_error": (ct.c_char_p, []),
     "ed_version": (ct.c_int, []),
+    "ed_cache_release": (ct.c_int, []),
     "ed_sector_create": (ct.c_int, [ct.c_int] * 5 + [ct.POINTER(_vp)]),
     "ed_sector_create_flip": (ct.c_int, [ct.c_int] * 6 + [ct.POINTER(_vp)]),
     "ed_sector_flip": (ct.c_int, [_vp]),
@@ -169,6 +170,11 @@ def lib():
             fn.argtypes = args
         _lib = handle
     return _lib
+
+
+def release_cached_memory() -> None:
+    """Return the device blocks the library keeps for reuse to HIP (ed_cache_release)."""
+    check(lib().ed_cache_release())
 
 
 def check(rc: int) -> None:

@@ -6,7 +6,11 @@
 
 #include <complex>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "ed_common.h"
@@ -116,12 +120,89 @@ int guarded(F&& f) {
     }
 }
 
+// Device memory of the library's buffers.  Freeing a block of tens of MB
+// with hipFree costs about 0.25 ms whatever its size, and one spectrum of a
+// large sector frees a dozen such blocks (basis arrays, stored-H arrays and
+// their build scratch, Lanczos vectors); so blocks of at least
+// DEV_CACHE_MIN bytes are kept after dev_free, per device and up to
+// DEV_CACHE_CAP bytes, and a later request takes the smallest kept block of
+// its device that is at least as large and at most 1/8 larger.  dev_free
+// first waits for the device, as hipFree does, so no queued work still
+// uses a block when it is handed out again.  When hipMalloc fails, the kept
+// blocks of the device are released and the allocation is retried;
+// ed_cache_release returns them on request.
+constexpr size_t DEV_CACHE_MIN = size_t(64) << 10;
+constexpr size_t DEV_CACHE_CAP = size_t(24) << 30;
+
+struct DevCache {
+    std::mutex mu;
+    std::unordered_map<void*, std::pair<int, size_t>> live;  // block -> (device, bytes)
+    std::multimap<size_t, std::pair<int, void*>> idle;       // bytes -> (device, block)
+    size_t idle_bytes = 0;
+};
+
+inline DevCache& dev_cache() {
+    static DevCache c;
+    return c;
+}
+
+// Release the kept blocks of device `dev` (all devices when dev < 0).
+inline void dev_cache_release(int dev) {
+    DevCache& c = dev_cache();
+    std::vector<void*> out;
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        for (auto it = c.idle.begin(); it != c.idle.end();) {
+            if (dev < 0 || it->second.first == dev) {
+                out.push_back(it->second.second);
+                c.idle_bytes -= it->first;
+                it = c.idle.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    for (void* p : out) (void)hipFree(p);
+}
+
+inline void* dev_alloc_bytes(size_t n) {
+    n = n ? n : 1;
+    void* p = nullptr;
+    if (n < DEV_CACHE_MIN) {
+        ED_HIP(hipMalloc(&p, n));
+        return p;
+    }
+    const size_t grain = size_t(2) << 20;
+    const size_t want = n < grain ? (n + 4095) / 4096 * 4096 : (n + grain - 1) / grain * grain;
+    int dev = 0;
+    ED_HIP(hipGetDevice(&dev));
+    DevCache& c = dev_cache();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        for (auto it = c.idle.lower_bound(want); it != c.idle.end() && it->first <= want + want / 8; ++it) {
+            if (it->second.first != dev) continue;
+            p = it->second.second;
+            c.live.emplace(p, std::make_pair(dev, it->first));
+            c.idle_bytes -= it->first;
+            c.idle.erase(it);
+            return p;
+        }
+    }
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        dev_cache_release(dev);
+        e = hipMalloc(&p, want);
+    }
+    ED_HIP(e);
+    std::lock_guard<std::mutex> g(c.mu);
+    c.live.emplace(p, std::make_pair(dev, want));
+    return p;
+}
+
 template <class T>
 T* dev_alloc(size_t n) {
-    void* p = nullptr;
-    if (n == 0) n = 1;
-    ED_HIP(hipMalloc(&p, n * sizeof(T)));
-    return static_cast<T*>(p);
+    return static_cast<T*>(dev_alloc_bytes((n ? n : 1) * sizeof(T)));
 }
 
 template <class T>
@@ -160,7 +241,29 @@ __device__ inline void workgroup_sum2(double& re, double& im) {
 }
 
 inline void dev_free(void* p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    DevCache& c = dev_cache();
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    std::unique_lock<std::mutex> g(c.mu);
+    const auto it = c.live.find(p);
+    if (it == c.live.end()) {  // a small block, not kept
+        g.unlock();
+        (void)hipFree(p);
+        return;
+    }
+    const int owner = it->second.first;
+    const size_t bytes = it->second.second;
+    c.live.erase(it);
+    const bool keep = owner == dev && c.idle_bytes + bytes <= DEV_CACHE_CAP;
+    g.unlock();
+    if (!keep || hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(p);
+        return;
+    }
+    g.lock();
+    c.idle.emplace(bytes, std::make_pair(owner, p));
+    c.idle_bytes += bytes;
 }
 
 // Combination list of one rule: reversed site pairs (or single sites) with

@@ -442,6 +442,14 @@ const char* ed_last_error(void) { return ed::g_last_error.c_str(); }
 
 int ed_version(void) { return 1; }
 
+int ed_cache_release(void) {
+    return guarded([&] {
+        int dev = 0;
+        ED_HIP(hipGetDevice(&dev));
+        dev_cache_release(dev);
+    });
+}
+
 }  // extern "C"
 
 namespace ed {

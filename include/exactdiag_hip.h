@@ -62,6 +62,11 @@ enum ed_position_kind {
 
 const char* ed_last_error(void);
 int ed_version(void);
+/* The library keeps the device blocks of 64 KiB or more that it frees (up to
+ * 24 GiB per device) for its next allocations, since hipFree of a large
+ * block costs about 0.25 ms; this returns the kept blocks of the current
+ * device to HIP.  No counterpart in the reference (CPU memory). */
+int ed_cache_release(void);
 
 /* Sector = (num_rungs L, num_holes h, num_down_spins nd, k = (kx, ky)).
  * Replaces Symmetries_Single_Spin_Half_2leg_Ladder + State_Index_Amplitude_
