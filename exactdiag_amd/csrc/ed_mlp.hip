@@ -79,6 +79,10 @@ __global__ void __launch_bounds__(64) gemm_bf16_kernel(int M, int N, int K, cons
 // cross the matrix edge are read element by element with zero fill; the host launches this kernel only when
 // the operands are 16-byte aligned and their other stride is a multiple of 8.
 constexpr int GBM = 128, GBN = 128, GBK = 32, GLDK = GBK + 8;
+// LDS offset of tile row r: pitch GLDK plus 8 elements (16 bytes) after every 8 rows, so that the 16 lanes that
+// store 8 rows apart land on 8 different banks instead of one (rows 8 apart at pitch 40 share a bank)
+__device__ inline int lrow(int r) { return r * GLDK + (r >> 3) * 8; }
+constexpr int GLDS = GBM * GLDK + (GBM / 8) * 8;
 
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
@@ -96,8 +100,8 @@ __global__ void __launch_bounds__(256) gemm_bf16_tiled_kernel(int M, int N, int 
                                                               const uint16_t* __restrict__ B, int64_t sbk,
                                                               int64_t sbn, float* __restrict__ C, int64_t ldc,
                                                               int accumulate) {
-    __shared__ __align__(16) uint16_t As[GBM * GLDK];
-    __shared__ __align__(16) uint16_t Bs[GBN * GLDK];
+    __shared__ __align__(16) uint16_t As[GLDS];
+    __shared__ __align__(16) uint16_t Bs[GLDS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
@@ -109,6 +113,10 @@ __global__ void __launch_bounds__(256) gemm_bf16_tiled_kernel(int M, int N, int 
     u16x8 ra[2], rb[2];
     // chunk c of 512 per operand and step: k-contiguous tiles are 4 chunks per row of 32 k (row = c / 4,
     // k = 8 (c % 4)); m- or n-contiguous tiles are 16 chunks per k (k = c / 16, row = 8 (c % 16))
+    // A k-contiguous operand: chunk c of 512 per step holds 8 consecutive k of row c / 4 (k = 8 (c % 4)),
+    // two chunks per thread.  Otherwise thread t holds rows 8 (t % 16) .. + 7 at the two k 2 (t / 16) and
+    // 2 (t / 16) + 1 (two 16-byte loads along the contiguous dimension) and stores each row's pair of k as
+    // one 4-byte word, so the tile reaches LDS transposed (k contiguous) in eight writes per thread.
     auto fetch = [&](int k0) {
         for (int i = 0; i < 2; ++i) {
             const int c = tid + 256 * i;
@@ -116,33 +124,39 @@ __global__ void __launch_bounds__(256) gemm_bf16_tiled_kernel(int M, int N, int 
                 const int m = m0 + c / 4, k = k0 + 8 * (c % 4);
                 ra[i] = load8(A + int64_t(m) * sam + k, K - k, m < M && k < K);
             } else {
-                const int k = k0 + c / 16, m = m0 + 8 * (c % 16);
+                const int k = k0 + 2 * (tid / 16) + i, m = m0 + 8 * (tid % 16);
                 ra[i] = load8(A + int64_t(k) * sak + m, M - m, k < K && m < M);
             }
             if constexpr (B_K) {
                 const int n = n0 + c / 4, k = k0 + 8 * (c % 4);
                 rb[i] = load8(B + int64_t(n) * sbn + k, K - k, n < N && k < K);
             } else {
-                const int k = k0 + c / 16, n = n0 + 8 * (c % 16);
+                const int k = k0 + 2 * (tid / 16) + i, n = n0 + 8 * (tid % 16);
                 rb[i] = load8(B + int64_t(k) * sbk + n, N - n, k < K && n < N);
             }
         }
     };
+    auto stash_pairs = [&](uint16_t* S, const u16x8 (&v)[2]) {
+        const int k = 2 * (tid / 16), r = 8 * (tid % 16);
+        for (int e = 0; e < 8; ++e)
+            *reinterpret_cast<uint32_t*>(&S[lrow(r + e) + k]) = uint32_t(v[0][e]) | (uint32_t(v[1][e]) << 16);
+    };
     auto stash = [&]() {
-        for (int i = 0; i < 2; ++i) {
-            const int c = tid + 256 * i;
-            if constexpr (A_K) {
-                *reinterpret_cast<u16x8*>(&As[(c / 4) * GLDK + 8 * (c % 4)]) = ra[i];
-            } else {
-                const int k = c / 16, m = 8 * (c % 16);
-                for (int e = 0; e < 8; ++e) As[(m + e) * GLDK + k] = ra[i][e];
+        if constexpr (A_K) {
+            for (int i = 0; i < 2; ++i) {
+                const int c = tid + 256 * i;
+                *reinterpret_cast<u16x8*>(&As[lrow(c / 4) + 8 * (c % 4)]) = ra[i];
             }
-            if constexpr (B_K) {
-                *reinterpret_cast<u16x8*>(&Bs[(c / 4) * GLDK + 8 * (c % 4)]) = rb[i];
-            } else {
-                const int k = c / 16, n = 8 * (c % 16);
-                for (int e = 0; e < 8; ++e) Bs[(n + e) * GLDK + k] = rb[i][e];
+        } else {
+            stash_pairs(As, ra);
+        }
+        if constexpr (B_K) {
+            for (int i = 0; i < 2; ++i) {
+                const int c = tid + 256 * i;
+                *reinterpret_cast<u16x8*>(&Bs[lrow(c / 4) + 8 * (c % 4)]) = rb[i];
             }
+        } else {
+            stash_pairs(Bs, rb);
         }
     };
     fetch(0);
@@ -154,9 +168,9 @@ __global__ void __launch_bounds__(256) gemm_bf16_tiled_kernel(int M, int N, int 
             bf16x8 a[2], b[2];
             for (int t = 0; t < 2; ++t) {
                 a[t] = __builtin_bit_cast(bf16x8,
-                                          *reinterpret_cast<const u16x8*>(&As[(wm + 32 * t + r) * GLDK + kk + 8 * half]));
+                                          *reinterpret_cast<const u16x8*>(&As[lrow(wm + 32 * t + r) + kk + 8 * half]));
                 b[t] = __builtin_bit_cast(bf16x8,
-                                          *reinterpret_cast<const u16x8*>(&Bs[(wn + 32 * t + r) * GLDK + kk + 8 * half]));
+                                          *reinterpret_cast<const u16x8*>(&Bs[lrow(wn + 32 * t + r) + kk + 8 * half]));
             }
             for (int i = 0; i < 2; ++i)
                 for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
