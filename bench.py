@@ -209,7 +209,7 @@ ALLQ_TARGET = 1e-3
 
 def steps_summary(steps):
     v = list(steps.values())
-    return {"rule": "per q: stop when the terminated spectrum changes by < 1e-3 (relative L1) over 20 steps, "
+    return {"rule": "per q: stop when the terminated spectrum changes by < 1e-3 (relative L1) over both 10 and 20 steps, "
                     f"checked every 5 steps from step 60, at most {ALLQ_STEPS_CAP}",
             "mean": float(sum(v) / max(len(v), 1)), "max": max(v, default=0), "total": sum(v),
             "per_q": {f"{q[0]},{q[1]}": n for q, n in steps.items()}}

@@ -216,7 +216,8 @@ def allq_fast(L, h, nd, weights, ws, eta, nsteps, k0=(0, 0), limited=True, tol=R
     With real=True, sectors whose stored H is real (kx = 0, L/2) run in real vectors of the same precision
     (float64 for the ground state, float32 for the spectra) when the start vector is real up to a global
     phase.  With spectrum_tol, each q's run stops by the rule of :class:`StopRule` (its terminated spectrum
-    changes by less than spectrum_tol, relative L1, over 20 steps; checked every 5 steps from step 60) and
+    changes by less than spectrum_tol, relative L1, over both 10 and 20 steps; checked every 5 steps from
+    step 60) and
     nsteps is only the cap; the steps each q took are in LAST_FAST_STEPS.  With precision=0, terminator=False,
     two_pass_gs=False and real=False it is the plain method on the stored H.  Returns (seconds, spectra)."""
     from exactdiag_amd.closure import sqrt_terminator
