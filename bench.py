@@ -12,7 +12,8 @@ the timed region, as the reference builds its CSR matrices once before its
 Lanczos loop); the matrix-free kernel is timed too and reported under
 ``matrix_free``.  Prints one JSON line (metric, value, unit, steps, timing and config fields) plus
 ``roofline`` (the H·v kernel against the 8 TB/s HBM peak, timed with HIP
-events on the launch stream) and ``cpu_baseline`` (the reference's own
+events on the launch stream, and under ``read_rate`` the lines it moves per
+second against the device's measured rate for scattered 16-byte loads) and ``cpu_baseline`` (the reference's own
 sparse-dot kernels, from oracle/_ref, timed on the host's cores over the
 whole sector's stored matrix, and one q of the all-q run end to end).
 """
@@ -675,6 +676,35 @@ def run_form(form, args=None):
     print(json.dumps(line), flush=True)
 
 
+def read_rate(traffic_bytes, ms_per_launch):
+    """The device's own read ceiling for the stored product (ed_hmat_line_probe): 128-byte lines per second for
+    16-byte loads from a 128 MiB buffer (the size of the 2x12 sector's complex128 x), scattered one line per lane
+    as the far gathers are, and 1 KiB in a row per wavefront; against it the lines the product moves per launch
+    (its calibrated L2-miss traffic / 128 B) over its event time."""
+    import ctypes as ct
+
+    import torch
+
+    from exactdiag_amd._lib import check, lib
+
+    sp = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for name, scattered in (("scattered", 1), ("contiguous", 0)):
+        r = ct.c_double(0.0)
+        check(lib().ed_hmat_line_probe(scattered, 128 << 20, ct.byref(r), sp))
+        out[f"probe_{name}_lines_per_s"] = r.value
+        out[f"probe_{name}_GBs"] = r.value * 128 / 1e9
+    if traffic_bytes:
+        lines = traffic_bytes / 128.0
+        out["kernel_lines_per_launch"] = lines
+        out["kernel_lines_per_s"] = lines / (ms_per_launch * 1e-3)
+        out["frac_of_scattered_probe"] = out["kernel_lines_per_s"] / out["probe_scattered_lines_per_s"]
+    log(f"read rate: probe scattered {out['probe_scattered_GBs']:.0f} GB/s, contiguous "
+        f"{out['probe_contiguous_GBs']:.0f} GB/s; product {out.get('frac_of_scattered_probe', float('nan')):.3f} "
+        "of the scattered rate")
+    return out
+
+
 def time_steps(step, steps, warmup, stream):
     """W untimed steps, then exactly K steps between synchronisations; (wall s, HIP-event ms) of the K."""
     import torch
@@ -807,7 +837,8 @@ def main():
                      "kernel": "hm_spmv_kernel (+ reduce_partials_kernel), per launch (HIP events)",
                      "alg_bytes_per_state": 2 * s_bytes + 12, "event_ms_per_launch": ms_per_step,
                      "achieved_incl_matrix": alg_stored / (ms_per_step * 1e-3) / 1e9,
-                     "frac_incl_matrix": alg_stored / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "frac_incl_matrix": alg_stored / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "read_rate": read_rate(traffic_from_profiles("hm_spmv") if prec == 0 else None, ms_per_step)},
         "matrix_free": {"value": D * args.steps / wall_mf / 1e9, "unit": "Gstates/s",
                         "event_ms_per_launch": ev_mf / args.steps,
                         "frac": alg / (ev_mf / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -30,7 +30,7 @@ EXPORTED = [
     "ed_batch_create", "ed_batch_destroy", "ed_batch_offsets", "ed_batch_hv", "ed_batch_lanczos", "ed_batch_tune",
     "ed_batch_set_config", "ed_batch_get_config", "ed_hv_tune", "ed_lanczos_ritz",
     "ed_hmat_create", "ed_hmat_destroy", "ed_hmat_info", "ed_hmat_hv", "ed_hmat_lanczos_run", "ed_hmat_lanczos_ritz",
-    "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_element_bytes", "ed_hmat_is_real", "ed_batch_create_stored",
+    "ed_hmat_tune", "ed_hmat_set_config", "ed_hmat_line_probe", "ed_hmat_element_bytes", "ed_hmat_is_real", "ed_batch_create_stored",
     "ed_hmat_lanczos_run_tol", "ed_spectrum_change", "ed_hmat_lanczos_run_basis",
     "ed_csr_from_sector", "ed_csr_from_operator", "ed_csr_create", "ed_csr_info", "ed_csr_copy_out", "ed_csr_destroy",
     "ed_csr_sum_create", "ed_csr_sum_info", "ed_csr_sum_hv", "ed_csr_sum_destroy",
@@ -97,6 +97,7 @@ _SIGNATURES = {
     "ed_hmat_lanczos_run": (ct.c_int, [_vp, _vp, ct.c_int, _dp, _dp, _ip, ct.c_int, _vp]),
     "ed_hmat_lanczos_ritz": (ct.c_int, [_vp, _vp, ct.c_int, _dp, _vp, ct.c_int, _vp]),
     "ed_hmat_tune": (ct.c_int, [_vp, _vp, _vp, ct.c_int, _ip, _ip, _ip, _dp, _vp]),
+    "ed_hmat_line_probe": (ct.c_int, [ct.c_int, _i64, _dp, _vp]),
     "ed_hmat_set_config": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int]),
     "ed_hmat_element_bytes": (ct.c_int, [_vp]),
     "ed_hmat_is_real": (ct.c_int, [_vp]),

@@ -474,6 +474,41 @@ __global__ void __launch_bounds__(BS) hm_spmv_kernel(StoredView m, int64_t D, co
     }
 }
 
+// Read-rate probe, the ceiling the stored product is measured against: every
+// lane loads 16 bytes, either each from its own 128-byte line at a hashed
+// place of a buffer of mask + 1 elements (SCATTER, as the far gathers do) or
+// 64 consecutive elements per wavefront (eight whole lines).  Four loads are
+// in flight per lane; the sum only keeps the loads from being removed.
+__device__ inline uint32_t probe_hash(uint32_t a) {
+    a ^= a >> 16;
+    a *= 0x7feb352du;
+    a ^= a >> 15;
+    a *= 0x846ca68bu;
+    a ^= a >> 16;
+    return a;
+}
+
+template <bool SCATTER>
+__global__ void __launch_bounds__(BS) line_probe_kernel(const double2* __restrict__ x, uint32_t mask, int iters,
+                                                        double* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * BS + threadIdx.x) >> 6;
+    double acc = 0.0;
+    for (int it = 0; it < iters; it += 4) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t seed = wave * 1315423911u + uint32_t(it + u) * 2654435761u;
+            const uint32_t j = SCATTER ? (probe_hash(seed ^ (lane * 97u)) & mask)
+                                       : ((probe_hash(seed) & mask) & ~63u) + lane;
+            v[u] = x[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += v[u].x + v[u].y;
+    }
+    if (acc == 1.0e300) out[0] = acc;  // never true for the zeroed buffer
+}
+
 uint64_t bits(double v) {
     uint64_t u;
     std::memcpy(&u, &v, 8);
@@ -1024,6 +1059,60 @@ int ed_hmat_hv(const ed_hmat* m, const double* d_x, double* d_y, double* d_dot, 
         hipStream_t st = static_cast<hipStream_t>(stream);
         const int64_t np = launch_hmat(m, d_x, d_y, d_dot ? m->d_partial : nullptr, precision, st);
         if (d_dot) launch_reduce_partials(m->d_partial, np, d_dot, st);
+    });
+}
+
+int ed_hmat_line_probe(int scattered, int64_t span_bytes, double* lines_per_s, void* stream) {
+    return guarded([&] {
+        ED_REQUIRE(lines_per_s, ED_ERR_ARG, "null argument");
+        ED_REQUIRE(span_bytes >= (int64_t(1) << 20) && span_bytes <= (int64_t(1) << 32) &&
+                       (span_bytes & (span_bytes - 1)) == 0,
+                   ED_ERR_ARG, "span_bytes must be a power of two in [1 MiB, 4 GiB]");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const size_t n = size_t(span_bytes) / sizeof(double2);
+        double2* x = dev_alloc<double2>(n);
+        double* out = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            out = dev_alloc<double>(1);
+            ED_HIP(hipMemsetAsync(x, 0, n * sizeof(double2), st));
+            ED_HIP(hipEventCreate(&e0));
+            ED_HIP(hipEventCreate(&e1));
+            const unsigned nwg = 8192;
+            const int iters = 256;
+            const uint32_t mask = uint32_t(n - 1);
+            auto run = [&] {
+                if (scattered)
+                    hipLaunchKernelGGL(line_probe_kernel<true>, dim3(nwg), dim3(BS), 0, st, x, mask, iters, out);
+                else
+                    hipLaunchKernelGGL(line_probe_kernel<false>, dim3(nwg), dim3(BS), 0, st, x, mask, iters, out);
+                ED_HIP(hipGetLastError());
+            };
+            run();
+            float best = 1e30f;
+            for (int r = 0; r < 3; ++r) {
+                ED_HIP(hipEventRecord(e0, st));
+                run();
+                ED_HIP(hipEventRecord(e1, st));
+                ED_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                ED_HIP(hipEventElapsedTime(&ms, e0, e1));
+                best = std::min(best, ms);
+            }
+            // wave-loads per launch x lines per wave-load (64 scattered, 8 for 1 KB in a row)
+            const double lines = double(nwg) * (BS / 64) * iters * (scattered ? 64.0 : 8.0);
+            *lines_per_s = lines / (double(best) * 1e-3);
+        } catch (...) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            dev_free(out);
+            dev_free(x);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        dev_free(out);
+        dev_free(x);
     });
 }
 

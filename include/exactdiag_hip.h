@@ -221,6 +221,14 @@ int ed_hmat_is_real(const ed_hmat* m);
 int ed_hmat_tune(ed_hmat* m, const double* d_x, double* d_y, int precision, int* tiles, int* unroll,
                  int* nontemporal, double* best_ms, void* stream);
 int ed_hmat_set_config(ed_hmat* m, int tiles, int unroll, int nontemporal);
+/* Read-rate probe (no counterpart in the reference; measurement only): the
+ * rate, in 128-byte lines per second, at which the device serves 16-byte
+ * loads from a buffer of span_bytes (a power of two in [1 MiB, 4 GiB]),
+ * scattered = 1: every lane of a wavefront from its own line at a hashed
+ * place (as the stored product's far gathers), 0: 64 consecutive elements per
+ * wavefront.  Best of three launches of 8.4 M wavefront loads each, timed with
+ * HIP events on stream; the buffer is taken from the device memory cache. */
+int ed_hmat_line_probe(int scattered, int64_t span_bytes, double* lines_per_s, void* stream);
 /* With a real precision (ED_PREC_R64, ED_PREC_R32) the start vector d_v0
  * (complex128) must be real: ED_ERR_ARG when its imaginary part exceeds
  * 1e-10 of its norm (rotate a global phase away first) or when the matrix is
