@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "ed_internal.h"
@@ -70,19 +71,22 @@ __global__ void __launch_bounds__(64) gemm_bf16_kernel(int M, int N, int K, cons
     }
 }
 
-// The same product on 128 x 128 tiles of C per 256-thread workgroup, staged through LDS.  Each wavefront
-// owns a 64 x 64 quarter (2 x 2 MFMA tiles).  Per 32-wide step of K the workgroup loads the A and B tiles
+// The same product on (64 TM) x (64 TN) tiles of C per 256-thread workgroup, staged through LDS.  Each
+// wavefront owns a (32 TM) x (32 TN) quarter (TM x TN MFMA tiles); a wider quarter reads fewer LDS operands
+// per MFMA (TM + TN reads for TM TN products).  Per 32-wide step of K the workgroup loads the A and B tiles
 // with 16-byte global loads of 8 consecutive elements along whichever dimension is contiguous in memory
 // (A_K: A's k stride is 1, else its m stride is 1; B_K: B's k stride is 1, else its n stride is 1) and stores
 // them in LDS as As[m][k] and Bs[n][k] (k contiguous, row pitch GLDK), so that every MFMA operand is one
-// 16-byte LDS read.  The next step's global loads are issued before the current step's MFMAs.  Chunks that
+// 16-byte LDS read.  The next step's global loads are issued before the current step's MFMAs and stored to
+// the second LDS buffer after them.  The wide tile is held to two wavefronts per SIMD (no accumulation
+// registers apart from the VGPRs), which it needs to beat the 128 x 128 tile.  Chunks that
 // cross the matrix edge are read element by element with zero fill; the host launches this kernel only when
 // the operands are 16-byte aligned and their other stride is a multiple of 8.
-constexpr int GBM = 128, GBN = 128, GBK = 32, GLDK = GBK + 8;
+constexpr int GBK = 32, GLDK = GBK + 8;
 // LDS offset of tile row r: pitch GLDK plus 8 elements (16 bytes) after every 8 rows, so that the 16 lanes that
 // store 8 rows apart land on 8 different banks instead of one (rows 8 apart at pitch 40 share a bank)
 __device__ inline int lrow(int r) { return r * GLDK + (r >> 3) * 8; }
-constexpr int GLDS = GBM * GLDK + (GBM / 8) * 8;
+__host__ __device__ constexpr int lds_rows(int rows) { return rows * GLDK + (rows / 8) * 8; }
 
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
@@ -94,91 +98,103 @@ __device__ inline u16x8 load8(const uint16_t* __restrict__ p, int n_in, bool ok)
     return v;
 }
 
-template <bool A_K, bool B_K>
-__global__ void __launch_bounds__(256) gemm_bf16_tiled_kernel(int M, int N, int K, const uint16_t* __restrict__ A,
+// One operand's tile of R rows (m for A, n for B) x GBK k per step, R / 64 chunks of 8 elements per thread.
+// K_CONTIG: chunk c = tid + 256 i holds 8 consecutive k of row c / 4 (k = 8 (c % 4)).  Otherwise chunks 2 g
+// and 2 g + 1 of thread t hold rows 128 g + 8 (t % 16) .. + 7 at k = 2 (t / 16) and 2 (t / 16) + 1 (16-byte
+// loads along the contiguous dimension); the stash writes each row's pair of k as one 4-byte word, so the
+// tile reaches LDS transposed (k contiguous) in eight writes per pair of chunks.
+template <int R, bool K_CONTIG>
+struct TileIO {
+    static constexpr int NC = R / 64;
+    __device__ static void fetch(u16x8 (&v)[NC], const uint16_t* __restrict__ X, int64_t s_row, int64_t s_k,
+                                 int row0, int rows, int k0, int K, int tid) {
+        for (int i = 0; i < NC; ++i) {
+            if constexpr (K_CONTIG) {
+                const int c = tid + 256 * i;
+                const int row = row0 + c / 4, k = k0 + 8 * (c % 4);
+                v[i] = load8(X + int64_t(row) * s_row + k, K - k, row < rows && k < K);
+            } else {
+                const int k = k0 + 2 * (tid / 16) + (i & 1), row = row0 + 128 * (i >> 1) + 8 * (tid % 16);
+                v[i] = load8(X + int64_t(k) * s_k + row, rows - row, k < K && row < rows);
+            }
+        }
+    }
+    __device__ static void stash(uint16_t* S, const u16x8 (&v)[NC], int tid) {
+        if constexpr (K_CONTIG) {
+            for (int i = 0; i < NC; ++i) {
+                const int c = tid + 256 * i;
+                *reinterpret_cast<u16x8*>(&S[lrow(c / 4) + 8 * (c % 4)]) = v[i];
+            }
+        } else {
+            const int k = 2 * (tid / 16);
+            for (int g = 0; g < NC / 2; ++g) {
+                const int r = 128 * g + 8 * (tid % 16);
+                for (int e = 0; e < 8; ++e)
+                    *reinterpret_cast<uint32_t*>(&S[lrow(r + e) + k]) =
+                        uint32_t(v[2 * g][e]) | (uint32_t(v[2 * g + 1][e]) << 16);
+            }
+        }
+    }
+};
+
+template <bool A_K, bool B_K, int TM, int TN>
+__global__ void __launch_bounds__(256, (TM * TN > 4 ? 2 : 1)) gemm_bf16_tiled_kernel(int M, int N, int K, const uint16_t* __restrict__ A,
                                                               int64_t sam, int64_t sak,
                                                               const uint16_t* __restrict__ B, int64_t sbk,
                                                               int64_t sbn, float* __restrict__ C, int64_t ldc,
                                                               int accumulate) {
-    __shared__ __align__(16) uint16_t As[GLDS];
-    __shared__ __align__(16) uint16_t Bs[GLDS];
+    constexpr int BM = 64 * TM, BN = 64 * TN;
+    using IA = TileIO<BM, A_K>;
+    using IB = TileIO<BN, B_K>;
+    // two LDS buffers: step s computes from buffer s % 2 while the tiles of step s + 1 are written to the
+    // other, so each step needs one barrier
+    __shared__ __align__(16) uint16_t As[2][lds_rows(BM)];
+    __shared__ __align__(16) uint16_t Bs[2][lds_rows(BN)];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
     const int r = lane & 31, half = lane >> 5;
-    f32x16 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
+    f32x16 acc[TM][TN];
+    for (int i = 0; i < TM; ++i)
+        for (int j = 0; j < TN; ++j)
             for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-    u16x8 ra[2], rb[2];
-    // chunk c of 512 per operand and step: k-contiguous tiles are 4 chunks per row of 32 k (row = c / 4,
-    // k = 8 (c % 4)); m- or n-contiguous tiles are 16 chunks per k (k = c / 16, row = 8 (c % 16))
-    // A k-contiguous operand: chunk c of 512 per step holds 8 consecutive k of row c / 4 (k = 8 (c % 4)),
-    // two chunks per thread.  Otherwise thread t holds rows 8 (t % 16) .. + 7 at the two k 2 (t / 16) and
-    // 2 (t / 16) + 1 (two 16-byte loads along the contiguous dimension) and stores each row's pair of k as
-    // one 4-byte word, so the tile reaches LDS transposed (k contiguous) in eight writes per thread.
+    u16x8 ra[IA::NC], rb[IB::NC];
     auto fetch = [&](int k0) {
-        for (int i = 0; i < 2; ++i) {
-            const int c = tid + 256 * i;
-            if constexpr (A_K) {
-                const int m = m0 + c / 4, k = k0 + 8 * (c % 4);
-                ra[i] = load8(A + int64_t(m) * sam + k, K - k, m < M && k < K);
-            } else {
-                const int k = k0 + 2 * (tid / 16) + i, m = m0 + 8 * (tid % 16);
-                ra[i] = load8(A + int64_t(k) * sak + m, M - m, k < K && m < M);
-            }
-            if constexpr (B_K) {
-                const int n = n0 + c / 4, k = k0 + 8 * (c % 4);
-                rb[i] = load8(B + int64_t(n) * sbn + k, K - k, n < N && k < K);
-            } else {
-                const int k = k0 + 2 * (tid / 16) + i, n = n0 + 8 * (tid % 16);
-                rb[i] = load8(B + int64_t(k) * sbk + n, N - n, k < K && n < N);
-            }
-        }
-    };
-    auto stash_pairs = [&](uint16_t* S, const u16x8 (&v)[2]) {
-        const int k = 2 * (tid / 16), r = 8 * (tid % 16);
-        for (int e = 0; e < 8; ++e)
-            *reinterpret_cast<uint32_t*>(&S[lrow(r + e) + k]) = uint32_t(v[0][e]) | (uint32_t(v[1][e]) << 16);
-    };
-    auto stash = [&]() {
-        if constexpr (A_K) {
-            for (int i = 0; i < 2; ++i) {
-                const int c = tid + 256 * i;
-                *reinterpret_cast<u16x8*>(&As[lrow(c / 4) + 8 * (c % 4)]) = ra[i];
-            }
-        } else {
-            stash_pairs(As, ra);
-        }
-        if constexpr (B_K) {
-            for (int i = 0; i < 2; ++i) {
-                const int c = tid + 256 * i;
-                *reinterpret_cast<u16x8*>(&Bs[lrow(c / 4) + 8 * (c % 4)]) = rb[i];
-            }
-        } else {
-            stash_pairs(Bs, rb);
-        }
+        IA::fetch(ra, A, sam, sak, m0, M, k0, K, tid);
+        IB::fetch(rb, B, sbn, sbk, n0, N, k0, K, tid);
     };
     fetch(0);
+    IA::stash(As[0], ra, tid);
+    IB::stash(Bs[0], rb, tid);
+    __syncthreads();
+    int cur = 0;
     for (int k0 = 0; k0 < K; k0 += GBK) {
-        stash();
-        __syncthreads();
-        if (k0 + GBK < K) fetch(k0 + GBK);
+        const bool more = k0 + GBK < K;
+        if (more) fetch(k0 + GBK);
+        const uint16_t* as = As[cur];
+        const uint16_t* bs = Bs[cur];
         for (int kk = 0; kk < GBK; kk += 16) {
-            bf16x8 a[2], b[2];
-            for (int t = 0; t < 2; ++t) {
+            bf16x8 a[TM], b[TN];
+            for (int t = 0; t < TM; ++t)
                 a[t] = __builtin_bit_cast(bf16x8,
-                                          *reinterpret_cast<const u16x8*>(&As[lrow(wm + 32 * t + r) + kk + 8 * half]));
+                                          *reinterpret_cast<const u16x8*>(&as[lrow(wm + 32 * t + r) + kk + 8 * half]));
+            for (int t = 0; t < TN; ++t)
                 b[t] = __builtin_bit_cast(bf16x8,
-                                          *reinterpret_cast<const u16x8*>(&Bs[lrow(wn + 32 * t + r) + kk + 8 * half]));
-            }
-            for (int i = 0; i < 2; ++i)
-                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                                          *reinterpret_cast<const u16x8*>(&bs[lrow(wn + 32 * t + r) + kk + 8 * half]));
+            for (int i = 0; i < TM; ++i)
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        // the other buffer was last read in the previous step, before the barrier that ended it
+        if (more) {
+            IA::stash(As[cur ^ 1], ra, tid);
+            IB::stash(Bs[cur ^ 1], rb, tid);
         }
         __syncthreads();
+        cur ^= 1;
     }
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < TM; ++i)
+        for (int j = 0; j < TN; ++j)
             for (int e = 0; e < 16; ++e) {
                 const int m = m0 + wm + 32 * i + 8 * (e >> 2) + 4 * half + (e & 3);
                 const int n = n0 + wn + 32 * j + r;
@@ -288,9 +304,9 @@ __global__ void reduce_double_kernel(const double* p, int n, double* out) {
 
 inline unsigned nblk(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
 
-// Which GEMM kernel a product uses: 1 = LDS-tiled (128 x 128 tiles, vector loads), 0 = one wavefront per
-// 32 x 32 tile.  ED_GEMM_KERNEL=0 or 1 forces one (for measurements); otherwise the tiled kernel whenever the
-// operands allow its 16-byte loads.
+// Which GEMM kernel a product uses: 1 = LDS-tiled with 128 x 128 tiles, 2 = LDS-tiled with 128 x 256 tiles,
+// 0 = one wavefront per 32 x 32 tile.  ED_GEMM_KERNEL forces one (for measurements); otherwise the tiled
+// kernel whenever the operands allow its 16-byte loads, its tile chosen by the size of C.
 int gemm_kernel_choice() {
     static const int v = [] {
         const char* e = std::getenv("ED_GEMM_KERNEL");
@@ -307,18 +323,28 @@ void gemm(hipStream_t st, int M, int N, int K, const uint16_t* A, int64_t sam, i
     const bool aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
     const int choice = gemm_kernel_choice();
     if (choice != 0 && aligned && (a_k || a_m) && (b_k || b_n)) {
-        const dim3 grid{unsigned((N + GBN - 1) / GBN), unsigned((M + GBM - 1) / GBM), 1u};
-        auto go = [&](auto kernel) {
+        // 128 x 256 tiles (wavefront quarters of 64 x 128) once the grid still has a few workgroups per CU with
+        // them, else 128 x 128
+        const bool wide = choice == 2 || (choice != 1 && int64_t(M) * N >= (int64_t(1) << 22));
+        auto go = [&](auto kernel, int bm, int bn) {
+            const dim3 grid{unsigned((N + bn - 1) / bn), unsigned((M + bm - 1) / bm), 1u};
             hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate);
         };
+        auto pick = [&](auto a_kc, auto b_kc) {
+            constexpr bool AK = decltype(a_kc)::value, BK = decltype(b_kc)::value;
+            if (wide)
+                go(gemm_bf16_tiled_kernel<AK, BK, 2, 4>, 128, 256);
+            else
+                go(gemm_bf16_tiled_kernel<AK, BK, 2, 2>, 128, 128);
+        };
         if (a_k && b_k)
-            go(gemm_bf16_tiled_kernel<true, true>);
+            pick(std::true_type{}, std::true_type{});
         else if (a_k)
-            go(gemm_bf16_tiled_kernel<true, false>);
+            pick(std::true_type{}, std::false_type{});
         else if (b_k)
-            go(gemm_bf16_tiled_kernel<false, true>);
+            pick(std::false_type{}, std::true_type{});
         else
-            go(gemm_bf16_tiled_kernel<false, false>);
+            pick(std::false_type{}, std::false_type{});
         return;
     }
     const dim3 grid{unsigned((N + 31) / 32), unsigned((M + 31) / 32), 1u};
