@@ -653,6 +653,15 @@ def run_form(form, args=None):
                              "traffic": traffic_from_profiles("mk_spmv", "multik_2x12_h2.md"),
                              "kernel": "mk_spmv_kernel, per launch (HIP events)", "alg_bytes_per_state": 28},
                 "form": res}
+    elif form == "gemm":
+        res = gemm_section()
+        big = res["products"]["square 8192"]
+        line = {"metric": "closure BF16 MFMA GEMM, 8192^3 (TFLOP/s)", "value": big["tflops"], "unit": "TFLOP/s",
+                "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 (fp32 accumulation)",
+                "data": "synthetic (random BF16 values in [2^-7, 1))",
+                "config": {"workload": "C[8192][8192] = A[8192][8192] B[8192][8192], A k-contiguous, B n-contiguous; "
+                                       "and the closure network's 256-row layers and one training step"},
+                "roofline": res["roofline"], "form": res}
     elif form == "batching":
         res = forms.batching_form(log=log)
         mk = res.get("multik", {})
@@ -739,10 +748,10 @@ def main():
     ap.add_argument("--allq-rungs", type=int, default=0,
                     help="with --form allq: the ladder's rungs (default: the headline's 12); others run the fast "
                          "and multi-momentum forms only")
-    ap.add_argument("--form", choices=["hv", "batching", "precision", "csr", "positions", "multik", "allq"],
+    ap.add_argument("--form", choices=["hv", "batching", "precision", "csr", "positions", "multik", "allq", "gemm"],
                     default="hv",
                     help="hv: the headline H·v + all-q line; allq: the all-q section alone; batching / precision / csr / "
-                         "positions / multik: their own line")
+                         "positions / multik / gemm: their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); several GPUs are not supported: "

@@ -83,10 +83,15 @@ __global__ void __launch_bounds__(64) gemm_bf16_kernel(int M, int N, int K, cons
 // cross the matrix edge are read element by element with zero fill; the host launches this kernel only when
 // the operands are 16-byte aligned and their other stride is a multiple of 8.
 constexpr int GBK = 32, GLDK = GBK + 8;
-// LDS offset of tile row r: pitch GLDK plus 8 elements (16 bytes) after every 8 rows, so that the 16 lanes that
-// store 8 rows apart land on 8 different banks instead of one (rows 8 apart at pitch 40 share a bank)
-__device__ inline int lrow(int r) { return r * GLDK + (r >> 3) * 8; }
-__host__ __device__ constexpr int lds_rows(int rows) { return rows * GLDK + (rows / 8) * 8; }
+// LDS offset of tile row r: pitch GLDK (80 bytes), and for a tile stashed as k-pairs (PAD) 8 more elements
+// (16 bytes) after every 16 rows.  Measured on gfx950 (cycles per wavefront instruction, 16-byte operand
+// reads of 32 rows x 2 chunks / 4-byte k-pair writes of rows 8 a + e, a = 0..15): pitch 80 alone 6.0 / 32.6,
+// with 16 bytes per 16 rows 8.9 / 4.5, with 16 bytes per 8 rows 14.0 / 4.4.  A tile written with 16-byte
+// stores (k-contiguous operand) has no pair writes and keeps the plain pitch.
+template <bool PAD>
+__device__ inline int lrow(int r) { return r * GLDK + (PAD ? (r >> 4) * 8 : 0); }
+template <bool PAD>
+__host__ __device__ constexpr int lds_rows(int rows) { return rows * GLDK + (PAD ? (rows / 16) * 8 : 0); }
 
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
@@ -106,6 +111,8 @@ __device__ inline u16x8 load8(const uint16_t* __restrict__ p, int n_in, bool ok)
 template <int R, bool K_CONTIG>
 struct TileIO {
     static constexpr int NC = R / 64;
+    static constexpr bool PAD = !K_CONTIG;
+    static constexpr int LDS = lds_rows<PAD>(R);
     __device__ static void fetch(u16x8 (&v)[NC], const uint16_t* __restrict__ X, int64_t s_row, int64_t s_k,
                                  int row0, int rows, int k0, int K, int tid) {
         for (int i = 0; i < NC; ++i) {
@@ -123,14 +130,14 @@ struct TileIO {
         if constexpr (K_CONTIG) {
             for (int i = 0; i < NC; ++i) {
                 const int c = tid + 256 * i;
-                *reinterpret_cast<u16x8*>(&S[lrow(c / 4) + 8 * (c % 4)]) = v[i];
+                *reinterpret_cast<u16x8*>(&S[lrow<PAD>(c / 4) + 8 * (c % 4)]) = v[i];
             }
         } else {
             const int k = 2 * (tid / 16);
             for (int g = 0; g < NC / 2; ++g) {
                 const int r = 128 * g + 8 * (tid % 16);
                 for (int e = 0; e < 8; ++e)
-                    *reinterpret_cast<uint32_t*>(&S[lrow(r + e) + k]) =
+                    *reinterpret_cast<uint32_t*>(&S[lrow<PAD>(r + e) + k]) =
                         uint32_t(v[2 * g][e]) | (uint32_t(v[2 * g + 1][e]) << 16);
             }
         }
@@ -148,8 +155,8 @@ __global__ void __launch_bounds__(256, (TM * TN > 4 ? 2 : 1)) gemm_bf16_tiled_ke
     using IB = TileIO<BN, B_K>;
     // two LDS buffers: step s computes from buffer s % 2 while the tiles of step s + 1 are written to the
     // other, so each step needs one barrier
-    __shared__ __align__(16) uint16_t As[2][lds_rows(BM)];
-    __shared__ __align__(16) uint16_t Bs[2][lds_rows(BN)];
+    __shared__ __align__(16) uint16_t As[2][IA::LDS];
+    __shared__ __align__(16) uint16_t Bs[2][IB::LDS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
@@ -176,11 +183,11 @@ __global__ void __launch_bounds__(256, (TM * TN > 4 ? 2 : 1)) gemm_bf16_tiled_ke
         for (int kk = 0; kk < GBK; kk += 16) {
             bf16x8 a[TM], b[TN];
             for (int t = 0; t < TM; ++t)
-                a[t] = __builtin_bit_cast(bf16x8,
-                                          *reinterpret_cast<const u16x8*>(&as[lrow(wm + 32 * t + r) + kk + 8 * half]));
+                a[t] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const u16x8*>(&as[lrow<IA::PAD>(wm + 32 * t + r) + kk + 8 * half]));
             for (int t = 0; t < TN; ++t)
-                b[t] = __builtin_bit_cast(bf16x8,
-                                          *reinterpret_cast<const u16x8*>(&bs[lrow(wn + 32 * t + r) + kk + 8 * half]));
+                b[t] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const u16x8*>(&bs[lrow<IB::PAD>(wn + 32 * t + r) + kk + 8 * half]));
             for (int i = 0; i < TM; ++i)
                 for (int j = 0; j < TN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
