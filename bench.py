@@ -427,8 +427,8 @@ def gemm_section():
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t0) / 50 * 1e3
     big = out["square 8192"]
-    return {"kernel": "gemm_bf16_tiled_kernel (128x256 tiles per workgroup for the square product, 128x128 for "
-                      "the network's layers; two LDS buffers, 16-byte loads, v_mfma_f32_32x32x16_bf16)",
+    return {"kernel": "gemm_bf16_tiled_kernel (256x256 tiles of eight wavefronts for the square product, 128x128 "
+                      "of four for the network's layers; two LDS buffers, 16-byte loads, v_mfma_f32_32x32x16_bf16)",
             "products": out, "train_step_ms_batch256": step_ms,
             "roofline": {"bound": "mfma", "achieved": big["tflops"], "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": big["frac"], "traffic": None, "shape": "8192^3"}}

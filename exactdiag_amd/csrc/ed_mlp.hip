@@ -71,15 +71,16 @@ __global__ void __launch_bounds__(64) gemm_bf16_kernel(int M, int N, int K, cons
     }
 }
 
-// The same product on (64 TM) x (64 TN) tiles of C per 256-thread workgroup, staged through LDS.  Each
-// wavefront owns a (32 TM) x (32 TN) quarter (TM x TN MFMA tiles); a wider quarter reads fewer LDS operands
-// per MFMA (TM + TN reads for TM TN products).  Per 32-wide step of K the workgroup loads the A and B tiles
+// The same product on (32 TM WM) x (32 TN WN) tiles of C per workgroup of WM x WN wavefronts, staged through
+// LDS.  Each wavefront owns a (32 TM) x (32 TN) piece (TM x TN MFMA tiles); a wider piece reads fewer LDS
+// operands per MFMA (TM + TN reads for TM TN products), and a larger workgroup tile writes fewer LDS tile
+// elements per MFMA.  Per 32-wide step of K the workgroup loads the A and B tiles
 // with 16-byte global loads of 8 consecutive elements along whichever dimension is contiguous in memory
 // (A_K: A's k stride is 1, else its m stride is 1; B_K: B's k stride is 1, else its n stride is 1) and stores
 // them in LDS as As[m][k] and Bs[n][k] (k contiguous, row pitch GLDK), so that every MFMA operand is one
 // 16-byte LDS read.  The next step's global loads are issued before the current step's MFMAs and stored to
-// the second LDS buffer after them.  The wide tile is held to two wavefronts per SIMD (no accumulation
-// registers apart from the VGPRs), which it needs to beat the 128 x 128 tile.  Chunks that
+// the second LDS buffer after them.  The 64 x 128 pieces are held to two wavefronts per SIMD (no
+// accumulation registers apart from the VGPRs), which they need to beat 64 x 64 pieces.  Chunks that
 // cross the matrix edge are read element by element with zero fill; the host launches this kernel only when
 // the operands are 16-byte aligned and their other stride is a multiple of 8.
 constexpr int GBK = 32, GLDK = GBK + 8;
@@ -108,20 +109,22 @@ __device__ inline u16x8 load8(const uint16_t* __restrict__ p, int n_in, bool ok)
 // and 2 g + 1 of thread t hold rows 128 g + 8 (t % 16) .. + 7 at k = 2 (t / 16) and 2 (t / 16) + 1 (16-byte
 // loads along the contiguous dimension); the stash writes each row's pair of k as one 4-byte word, so the
 // tile reaches LDS transposed (k contiguous) in eight writes per pair of chunks.
-template <int R, bool K_CONTIG>
+template <int R, bool K_CONTIG, int NT>
 struct TileIO {
-    static constexpr int NC = R / 64;
+    static constexpr int NC = 4 * R / NT;
     static constexpr bool PAD = !K_CONTIG;
     static constexpr int LDS = lds_rows<PAD>(R);
+    static_assert(NC >= 1 && (K_CONTIG || NC % 2 == 0), "tile too small for the thread count");
     __device__ static void fetch(u16x8 (&v)[NC], const uint16_t* __restrict__ X, int64_t s_row, int64_t s_k,
                                  int row0, int rows, int k0, int K, int tid) {
         for (int i = 0; i < NC; ++i) {
             if constexpr (K_CONTIG) {
-                const int c = tid + 256 * i;
+                const int c = tid + NT * i;
                 const int row = row0 + c / 4, k = k0 + 8 * (c % 4);
                 v[i] = load8(X + int64_t(row) * s_row + k, K - k, row < rows && k < K);
             } else {
-                const int k = k0 + 2 * (tid / 16) + (i & 1), row = row0 + 128 * (i >> 1) + 8 * (tid % 16);
+                const int k = k0 + 2 * ((tid / 16) % 16) + (i & 1);
+                const int row = row0 + 128 * (tid / 256 + (NT / 256) * (i >> 1)) + 8 * (tid % 16);
                 v[i] = load8(X + int64_t(k) * s_k + row, rows - row, k < K && row < rows);
             }
         }
@@ -129,13 +132,13 @@ struct TileIO {
     __device__ static void stash(uint16_t* S, const u16x8 (&v)[NC], int tid) {
         if constexpr (K_CONTIG) {
             for (int i = 0; i < NC; ++i) {
-                const int c = tid + 256 * i;
+                const int c = tid + NT * i;
                 *reinterpret_cast<u16x8*>(&S[lrow<PAD>(c / 4) + 8 * (c % 4)]) = v[i];
             }
         } else {
-            const int k = 2 * (tid / 16);
+            const int k = 2 * ((tid / 16) % 16);
             for (int g = 0; g < NC / 2; ++g) {
-                const int r = 128 * g + 8 * (tid % 16);
+                const int r = 128 * (tid / 256 + (NT / 256) * g) + 8 * (tid % 16);
                 for (int e = 0; e < 8; ++e)
                     *reinterpret_cast<uint32_t*>(&S[lrow<PAD>(r + e) + k]) =
                         uint32_t(v[2 * g][e]) | (uint32_t(v[2 * g + 1][e]) << 16);
@@ -144,22 +147,21 @@ struct TileIO {
     }
 };
 
-template <bool A_K, bool B_K, int TM, int TN>
-__global__ void __launch_bounds__(256, (TM * TN > 4 ? 2 : 1)) gemm_bf16_tiled_kernel(int M, int N, int K, const uint16_t* __restrict__ A,
-                                                              int64_t sam, int64_t sak,
-                                                              const uint16_t* __restrict__ B, int64_t sbk,
-                                                              int64_t sbn, float* __restrict__ C, int64_t ldc,
-                                                              int accumulate) {
-    constexpr int BM = 64 * TM, BN = 64 * TN;
-    using IA = TileIO<BM, A_K>;
-    using IB = TileIO<BN, B_K>;
+template <bool A_K, bool B_K, int TM, int TN, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN, (WM * WN == 4 && TM * TN > 4 ? 2 : 1))
+    gemm_bf16_tiled_kernel(int M, int N, int K, const uint16_t* __restrict__ A, int64_t sam, int64_t sak,
+                           const uint16_t* __restrict__ B, int64_t sbk, int64_t sbn, float* __restrict__ C,
+                           int64_t ldc, int accumulate) {
+    constexpr int NT = 64 * WM * WN, BM = 32 * TM * WM, BN = 32 * TN * WN;
+    using IA = TileIO<BM, A_K, NT>;
+    using IB = TileIO<BN, B_K, NT>;
     // two LDS buffers: step s computes from buffer s % 2 while the tiles of step s + 1 are written to the
     // other, so each step needs one barrier
     __shared__ __align__(16) uint16_t As[2][IA::LDS];
     __shared__ __align__(16) uint16_t Bs[2][IB::LDS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-    const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+    const int wm = (wave / WN) * 32 * TM, wn = (wave % WN) * 32 * TN;
     const int r = lane & 31, half = lane >> 5;
     f32x16 acc[TM][TN];
     for (int i = 0; i < TM; ++i)
@@ -311,8 +313,8 @@ __global__ void reduce_double_kernel(const double* p, int n, double* out) {
 
 inline unsigned nblk(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
 
-// Which GEMM kernel a product uses: 1 = LDS-tiled with 128 x 128 tiles, 2 = LDS-tiled with 128 x 256 tiles,
-// 0 = one wavefront per 32 x 32 tile.  ED_GEMM_KERNEL forces one (for measurements); otherwise the tiled
+// Which GEMM kernel a product uses: 1 = LDS-tiled with 128 x 128 tiles, 2 = 128 x 256 tiles, 3 = 256 x 256
+// tiles, 0 = one wavefront per 32 x 32 tile.  ED_GEMM_KERNEL forces one (for measurements); otherwise the tiled
 // kernel whenever the operands allow its 16-byte loads, its tile chosen by the size of C.
 int gemm_kernel_choice() {
     static const int v = [] {
@@ -330,19 +332,24 @@ void gemm(hipStream_t st, int M, int N, int K, const uint16_t* A, int64_t sam, i
     const bool aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
     const int choice = gemm_kernel_choice();
     if (choice != 0 && aligned && (a_k || a_m) && (b_k || b_n)) {
-        // 128 x 256 tiles (wavefront quarters of 64 x 128) once the grid still has a few workgroups per CU with
-        // them, else 128 x 128
-        const bool wide = choice == 2 || (choice != 1 && int64_t(M) * N >= (int64_t(1) << 22));
-        auto go = [&](auto kernel, int bm, int bn) {
+        // 256 x 256 tiles (eight wavefronts of 64 x 128) once C has 256 of them (one per CU), 128 x 256 (four
+        // wavefronts of 64 x 128) once it has 128, else 128 x 128; measured at 8192^3 with B n-contiguous: 787,
+        // 739 and 629 TFLOP/s
+        const int64_t mn = int64_t(M) * N;
+        const bool big = choice == 3 || (choice < 0 && mn >= (int64_t(1) << 24));
+        const bool wide = choice == 2 || (choice < 0 && mn >= (int64_t(1) << 22));
+        auto go = [&](auto kernel, int bm, int bn, int nt) {
             const dim3 grid{unsigned((N + bn - 1) / bn), unsigned((M + bm - 1) / bm), 1u};
-            hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate);
+            hipLaunchKernelGGL(kernel, grid, dim3(nt), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate);
         };
         auto pick = [&](auto a_kc, auto b_kc) {
             constexpr bool AK = decltype(a_kc)::value, BK = decltype(b_kc)::value;
-            if (wide)
-                go(gemm_bf16_tiled_kernel<AK, BK, 2, 4>, 128, 256);
+            if (big)
+                go(gemm_bf16_tiled_kernel<AK, BK, 2, 4, 4, 2>, 256, 256, 512);
+            else if (wide)
+                go(gemm_bf16_tiled_kernel<AK, BK, 2, 4, 2, 2>, 128, 256, 256);
             else
-                go(gemm_bf16_tiled_kernel<AK, BK, 2, 2>, 128, 128);
+                go(gemm_bf16_tiled_kernel<AK, BK, 2, 2, 2, 2>, 128, 128, 256);
         };
         if (a_k && b_k)
             pick(std::true_type{}, std::true_type{});
