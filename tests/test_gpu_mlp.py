@@ -47,12 +47,13 @@ def test_gemm_lane_layout_and_values(torch, shape):
     assert np.max(np.abs(Ct.cpu().numpy() - ref.T)) < 1e-4 * max(1.0, np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("shape", [(200, 136, 88), (264, 512, 1024), (128, 128, 32), (8, 520, 40), (2104, 2056, 72)])
+@pytest.mark.parametrize("shape", [(200, 136, 88), (264, 512, 1024), (128, 128, 32), (8, 520, 40), (2104, 2056, 72),
+                                   (4104, 4104, 40)])
 @pytest.mark.parametrize("a_kcontig,b_kcontig", [(True, False), (True, True), (False, False), (False, True)])
 def test_tiled_gemm_all_operand_layouts(torch, shape, a_kcontig, b_kcontig):
     """The LDS-tiled kernel (taken when a stride is 1 and the other a multiple of 8) for every layout of A and
-    B, with tiles that cross the matrix edges, against float64 products of the same BF16 values; the last
-    shape has C large enough for the 128 x 256 tiles."""
+    B, with tiles that cross the matrix edges, against float64 products of the same BF16 values; the last two
+    shapes have C large enough for the 128 x 256 and the 256 x 256 tiles."""
     from exactdiag_amd._lib import check, lib
     from oracle import mlp_oracle as mo
 
