@@ -61,11 +61,14 @@ def test_lanczos_coefficients_match_oracle(oracle_mod, torch):
     w = [1, 1, 0.4, 0.4]
     rng = np.random.default_rng(11)
     v0 = rng.normal(size=o.dim) + 1j * rng.normal(size=o.dim)
-    a, b = g.lanczos(w, torch.as_tensor(v0, device="cuda"), 40)
-    ao, bo = o.lanczos(w, v0, 40)
-    assert len(a) == len(ao) == 40
-    assert np.max(np.abs(a[:25] - ao[:25])) < 1e-10
-    assert np.max(np.abs(b[:25] - bo[:25])) < 1e-10
+    # every coefficient of 60 steps (measured difference 3e-14); later steps of this 1,386-state sector part
+    # ways by design of plain Lanczos: once orthogonality is lost, copies of converged Ritz values appear at steps
+    # that depend on the rounding of each implementation (the two runs differ by 0.6 from about step 60 on)
+    a, b = g.lanczos(w, torch.as_tensor(v0, device="cuda"), 60)
+    ao, bo = o.lanczos(w, v0, 60)
+    assert len(a) == len(ao) == 60
+    assert np.max(np.abs(a - ao)) < 1e-12
+    assert np.max(np.abs(b - bo)) < 1e-12
     assert b[0] == 0.0
 
 
